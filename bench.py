@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Throughput of the batched rope solver on one MI355X, in vertex-substeps/s.
+
+Workload (the headline configuration): 4096 ropes with vertex counts U{16..256} (seed 1), 0.015 m
+segments, per-segment EA / EI, contact groups of 4 with rope-rope contact, 8 kinematic SDF meshes
+(64^3 float32 grids) per environment on SE(3) random walks, ground plane, root program, 100 substeps
+of 1/6000 s per 1/60 s frame, 8 iterations. A step is one frame. Inputs (geometry, per-frame root
+programs and mesh targets) are resident on the GPU before the timed region.
+
+One JSON line: metric, value, ms_per_step, roofline of the persistent frame kernel (algorithmic bytes
+per launch over its HIP-event-timed duration on the solver stream), a VALU figure, and the CPU
+baseline (the float32 oracle on the host cores, a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "cosserat-rod-solver-cuda_amd")]
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+B_ALG = 136.0  # bytes per vertex-substep: the float32 state round trip (x, v, v_last, q, w, l, k, rest q, flags)
+
+
+def flops_per_vertex_substep(iters, pair_tests_per_vertex=0.0):
+    """Algorithmic flop estimate per vertex-substep: init + end ~60, per iteration position sweep ~200 and
+    orientation sweep ~250, rope-rope pair tests ~95 each."""
+    return 60.0 + iters * 450.0 + 95.0 * pair_tests_per_vertex
+
+
+def cpu_baseline(sample_ropes, threads):
+    import numpy as np
+
+    from oracle.pyoracle import OracleBatch
+    from ropesim import scenes
+    s = scenes.config3(n_ropes=sample_ropes, frames=1)
+    o = OracleBatch(s.positions, s.density, s.k_stretch, s.k_bend, s.max_h, s.num_itr, prec=32, offsets=s.offsets)
+    s.apply(o)
+    fi = s.frame_inputs(1)[0]
+    t0 = time.perf_counter()
+    s.step(o, fi)
+    dt = time.perf_counter() - t0
+    assert np.isfinite(o.positions()).all()
+    return {"value": s.num_vertices * s.substeps / dt, "unit": "vertex-substeps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"float32 oracle, {sample_ropes} ropes ({s.num_vertices} vertices, {sample_ropes // 4} "
+                      f"environments) of the same generator, 1 frame of {s.substeps} substeps, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ropes", type=int, default=4096)
+    ap.add_argument("--mode", type=int, default=1, help="0: per-substep kernels, 1: persistent frame kernel")
+    ap.add_argument("--variant", type=int, default=-1, help="frame-kernel variant (-1: autotune)")
+    ap.add_argument("--cpu-sample-ropes", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py measures one GPU: the rope batch runs on a single device (environments would shard "
+                 "across GPUs, see DESIGN.md); run with --gpus 1")
+
+    import numpy as np
+    import torch
+
+    import ropesim
+    from ropesim import scenes
+
+    t_setup = time.perf_counter()
+    K, W = args.steps, args.warmup
+    s = scenes.config3(n_ropes=args.ropes, frames=K + W)
+    b = s.build(ropesim)
+    fis = s.frame_inputs(K + W)
+    dev = [{k: torch.as_tensor(v, device="cuda") for k, v in fi.items() if v is not None} for fi in fis]
+    torch.cuda.synchronize()
+    tuned_ms = None
+    b.set_dispatch(args.mode, args.variant if args.mode == 1 else -1)
+    if args.mode == 1 and args.variant < 0:
+        tuned_ms = b.autotune(s.dt, s.substeps, True)
+    setup_s = time.perf_counter() - t_setup
+
+    for k in range(W):
+        s.step(b, dev[k], lambda a: a)
+    torch.cuda.synchronize()
+    b.set_kernel_timing(True)
+    b.kernel_time()
+    t0 = time.perf_counter()
+    for k in range(W, W + K):
+        s.step(b, dev[k], lambda a: a)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms, launches = b.kernel_time()
+    b.set_kernel_timing(False)
+    x = torch.as_tensor(b.positions(), device="cuda")
+    assert bool(torch.isfinite(x).all()), "non-finite state"
+
+    units = s.num_vertices * s.substeps          # vertex-substeps per frame
+    value = units * K / wall
+    line = {
+        "metric": "vertex-substeps/s (4096 uneven ropes, contact on)",
+        "value": value,
+        "unit": "vertex-substeps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * wall / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"config3: {args.ropes} uneven ropes (V~U{{16..256}}, sum V={s.num_vertices}), "
+                               f"groups of 4 with rope-rope contact, 8 SDF meshes/env (64^3), plane, root program, "
+                               f"{s.substeps} substeps x {s.num_itr} iterations per 1/60 s frame",
+                   "ropes": args.ropes, "vertices": s.num_vertices, "substeps": s.substeps, "iterations": s.num_itr,
+                   "dispatch": list(b.dispatch), "autotune_ms": tuned_ms, "setup_s": round(setup_s, 1)},
+    }
+    if launches > 0:
+        per_launch_s = kern_ms / launches * 1e-3
+        achieved = B_ALG * units / per_launch_s / 1e9
+        line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                            "kernel": "k_cell (persistent frame kernel)", "kernel_ms": kern_ms / launches}
+        fl = flops_per_vertex_substep(s.num_itr)
+        tf = fl * units / per_launch_s / 1e12
+        line["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP32_PEAK_TFLOPS,
+                        "flop_per_vertex_substep": fl, "note": "solver flops only; rope-rope pair tests excluded"}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_ropes, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,412 @@
+// Per-vertex device routines of the substep, shared by the per-substep kernels and the persistent
+// frame kernel so both run exactly the same arithmetic. Each cites the reference code whose
+// behaviour it reproduces (paths relative to the reference repository; YB =
+// third_party/yarnball/KittenEngine/YarnBall).
+#pragma once
+
+#include "rs_math.hpp"
+#include "rs_types.hpp"
+
+namespace rs {
+
+__device__ __forceinline__ f3 ld3(const float4& a) { return mk3(a.x, a.y, a.z); }
+__device__ __forceinline__ q4 ldq(const float4& a) { return q4{a.x, a.y, a.z, a.w}; }
+__device__ __forceinline__ float4 st3(f3 a) { return make_float4(a.x, a.y, a.z, 0.f); }
+__device__ __forceinline__ float4 stq(q4 a) { return make_float4(a.x, a.y, a.z, a.w); }
+__device__ __forceinline__ uint32_t flags_of(const float4& prm) { return __float_as_uint(prm.w); }
+
+// World AABB of the object-frame lattice box under a pose (YB/sim/contact.cuh:79-87).
+__device__ __forceinline__ void world_box(const Grid& g, f3 pos, q4 q, f3& lo, f3& hi) {
+    lo = mk3(INFINITY, INFINITY, INFINITY);
+    hi = mk3(-INFINITY, -INFINITY, -INFINITY);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const f3 corner = mk3((c & 1) ? g.hx : g.ox, (c & 2) ? g.hy : g.oy, (c & 4) ? g.hz : g.oz);
+        const f3 w = qrot(q, corner) + pos;
+        lo = vmin(lo, w);
+        hi = vmax(hi, w);
+    }
+}
+
+// Pose of a kinematic mesh at the end of substep `sub` of `nSub` (YB/sim/iteration.cu:12-33): constant
+// linear and angular velocity over the frame; a target equal to the current pose stays exactly static.
+__device__ __forceinline__ Pose substep_pose(const Motion& M, const Grid& g, int sub, int nSub, float h) {
+    const float a = float(sub + 1) / float(nSub);
+    const float invDt = 1.f / (float(nSub) * h);
+    const f3 rot = mk3(M.rx, M.ry, M.rz), dp = mk3(M.dpx, M.dpy, M.dpz), p0 = mk3(M.p0x, M.p0y, M.p0z);
+    const q4 q0{M.q0x, M.q0y, M.q0z, M.q0w};
+    const float th = len(rot);
+    f3 pos, vel, om;
+    q4 q;
+    if (th > 0.f || dp.x != 0.f || dp.y != 0.f || dp.z != 0.f) {
+        pos = p0 + a * dp;
+        q4 dq{0.f, 0.f, 0.f, 1.f};
+        if (th > 0.f) {
+            const float half = (a * th) * 0.5f;
+            dq = mkq(sinf(half) * (rot / th), cosf(half));
+        }
+        q = qnormalize(qmul(dq, q0));
+        vel = dp * invDt;
+        om = rot * invDt;
+    } else {
+        pos = p0;
+        q = q0;
+        vel = mk3(0.f, 0.f, 0.f);
+        om = vel;
+    }
+    f3 lo, hi;
+    world_box(g, pos, q, lo, hi);
+    return Pose{pos.x, pos.y, pos.z, q.x, q.y, q.z, q.w, vel.x, vel.y, vel.z, om.x, om.y, om.z,
+                lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, 0.f};
+}
+
+__device__ __forceinline__ Pose load_pose(const Pose* p) {
+    const float4* s = reinterpret_cast<const float4*>(p);
+    const float4 a = s[0], b = s[1], c = s[2], d = s[3], e = s[4];
+    return Pose{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w, e.x, e.y, e.z, e.w};
+}
+__device__ __forceinline__ void store_pose(Pose* p, const Pose& P) {
+    float4* s = reinterpret_cast<float4*>(p);
+    s[0] = make_float4(P.px, P.py, P.pz, P.qx);
+    s[1] = make_float4(P.qy, P.qz, P.qw, P.vx);
+    s[2] = make_float4(P.vy, P.vz, P.wx, P.wy);
+    s[3] = make_float4(P.wz, P.lox, P.loy, P.loz);
+    s[4] = make_float4(P.hix, P.hiy, P.hiz, 0.f);
+}
+
+// Penalty + regularised Coulomb contact at one surface (YB/sim/contact.cuh:27-46).
+__device__ __forceinline__ void surface_contact(const Params& P, f3 n, float gap, f3 d, float k, float muS, float muK,
+                                                f3& f, s3& H) {
+    const float eps = P.cVSlip;
+    const float N = -k * gap;
+    f = f + N * n;
+    s3 nn = souter(n);
+    sadd(H, sscale(nn, k));
+    const f3 ut = d - dot(n, d) * n;
+    const float ul = len(ut);
+    const float vt = ul / P.h;
+    float kt;
+    if (vt <= eps) {
+        kt = muS * N / (eps * P.h);
+    } else {
+        const float s = 1.f - fminf(1.f, vt / eps - 1.f);
+        kt = (s * muS + (1.f - s) * muK) * N / ul;
+    }
+    f = f - kt * ut;
+    sadd_diag(nn, -1.f);
+    ssub(H, sscale(nn, kt));
+}
+
+// Trilinear SDF sample and the exact gradient of the interpolant (YB/sim/contact.cuh:57-76).
+__device__ __forceinline__ bool sdf_sample(const Grid& g, f3 p, float& phi, f3& grad) {
+    const f3 o = mk3(g.ox, g.oy, g.oz);
+    const f3 u = (p - o) * g.invH;
+    if (!(u.x >= 0.f && u.y >= 0.f && u.z >= 0.f && u.x < float(g.nx - 1) && u.y < float(g.ny - 1) &&
+          u.z < float(g.nz - 1)))
+        return false;
+    const int i = (int)u.x, j = (int)u.y, k = (int)u.z;
+    const f3 fr = (p - (o + mk3(float(i), float(j), float(k)) * g.h)) * g.invH;
+    const float fx = fminf(fmaxf(fr.x, 0.f), 1.f), fy = fminf(fmaxf(fr.y, 0.f), 1.f), fz = fminf(fmaxf(fr.z, 0.f), 1.f);
+    const int sy = g.nz, sx = g.ny * g.nz;
+    const float* c = g.v + (size_t)i * sx + (size_t)j * sy + k;
+    const float c000 = __ldg(c), c001 = __ldg(c + 1), c010 = __ldg(c + sy), c011 = __ldg(c + sy + 1);
+    const float c100 = __ldg(c + sx), c101 = __ldg(c + sx + 1), c110 = __ldg(c + sx + sy), c111 = __ldg(c + sx + sy + 1);
+    const float dx00 = c100 - c000, dx01 = c101 - c001, dx10 = c110 - c010, dx11 = c111 - c011;
+    const float c00 = c000 + fx * dx00, c01 = c001 + fx * dx01, c10 = c010 + fx * dx10, c11 = c011 + fx * dx11;
+    const float dy0 = c10 - c00, dy1 = c11 - c01;
+    const float c0 = c00 + fy * dy0, c1 = c01 + fy * dy1;
+    phi = c0 + fz * (c1 - c0);
+    const float gx0 = dx00 + fy * (dx10 - dx00), gx1 = dx01 + fy * (dx11 - dx01);
+    grad = mk3(gx0 + fz * (gx1 - gx0), dy0 + fz * (dy1 - dy0), c1 - c0) * g.invH;
+    return true;
+}
+
+// Contact of a vertex with one posed mesh at the probe (pos - P.pos) + dx (YB/sim/contact.cuh:94-110).
+__device__ __forceinline__ bool sdf_contact(const Params& Pm, const Grid& g, const Pose& P, f3 p, f3 d, f3& n,
+                                            float& gap, f3& dRel) {
+    const q4 q{P.qx, P.qy, P.qz, P.qw};
+    const f3 pp = mk3(P.px, P.py, P.pz);
+    float phi;
+    f3 gr;
+    if (!sdf_sample(g, qrot(qconj(q), (p - pp) + d), phi, gr)) return false;
+    gap = phi - Pm.cRadius;
+    if (gap >= 0.f) return false;
+    const float gl = len(gr);
+    if (!(gl > 0.f)) return false;
+    n = qrot(q, gr / gl);
+    dRel = d;
+    if (Pm.sdfMoving) dRel = dRel - Pm.h * (mk3(P.vx, P.vy, P.vz) + cross(mk3(P.wx, P.wy, P.wz), p - pp));
+    return true;
+}
+
+// Broad-phase candidates are kept packed: 6 bits per mesh id, up to 8 ids in 64 bits.
+__device__ __forceinline__ int cand_get(uint64_t packed, int c) { return (int)((packed >> (6 * c)) & 63u); }
+
+// Planes, then candidate meshes (YB/sim/contact.cuh:141-163). `ncand` < 0: every mesh.
+__device__ __forceinline__ void surfaces_at(const Params& P, const Grid* grids, const Pose* envPoses, int ncand,
+                                            uint64_t cand, f3 p, f3 d, float k, f3& f, s3& H) {
+    for (int pl = 0; pl < P.numPlanes; pl++) {
+        const f3 n = mk3(P.planes[pl][0], P.planes[pl][1], P.planes[pl][2]);
+        const float gap = ((dot(n, p) - P.planes[pl][3]) + dot(n, d)) - P.cRadius;
+        if (gap >= 0.f) continue;
+        surface_contact(P, n, gap, d, k, P.cMuS, P.cMuK, f, H);
+    }
+    if (P.numSdf > 0) {
+        const int cnt = ncand < 0 ? P.numSdf : ncand;
+        for (int c = 0; c < cnt; c++) {
+            const int m = ncand < 0 ? c : cand_get(cand, c);
+            const Grid g = grids[m];
+            const Pose Ps = load_pose(envPoses + m);
+            f3 n, dRel;
+            float gap;
+            if (!sdf_contact(P, g, Ps, p, d, n, gap, dRel)) continue;
+            surface_contact(P, n, gap, dRel, k, g.muS, g.muK, f, H);
+        }
+    }
+}
+
+// Force readout at the end of a substep (YB/sim/contact.cuh:172-204): force on the vertex, and the wrench
+// the vertex applies to each mesh about its origin accumulated with float atomics.
+__device__ __forceinline__ f3 surface_readout(const Params& P, const Grid* grids, const Pose* envPoses, float* envWrench,
+                                              int ncand, uint64_t cand, f3 p, f3 d, float k) {
+    s3 H = szero();
+    f3 fv = mk3(0.f, 0.f, 0.f);
+    for (int pl = 0; pl < P.numPlanes; pl++) {
+        const f3 n = mk3(P.planes[pl][0], P.planes[pl][1], P.planes[pl][2]);
+        const float gap = ((dot(n, p) - P.planes[pl][3]) + dot(n, d)) - P.cRadius;
+        if (gap >= 0.f) continue;
+        surface_contact(P, n, gap, d, k, P.cMuS, P.cMuK, fv, H);
+    }
+    if (P.numSdf > 0) {
+        const int cnt = ncand < 0 ? P.numSdf : ncand;
+        for (int c = 0; c < cnt; c++) {
+            const int m = ncand < 0 ? c : cand_get(cand, c);
+            const Grid g = grids[m];
+            const Pose Ps = load_pose(envPoses + m);
+            f3 n, dRel;
+            float gap;
+            if (!sdf_contact(P, g, Ps, p, d, n, gap, dRel)) continue;
+            f3 fm = mk3(0.f, 0.f, 0.f);
+            surface_contact(P, n, gap, dRel, k, g.muS, g.muK, fm, H);
+            fv = fv + fm;
+            const f3 r = ((p - mk3(Ps.px, Ps.py, Ps.pz)) + d) - P.cRadius * n;
+            const f3 tq = cross(r, -fm);
+            float* w = envWrench + m * 6;
+            atomicAdd(w + 0, -fm.x);
+            atomicAdd(w + 1, -fm.y);
+            atomicAdd(w + 2, -fm.z);
+            atomicAdd(w + 3, tq.x);
+            atomicAdd(w + 4, tq.y);
+            atomicAdd(w + 5, tq.z);
+        }
+    }
+    return fv;
+}
+
+// Candidate meshes of a vertex for the next substep (YB/sim/contact.cuh:121-136). Returns the count
+// (-1 on overflow) and the packed ids.
+__device__ __forceinline__ int build_candidates(const Params& P, const Pose* envPoses, f3 p, float padv, uint64_t& packed) {
+    int n = 0;
+    packed = 0;
+    for (int m = 0; m < P.numSdf; m++) {
+        const Pose Ps = load_pose(envPoses + m);
+        const float pm = padv + (P.sdfMoving ? P.h * (len(mk3(Ps.vx, Ps.vy, Ps.vz)) +
+                                                      len(mk3(Ps.wx, Ps.wy, Ps.wz)) * 0.5f *
+                                                          len(mk3(Ps.hix, Ps.hiy, Ps.hiz) - mk3(Ps.lox, Ps.loy, Ps.loz)))
+                                             : 0.f);
+        if (p.x >= Ps.lox - pm && p.x <= Ps.hix + pm && p.y >= Ps.loy - pm && p.y <= Ps.hiy + pm &&
+            p.z >= Ps.loz - pm && p.z <= Ps.hiz + pm) {
+            if (n < MAX_CAND) packed |= (uint64_t)m << (6 * n);
+            n++;
+        }
+    }
+    return n <= MAX_CAND ? n : -1;
+}
+
+// initItr for one vertex (YB/sim/iteration.cu:34-69): displacement guess, inertial target y, padding.
+__device__ __forceinline__ void init_vertex(const Params& P, float invMass, f3 v, f3 vl, f3& d, f3& y, float& padding) {
+    const f3 g = mk3(P.gx, P.gy, P.gz);
+    d = P.h * v;
+    y = d + (P.h * P.h) * g;
+    padding = P.scaledDetR + len(y);
+    if (invMass != 0.f) {
+        const float g2 = len2(g);
+        if (g2 > 0.f) {
+            const f3 a = (v - vl) / P.lastH;
+            const float s = clampf(dot(a, g) / g2, 0.f, 1.f);
+            d = d + (P.h * P.h * s) * g;
+        }
+    }
+}
+
+// One rope-rope barrier + friction term of segment (i, i+1) against segment (j, j+1)
+// (YB/sim/cosserat.cu:99-158).
+__device__ __forceinline__ void collision_term(const Params& P, float dmp, f3 p0, f3 p1, f3 d0, f3 p1dx, f3 b0, f3 b1,
+                                               f3 db0, f3 db1, f3& f, s3& H, f3& f2, s3& H2) {
+    const float invb = 1.f / P.barrier;
+    const float rad2 = 2.f * P.radius;
+    const float kCol = P.kCollision * invb;
+    float u, v;
+    seg_closest(mk3(0.f, 0.f, 0.f), (p1 - p0) + (p1dx - d0), (b0 - p0) + (db0 - d0), (b1 - p0) + (db1 - d0), u, v);
+    const f3 dpos = mix(p0, p1, u) - mix(b0, b1, v);
+    const f3 ddpos = mix(d0, p1dx, u) - mix(db0, db1, v);
+    f3 normal = dpos + ddpos;
+    float dd = len(normal);
+    if (!(dd > 0.f)) return;
+    normal = normal * (1.f / dd);
+    const float wy = u;
+    const float wx = 1.f - u;
+    dd = dd - rad2;
+    dd = dd * invb;
+    if (dd > 1.f) return;
+    dd = fmaxf(dd, 1e-3f);
+    const float invd = 1.f / dd;
+    const float logd = logf(dd);
+    float dH = ((-3.f + (2.f + invd) * invd) - 2.f * logd) * kCol * invb;
+    const float ff = -(1.f - dd) * ((dd - 1.f) + 2.f * dd * logd) * invd * kCol;
+    f = f + (ff * wx - dmp * dH * wx * wx * dot(normal, d0)) * normal;
+    f2 = f2 + (ff * wy - dmp * dH * wy * wy * dot(normal, p1dx)) * normal;
+    dH = dH * (1.f + dmp);
+    s3 op = souter(normal);
+    sadd(H, sscale(op, dH * wx * wx));
+    sadd(H2, sscale(op, dH * wy * wy));
+    const f3 uu = ddpos - dot(normal, ddpos) * normal;
+    const float ul = len(uu);
+    if (ul > 0.f) {
+        const float f1 = fminf(P.kFriction, P.fricMu * ff / ul);
+        sadd_diag(op, -1.f);
+        f = f - f1 * wx * uu;
+        ssub(H, sscale(op, wx * wx * f1));
+        f2 = f2 - f1 * wy * uu;
+        ssub(H2, sscale(op, wy * wy * f1));
+    }
+}
+
+// Inertia + surface terms of vertex i and the stretch term of segment (i, i+1)
+// (YB/sim/cosserat.cu:41-88). Rope-rope terms are added by the caller.
+__device__ __forceinline__ void vertex_terms(const Params& P, const Grid* grids, const Pose* envPoses, int ncand,
+                                             uint64_t cand, float invMass, float lRest, float kStretch, uint32_t fl,
+                                             f3 p0, f3 d0, f3 y, q4 qi, f3 p1, f3 p1dx, f3& f, s3& H, f3& f2, s3& H2) {
+    const float h = P.h;
+    const float dmp = P.damping / h;
+    f = mk3(0.f, 0.f, 0.f);
+    H = szero();
+    f2 = f;
+    H2 = H;
+    if (invMass != 0.f) {
+        H = sdiag(1.f / (invMass * h * h));
+        f = (1.f / (h * h * invMass)) * (y - d0);
+        if (P.numPlanes > 0 || P.numSdf > 0) {
+            const float kap = P.cKappa / h;
+            surfaces_at(P, grids, envPoses, ncand, cand, p0, d0, kap * kap / invMass, f, H);
+        }
+    }
+    if (fl & HAS_NEXT) {
+        const float invl = 1.f / lRest;
+        const f3 c = ((p1 - p0) + (p1dx - d0)) * invl - qrot(qi, mk3(1.f, 0.f, 0.f));
+        const float k = kStretch * invl;
+        float dd = k * invl;
+        f = f + (k * c - (dmp * dd) * d0);
+        f2 = f2 + (-k * c - (dmp * dd) * p1dx);
+        dd = dd * (1.f + dmp);
+        sadd_diag(H, dd);
+        sadd_diag(H2, dd);
+    }
+}
+
+// Local Newton update of a free vertex with the optional step clamp (YB/sim/cosserat.cu:206-215).
+__device__ __forceinline__ f3 solve_update(const Params& P, f3 d, const s3& H, f3 f, bool limit, float stepLimit) {
+    d = d + P.accel * ssolve(H, f);
+    if (limit) {
+        const float l = len(d);
+        if (l > stepLimit && l > 0.f) d = d * (stepLimit / l);
+    }
+    return d;
+}
+
+// Closed-form quasi-static frame update of segment i (YB/sim/cosserat.cu:240-272).
+__device__ __forceinline__ q4 orient_update(uint32_t fl, float lRest, float kStretch, f3 p0, f3 p1, f3 d, f3 p1dx,
+                                            q4 qPrev, q4 q0, q4 qNext, q4 qrPrev, q4 qr) {
+    f3 e = ((p1 - p0) + (p1dx - d)) / lRest;
+    e = e * (-2.f * kStretch);
+    q4 b{0.f, 0.f, 0.f, 0.f};
+    if (fl & HAS_PREV) {
+        const float s = qdot(qmul(qconj(qPrev), q0), qrPrev) > 0.f ? 1.f : -1.f;
+        b = qadd(b, qscale(s, qmul(qPrev, qrPrev)));
+    }
+    if (fl & HAS_NEXT_ORIENT) {
+        const float s = qdot(qmul(qconj(q0), qNext), qr) > 0.f ? 1.f : -1.f;
+        b = qadd(b, qscale(s, qmul(qNext, qconj(qr))));
+    }
+    const float lambda = len(e) + qlen(b);
+    const q4 prod = qmul(qmul(mkq(e, 0.f), b), q4{1.f, 0.f, 0.f, 0.f});
+    return qnormalize(qadd(prod, qscale(lambda, b)));
+}
+
+// Velocity update with quadratic and linear drag (YB/sim/iteration.cu:85-112). `t` is the segment
+// tangent (the tip uses its one real segment).
+__device__ __forceinline__ f3 end_velocity(const Params& P, f3 d, f3 t) {
+    const float invH = 1.f / P.h;
+    f3 v = d * invH;
+    const float kn = P.dragQuad;
+    if (kn > 0.f) {
+        f3 vt = t * dot(t, v);
+        f3 vn = v - vt;
+        vn = vn / (1.f + kn * len(vn) * P.h);
+        vt = vt / (1.f + P.dragQuadT * len(vt) * P.h);
+        v = vn + vt;
+    }
+    return v * (1.f - P.drag * P.h);
+}
+
+// SLERP of the root program (csrc/batch.cu:106-114).
+__device__ __forceinline__ q4 slerp4(q4 a, q4 b, float t) {
+    float d = qdot(a, b);
+    if (d < 0.f) {
+        b = q4{-b.x, -b.y, -b.z, -b.w};
+        d = -d;
+    }
+    if (d > 0.9995f) {
+        const q4 m{a.x * (1.f - t) + b.x * t, a.y * (1.f - t) + b.y * t, a.z * (1.f - t) + b.z * t,
+                   a.w * (1.f - t) + b.w * t};
+        return qnormalize(m);
+    }
+    const float th = acosf(fminf(d, 1.f));
+    const float s = sinf(th);
+    return qadd(qscale(sinf((1.f - t) * th) / s, a), qscale(sinf(t * th) / s, b));
+}
+
+// Root of rope e at substep s of the second-order motion program (csrc/batch.cu:122-159).
+__device__ __forceinline__ void root_program(const Drive& D, int e, int s, int nSub, float h, f3& x, f3& v, q4& q,
+                                             bool& setQ) {
+    const f3 P0 = mk3(D.p0[3 * e], D.p0[3 * e + 1], D.p0[3 * e + 2]);
+    const f3 V0 = mk3(D.v0[3 * e], D.v0[3 * e + 1], D.v0[3 * e + 2]);
+    const f3 P1 = mk3(D.p1[3 * e], D.p1[3 * e + 1], D.p1[3 * e + 2]);
+    const float dt = D.dt;
+    const f3 A = (2.f / (dt * dt)) * ((P1 - P0) - V0 * dt);
+    const float tau = float(s) * h;
+    x = (P0 + V0 * tau) + (0.5f * tau * tau) * A;
+    v = V0 + (tau + 0.5f * h) * A;
+    setQ = D.hasQ0 != 0;
+    if (setQ) {
+        const q4 qa = qunit_or_identity(q4{D.q0[4 * e], D.q0[4 * e + 1], D.q0[4 * e + 2], D.q0[4 * e + 3]});
+        q4 r = qa;
+        if (D.hasQ1) {
+            const q4 qb = qunit_or_identity(q4{D.q1[4 * e], D.q1[4 * e + 1], D.q1[4 * e + 2], D.q1[4 * e + 3]});
+            r = slerp4(qa, qb, float(s + 1) / float(nSub));
+        }
+        q = qunit_or_identity(r);
+    }
+}
+
+// Exact rope-rope candidate test of segment (a0, a0 + a1) against (b0, b1) relative to a0, radius r
+// (csrc/batch.cu:38-42). Returns the squared distance in `d2` for the step limit.
+__device__ __forceinline__ bool pair_hit(f3 a1, f3 b0, f3 b1, float r, float& d2) {
+    float u, v;
+    seg_closest(mk3(0.f, 0.f, 0.f), a1, b0, b1, u, v);
+    const f3 d = u * a1 - mix(b0, b1, v);
+    d2 = len2(d);
+    return d2 < r * r;
+}
+
+}  // namespace rs

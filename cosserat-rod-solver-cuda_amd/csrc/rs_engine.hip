@@ -1,0 +1,1312 @@
+// Host orchestration of the rope batch behind the C ABI (include/ropesim_hip.h): construction,
+// device buffers, the mesh pose state machine, frame dispatch (per-substep kernels or the persistent
+// frame kernel), validation and error reporting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ropesim_hip.h"
+#include "rs_kernels.hpp"
+#include "rs_math.hpp"
+#include "rs_types.hpp"
+
+using namespace rs;
+
+namespace {
+
+thread_local std::string g_msg;
+
+struct Invalid : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct DeviceErr : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct SolverErr : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hcheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw DeviceErr(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T> T* dalloc(size_t n, const char* what) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    hcheck(hipMalloc(&p, n * sizeof(T)), what);
+    return static_cast<T*>(p);
+}
+
+template <class F> int guarded(F&& fn) {
+    try {
+        fn();
+        return RS_OK;
+    } catch (const Invalid& e) {
+        g_msg = e.what();
+        return RS_INVALID;
+    } catch (const std::invalid_argument& e) {
+        g_msg = e.what();
+        return RS_INVALID;
+    } catch (const SolverErr& e) {
+        g_msg = e.what();
+        return RS_SOLVER;
+    } catch (const DeviceErr& e) {
+        g_msg = e.what();
+        return RS_DEVICE;
+    } catch (const std::exception& e) {
+        g_msg = e.what();
+        return RS_DEVICE;
+    }
+}
+
+float4 f4(f3 a, float w = 0.f) { return make_float4(a.x, a.y, a.z, w); }
+float4 f4q(q4 a) { return make_float4(a.x, a.y, a.z, a.w); }
+
+int device_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 256;
+    return n;
+}
+size_t device_max_lds() {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return 65536;
+    return std::max<size_t>(p.sharedMemPerBlock, (size_t)p.maxSharedMemoryPerMultiProcessor > 0
+                                                     ? std::min<size_t>(p.maxSharedMemoryPerMultiProcessor, 163840)
+                                                     : 65536);
+}
+
+}  // namespace
+
+struct rs_batch {
+    std::mutex mtx;
+    int N = 0, nv = 0;
+    std::vector<int> roff, ropeOf;
+    // host mirrors (authoritative before the first step; refreshed on demand afterwards)
+    std::vector<float4> hx, hq, hprm, hqrest, hv;
+    std::vector<float4> initX, initQ;
+    // knobs (defaults: YB/YarnBall.cu:14-45)
+    float gx = 0.f, gy = -9.8f, gz = 0.f;
+    float drag = 0.2f, damping = 1e-6f, dragQuad = 0.f, dragQuadT = 0.f, accel = 1.f;
+    int numItr = 8;
+    float maxH = 1e-3f, h = 1e-3f, lastH = 0.f, time = 0.f;
+    int numPlanes = 0;
+    float planes[8][4] = {};
+    float cRadius = 0.f, cKappa = 2.f, cMuS = 0.f, cMuK = 0.f, cVSlip = 1e-3f;
+    bool ropeContact = false;
+    int groupRopes = 1;
+    float radius = 1e-4f, barrier = 8e-4f, kCollision = 1e-5f, detScaler = 1.2f, fricMu = 0.1f, kFriction = 5.f;
+    float minSegLen = 0.f, maxSegLen = 0.f;
+    bool stepped = false, fixRoot = false;
+    int lastErrorCode = 0, lastWarningCode = 0;
+    // device
+    hipStream_t stream = nullptr;
+    Dev D{};
+    float4 *d_initX = nullptr, *d_initQ = nullptr;
+    int* d_roff = nullptr;
+    int* d_ids = nullptr;
+    float* d_prog = nullptr;  // [17 N]
+    // cells of the persistent frame kernel
+    int mode = 1, variant = -1;
+    int cellsFor = -2;
+    std::vector<Cell> cells;
+    Cell* d_cells = nullptr;
+    int* d_order = nullptr;
+    int* d_hits = nullptr;
+    bool countHits = false;
+    // optional timing of the frame kernels on the solver stream (event pairs, summed on query)
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    // meshes (csrc/sdf.h)
+    std::vector<Grid> grids;
+    std::vector<float*> gridMem;
+    Grid* d_grids = nullptr;
+    Pose7 *d_cur = nullptr, *d_tgt = nullptr;
+    uint8_t* d_tele = nullptr;
+    Motion* d_motion = nullptr;
+    float* d_stage = nullptr;
+    uint8_t* d_stageTele = nullptr;
+    bool pendingSet = false, moving = false, sdfMoving = false, forcesOn = false, meshesTouched = false;
+    int lastSubsteps = 1;
+
+    ~rs_batch() {
+        if (stream) hipStreamSynchronize(stream);
+        for (auto& e : events) {
+            hipEventDestroy(e.first);
+            hipEventDestroy(e.second);
+        }
+        void* ptrs[] = {D.x, D.v, D.vlast, D.q, D.q2, D.dx, D.dx2, D.prm, D.qrest, D.ropeOf, d_roff, D.pad, D.maxStep,
+                        D.ncol, D.coll, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.outPos,
+                        D.outVel, D.outQuat, D.err, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_hits,
+                        d_grids, d_cur, d_tgt, d_tele, d_stage, d_stageTele};
+        for (void* p : ptrs)
+            if (p) hipFree(p);
+        for (float* g : gridMem) hipFree(g);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    // ---------------------------------------------------------------- construction
+    // RopeBatch ctor + Sim::configure + setKStretch / setKBend (csrc/batch.cu:193-276,
+    // YB/YarnBall.cu:112-259), in float32 like the reference's host code.
+    void build(const rs_desc& d) {
+        if (d.num_ropes < 1) throw Invalid("num_envs must be >= 1");
+        if (!d.offsets || !d.positions) throw Invalid("offsets and positions are required");
+        N = d.num_ropes;
+        roff.assign(d.offsets, d.offsets + N + 1);
+        if (roff[0] != 0) throw Invalid("offsets[0] must be 0");
+        for (int r = 0; r < N; r++)
+            if (roff[r + 1] - roff[r] < 3) throw Invalid("every rope needs >= 3 vertices");
+        nv = roff[N];
+        if (!(d.density > 0) || !(d.k_stretch > 0) || !(d.k_bend > 0) || !(d.max_h > 0) || d.num_itr < 1)
+            throw Invalid("density, k_stretch, k_bend, max_h > 0 and num_itr >= 1 required");
+        ropeOf.resize(nv);
+        for (int r = 0; r < N; r++)
+            for (int i = roff[r]; i < roff[r + 1]; i++) ropeOf[i] = r;
+        std::vector<f3> pos(nv);
+        for (int i = 0; i < nv; i++) {
+            pos[i] = mk3(d.positions[3 * i], d.positions[3 * i + 1], d.positions[3 * i + 2]);
+            if (!std::isfinite(pos[i].x) || !std::isfinite(pos[i].y) || !std::isfinite(pos[i].z))
+                throw Invalid("positions must be finite");
+        }
+        for (int r = 0; r < N; r++)
+            for (int i = roff[r]; i + 1 < roff[r + 1]; i++)
+                if (!(len(pos[i + 1] - pos[i]) > 0.f)) throw Invalid("consecutive vertices must not coincide");
+        std::vector<uint8_t> pin(nv, 0);
+        if (d.pinned)
+            for (int i = 0; i < nv; i++) pin[i] = d.pinned[i] ? 1 : 0;
+        else
+            for (int r = 0; r < N; r++) pin[roff[r]] = 1;
+        fixRoot = d.fix_root_orientation != 0;
+        if (fixRoot)
+            for (int r = 0; r < N; r++)
+                if (!pin[roff[r]]) throw Invalid("fix_root_orientation=True needs every root pinned");
+
+        std::vector<uint32_t> flags(nv, HAS_NEXT);
+        std::vector<float> invMass(nv, 1.f), lRest(nv, 1.f), kS(nv);
+        std::vector<q4> q(nv), qRest(nv, q4{0.f, 0.f, 0.f, 5.f});
+        for (int r = 0; r < N; r++) {
+            flags[roff[r + 1] - 1] = 0;
+            if (fixRoot) flags[roff[r]] |= FIX_ORIENT;
+        }
+        for (int i = 0; i < nv; i++)
+            if (pin[i]) invMass[i] = 0.f;
+        float minSeg = d.min_seg;
+        if (!(minSeg > 0.f)) {
+            minSeg = 1e30f;
+            for (int r = 0; r < N; r++)
+                for (int i = roff[r]; i + 1 < roff[r + 1]; i++) minSeg = std::min(minSeg, len(pos[i + 1] - pos[i]));
+        }
+        radius = 0.01f * minSeg;
+        barrier = 0.08f * minSeg;
+
+        maxSegLen = 0.f;
+        minSegLen = 3.402823466e38f;
+        q4 lastQ{0.f, 0.f, 0.f, 1.f};
+        for (int i = 0; i < nv; i++) {
+            if (i < nv - 1) {
+                const bool hn = (flags[i] & HAS_NEXT) != 0;
+                flags[i + 1] = hn ? (flags[i + 1] | HAS_PREV) : (flags[i + 1] & ~HAS_PREV);
+                const bool hnn = hn && (flags[i + 1] & HAS_NEXT);
+                flags[i] = hnn ? (flags[i] | HAS_NEXT_ORIENT) : (flags[i] & ~HAS_NEXT_ORIENT);
+                if (!hn) flags[i] |= FIX_ORIENT;
+            }
+            lRest[i] = 1.f / float(nv);
+            float mass = 0.f;
+            if (flags[i] & HAS_PREV) mass = mass + lRest[i - 1];
+            if (flags[i] & HAS_NEXT) {
+                const f3 seg0 = pos[i + 1] - pos[i];
+                lRest[i] = len(seg0);
+                const f3 sd = seg0 * (1.f / sqrtf(dot(seg0, seg0)));
+                q4 qq;
+                if (sd.x < -0.999999f) {
+                    qq = q4{0.f, 0.f, 1.f, 0.f};
+                } else {  // fromTo(e_x, sd) (Rotor.h:57-64)
+                    const f3 from = mk3(1.f, 0.f, 0.f);
+                    f3 hh = (from + sd) * 0.5f;
+                    const float l = len2(hh);
+                    if (l > 0.f) {
+                        hh = hh * (1.f / sqrtf(l));
+                    } else {  // orthoBasisX(from)[1] (Common.h:244-259)
+                        const f3 n = from;
+                        if (n.z >= n.y) {
+                            const float a = 1.f / (1.f + n.z), b = -n.x * n.y * a;
+                            hh = mk3(b, 1.f - n.y * n.y * a, -n.y);
+                        } else {
+                            const float a = 1.f / (1.f + n.y), b = -n.x * n.z * a;
+                            hh = mk3(1.f - n.x * n.x * a, -n.x, b);
+                        }
+                    }
+                    qq = mkq(cross(from, hh), dot(from, hh));
+                }
+                q4 t = qmul(qconj(qq), lastQ);
+                const q4 tw{t.x, 0.f, 0.f, t.w};
+                t = (qdot(tw, tw) > 1e-12f) ? qnormalize(tw) : q4{0.f, 0.f, 0.f, 1.f};
+                lastQ = qmul(qq, t);
+                mass = mass + lRest[i];
+                maxSegLen = std::max(maxSegLen, lRest[i]);
+                minSegLen = std::min(minSegLen, lRest[i]);
+            }
+            q[i] = lastQ;
+            mass = mass * (0.5f * d.density);
+            invMass[i] = mass != 0.f ? invMass[i] * (1.f / mass) : 0.f;
+        }
+        for (int i = 0; i < nv - 1; i++) qRest[i] = qscale(qlen(qRest[i]), qmul(qconj(q[i]), q[i + 1]));
+        if (d.straight_rest)
+            for (int r = 0; r < N; r++)
+                for (int i = roff[r]; i + 1 < roff[r + 1]; i++) qRest[i] = q4{0.f, 0.f, 0.f, 1.f};
+        for (int i = 0; i < nv; i++) kS[i] = d.k_stretch * lRest[i];
+        const float kb = d.k_bend * 4.f;
+        for (int i = 0; i < nv; i++) qRest[i] = qscale(kb / lRest[i], qnormalize(qRest[i]));
+        maxH = d.max_h;
+        h = d.max_h;
+        numItr = d.num_itr;
+
+        hx.resize(nv);
+        hq.resize(nv);
+        hprm.resize(nv);
+        hqrest.resize(nv);
+        hv.assign(nv, make_float4(0.f, 0.f, 0.f, 0.f));
+        for (int i = 0; i < nv; i++) {
+            hx[i] = f4(pos[i]);
+            hq[i] = f4q(q[i]);
+            uint32_t fl = flags[i];
+            float flf;
+            std::memcpy(&flf, &fl, 4);
+            hprm[i] = make_float4(invMass[i], lRest[i], kS[i], flf);
+            hqrest[i] = f4q(qRest[i]);
+        }
+        initX = hx;
+        initQ = hq;
+        allocate();
+        upload_all();
+    }
+
+    void allocate() {
+        hcheck(hipStreamCreate(&stream), "hipStreamCreate");
+        const size_t n = nv;
+        D.x = dalloc<float4>(n, "x");
+        D.v = dalloc<float4>(n, "v");
+        D.vlast = dalloc<float4>(n, "vlast");
+        D.q = dalloc<float4>(n, "q");
+        D.q2 = dalloc<float4>(n, "q2");
+        D.dx = dalloc<float4>(n, "dx");
+        D.dx2 = dalloc<float4>(n, "dx2");
+        D.prm = dalloc<float4>(n, "prm");
+        D.qrest = dalloc<float4>(n, "qrest");
+        D.ropeOf = dalloc<int>(n, "ropeOf");
+        d_roff = dalloc<int>(N + 1, "roff");
+        D.roff = d_roff;
+        D.pad = dalloc<float>(n, "pad");
+        D.maxStep = dalloc<float>(n, "maxStep");
+        D.ncol = dalloc<int>(n, "ncol");
+        D.outPos = dalloc<float>(3 * n, "outPos");
+        D.outVel = dalloc<float>(3 * n, "outVel");
+        D.outQuat = dalloc<float>(4 * n, "outQuat");
+        D.err = dalloc<int>(2, "err");
+        d_initX = dalloc<float4>(n, "initX");
+        d_initQ = dalloc<float4>(n, "initQ");
+        d_ids = dalloc<int>(N, "ids");
+        d_prog = dalloc<float>(17 * (size_t)N, "prog");
+        d_hits = dalloc<int>(1, "hits");
+        hcheck(hipMemset(D.err, 0, 2 * sizeof(int)), "memset err");
+        hcheck(hipMemset(D.vlast, 0, n * sizeof(float4)), "memset vlast");
+        hcheck(hipMemset(D.ncol, 0, n * sizeof(int)), "memset ncol");
+        hcheck(hipMemset(d_hits, 0, sizeof(int)), "memset hits");
+    }
+
+    void upload_all() {
+        const size_t n = nv;
+        hcheck(hipMemcpy(D.x, hx.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload x");
+        hcheck(hipMemcpy(D.v, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload v");
+        hcheck(hipMemcpy(D.q, hq.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload q");
+        hcheck(hipMemcpy(D.prm, hprm.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload prm");
+        hcheck(hipMemcpy(D.qrest, hqrest.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload qrest");
+        hcheck(hipMemcpy(D.ropeOf, ropeOf.data(), n * sizeof(int), hipMemcpyHostToDevice), "upload ropeOf");
+        hcheck(hipMemcpy(d_roff, roff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice), "upload roff");
+        hcheck(hipMemcpy(d_initX, initX.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload initX");
+        hcheck(hipMemcpy(d_initQ, initQ.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload initQ");
+        gather_nolock();
+        hcheck(hipStreamSynchronize(stream), "construct");
+    }
+
+    // Host mirror of the dynamic state from the device.
+    void download() {
+        hcheck(hipStreamSynchronize(stream), "sync");
+        const size_t n = nv;
+        hcheck(hipMemcpy(hx.data(), D.x, n * sizeof(float4), hipMemcpyDeviceToHost), "download x");
+        hcheck(hipMemcpy(hv.data(), D.v, n * sizeof(float4), hipMemcpyDeviceToHost), "download v");
+        hcheck(hipMemcpy(hq.data(), D.q, n * sizeof(float4), hipMemcpyDeviceToHost), "download q");
+    }
+
+    // ---------------------------------------------------------------- parameters
+    Params params() const {
+        Params P{};
+        P.nv = nv;
+        P.nropes = N;
+        P.numItr = numItr;
+        P.h = h;
+        P.lastH = lastH;
+        P.gx = gx;
+        P.gy = gy;
+        P.gz = gz;
+        P.damping = damping;
+        P.drag = drag;
+        P.dragQuad = dragQuad;
+        P.dragQuadT = dragQuadT;
+        P.accel = accel;
+        P.numPlanes = numPlanes;
+        std::memcpy(P.planes, planes, sizeof(planes));
+        P.cRadius = cRadius;
+        P.cKappa = cKappa;
+        P.cMuS = cMuS;
+        P.cMuK = cMuK;
+        P.cVSlip = cVSlip;
+        P.numSdf = (int)grids.size();
+        P.sdfMoving = sdfMoving ? 1 : 0;
+        P.forcesOn = forcesOn ? 1 : 0;
+        P.ropeContact = ropeContact ? 1 : 0;
+        P.groupRopes = groupRopes;
+        P.radius = radius;
+        P.barrier = barrier;
+        P.kCollision = kCollision;
+        P.fricMu = fricMu;
+        P.kFriction = kFriction;
+        P.detR = radius + 0.5f * barrier;  // uploadMeta (YB/YarnBall.cu:261-272)
+        P.scaledDetR = P.detR * detScaler;
+        return P;
+    }
+
+    // ---------------------------------------------------------------- meshes
+    void ensure_mesh_state() {
+        if (meshesTouched) return;
+        meshesTouched = true;
+        const size_t total = (size_t)N * MAX_SDF;
+        d_grids = dalloc<Grid>(MAX_SDF, "grids");
+        d_cur = dalloc<Pose7>(total, "sdf cur");
+        d_tgt = dalloc<Pose7>(total, "sdf tgt");
+        d_tele = dalloc<uint8_t>(total, "sdf tele");
+        D.motion = dalloc<Motion>(total, "sdf motion");
+        D.poses = dalloc<Pose>(total, "sdf poses");
+        d_stage = dalloc<float>(total * 7, "sdf stage");
+        d_stageTele = dalloc<uint8_t>(N, "sdf stage tele");
+        D.ncand = dalloc<int>(nv, "ncand");
+        D.cand = dalloc<int>((size_t)nv * MAX_CAND, "cand");
+        D.force = dalloc<float>(3 * (size_t)nv, "contact force");
+        D.wrench = dalloc<float>(total * 6, "wrench");
+        D.grids = d_grids;
+        std::vector<Pose7> ident(total, Pose7{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f});
+        hcheck(hipMemcpy(d_cur, ident.data(), total * sizeof(Pose7), hipMemcpyHostToDevice), "upload sdf cur");
+        hcheck(hipMemcpy(d_tgt, ident.data(), total * sizeof(Pose7), hipMemcpyHostToDevice), "upload sdf tgt");
+        hcheck(hipMemset(d_tele, 0, total), "memset tele");
+        hcheck(hipMemset(D.poses, 0, total * sizeof(Pose)), "memset poses");
+        hcheck(hipMemset(D.ncand, 0xFF, nv * sizeof(int)), "memset ncand");
+        hcheck(hipMemset(D.force, 0, 3 * nv * sizeof(float)), "memset force");
+        hcheck(hipMemset(D.wrench, 0, total * 6 * sizeof(float)), "memset wrench");
+    }
+    void upload_grids() {
+        std::vector<Grid> all(MAX_SDF);
+        std::memset(all.data(), 0, sizeof(Grid) * MAX_SDF);
+        for (size_t i = 0; i < grids.size(); i++) all[i] = grids[i];
+        hcheck(hipMemcpyAsync(d_grids, all.data(), sizeof(Grid) * MAX_SDF, hipMemcpyHostToDevice, stream),
+               "upload grids");
+        hcheck(hipStreamSynchronize(stream), "upload grids");
+    }
+    void set_targets_dev(int index, const float* pos, const float* q, int rows, const uint8_t* tele, bool teleAll) {
+        hcheck(launch_scatter_targets(d_tgt, d_tele, index, pos, q, rows, (int)grids.size(), tele, teleAll ? 1 : 0, N,
+                                      stream),
+               "set_mesh_poses");
+        pendingSet = true;
+    }
+    void set_targets_host(int index, const float* pos, const float* q, int rows, const uint8_t* tele, bool teleAll) {
+        const int n = index < 0 ? N * (int)grids.size() : rows;
+        float* dq = d_stage + 3 * (size_t)N * MAX_SDF;
+        hcheck(hipMemcpyAsync(d_stage, pos, sizeof(float) * 3 * n, hipMemcpyHostToDevice, stream), "upload target pos");
+        hcheck(hipMemcpyAsync(dq, q, sizeof(float) * 4 * n, hipMemcpyHostToDevice, stream), "upload target quat");
+        const uint8_t* tf = nullptr;
+        if (tele && !teleAll) {
+            hcheck(hipMemcpyAsync(d_stageTele, tele, N, hipMemcpyHostToDevice, stream), "upload teleport flags");
+            tf = d_stageTele;
+        }
+        set_targets_dev(index, d_stage, dq, rows, tf, teleAll);
+        hcheck(hipStreamSynchronize(stream), "set_mesh_poses");  // the host arrays may die on return
+    }
+    void begin_step() {  // SdfStore::begin_step (csrc/sdf.cu:217-232)
+        if (!meshesTouched) return;
+        if (pendingSet) {
+            hcheck(launch_stage_motion(d_cur, d_tgt, d_tele, const_cast<Motion*>(D.motion), N * MAX_SDF, stream),
+                   "stage motion");
+            sdfMoving = true;
+            moving = true;
+        } else {
+            sdfMoving = false;
+        }
+        if (!grids.empty()) hcheck(launch_fill_i32(D.ncand, -1, nv, stream), "reset candidates");
+        if (forcesOn) {
+            hcheck(launch_fill_f32(D.force, 0.f, 3L * nv, stream), "zero forces");
+            hcheck(launch_fill_f32(D.wrench, 0.f, 6L * N * MAX_SDF, stream), "zero wrenches");
+        }
+    }
+    void end_step(int substeps) {  // SdfStore::end_step (csrc/sdf.cu:239-253)
+        lastSubsteps = substeps > 0 ? substeps : 1;
+        if (!meshesTouched) return;
+        if (moving) {
+            hcheck(launch_commit(d_cur, d_tgt, d_tele, D.poses, d_grids, (int)grids.size(), -1, N, stream), "commit");
+            moving = false;
+            pendingSet = false;
+        }
+        if (forcesOn) {
+            const float k = 1.f / (float)lastSubsteps;
+            hcheck(launch_scale_f32(D.force, k, 3L * nv, stream), "average forces");
+            hcheck(launch_scale_f32(D.wrench, k, 6L * N * MAX_SDF, stream), "average wrenches");
+        }
+    }
+
+    // ---------------------------------------------------------------- cells
+    void build_cells(int var) {
+        const int cap = kCellVariants[var].block * kCellVariants[var].vpt;
+        const int G = ropeContact ? groupRopes : 1;
+        cells.clear();
+        Cell cur{0, 0, 0, 0};
+        for (int g0 = 0; g0 < N; g0 += G) {
+            const int g1 = std::min(N, g0 + G);
+            const int unitNv = roff[g1] - roff[g0];
+            if (unitNv > cap) throw Invalid("a rope / contact group does not fit the cell");
+            if (cur.nv + unitNv > cap && cur.nropes > 0) {
+                cells.push_back(cur);
+                cur = Cell{roff[g0], 0, g0, 0};
+            }
+            if (cur.nropes == 0) cur = Cell{roff[g0], 0, g0, 0};
+            cur.nv += unitNv;
+            cur.nropes += g1 - g0;
+        }
+        if (cur.nropes > 0) cells.push_back(cur);
+        std::vector<int> order(cells.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a].nv > cells[b].nv; });
+        if (d_cells) hipFree(d_cells);
+        if (d_order) hipFree(d_order);
+        d_cells = dalloc<Cell>(cells.size(), "cells");
+        d_order = dalloc<int>(order.size(), "cell order");
+        hcheck(hipMemcpy(d_cells, cells.data(), cells.size() * sizeof(Cell), hipMemcpyHostToDevice), "upload cells");
+        hcheck(hipMemcpy(d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice), "upload order");
+        cellsFor = var;
+    }
+    int max_unit() const {
+        const int G = ropeContact ? groupRopes : 1;
+        int m = 0;
+        for (int g0 = 0; g0 < N; g0 += G) m = std::max(m, roff[std::min(N, g0 + G)] - roff[g0]);
+        return m;
+    }
+    bool variant_supported(int var) const {
+        const int cap = kCellVariants[var].block * kCellVariants[var].vpt;
+        return cap >= max_unit() && cell_lds_bytes(kCellVariants[var].block, kCellVariants[var].vpt) <= device_max_lds();
+    }
+    int default_variant() const {
+        // smallest supported cell that still fills the machine with work; the autotuner refines it
+        int best = -1;
+        for (int v = 0; v < kNumCellVariants; v++) {
+            if (!variant_supported(v)) continue;
+            if (kCellVariants[v].vpt != 1) continue;
+            best = v;
+            break;
+        }
+        if (best < 0)
+            for (int v = 0; v < kNumCellVariants; v++)
+                if (variant_supported(v)) return v;
+        return best;
+    }
+
+    // ---------------------------------------------------------------- frames
+    void prepare(float hs) {  // SimAccess::prepare + uploadMeta checks (csrc/batch.cu:57-62, YB/YarnBall.cu:261-272)
+        if (!(detScaler > 1.f)) throw SolverErr("Detection scaler must be greater than 1.");
+        if (2.f * minSegLen < 2.f * radius + barrier)
+            throw SolverErr("Use thinner yarn or use longer segments. (2 * Min seg length must be at least 2 * radius + barrierThickness");
+        lastH = h;
+        h = hs;
+    }
+    void stage_inputs(const Drive& in, Drive& out) {
+        out = in;
+        float* P0 = d_prog;
+        float* V0 = P0 + 3 * N;
+        float* P1 = V0 + 3 * N;
+        float* Q0 = P1 + 3 * N;
+        float* Q1 = Q0 + 4 * N;
+        if (in.p0) hcheck(hipMemcpyAsync(P0, in.p0, 12 * (size_t)N, hipMemcpyDeviceToDevice, stream), "stage p0");
+        if (in.v0) hcheck(hipMemcpyAsync(V0, in.v0, 12 * (size_t)N, hipMemcpyDeviceToDevice, stream), "stage v0");
+        if (in.p1) hcheck(hipMemcpyAsync(P1, in.p1, 12 * (size_t)N, hipMemcpyDeviceToDevice, stream), "stage p1");
+        if (in.q0) hcheck(hipMemcpyAsync(Q0, in.q0, 16 * (size_t)N, hipMemcpyDeviceToDevice, stream), "stage q0");
+        if (in.q1) hcheck(hipMemcpyAsync(Q1, in.q1, 16 * (size_t)N, hipMemcpyDeviceToDevice, stream), "stage q1");
+        out.p0 = P0;
+        out.v0 = V0;
+        out.p1 = P1;
+        out.q0 = Q0;
+        out.q1 = Q1;
+        out.hasQ0 = in.q0 != nullptr;
+        out.hasQ1 = in.q1 != nullptr;
+    }
+    void run_substeps(const Drive& dr, int S) {
+        const Params P = params();
+        if (mode == 0) {
+            if (dr.mode == DRIVE_CONST) hcheck(launch_root_const(P, D, dr, stream), "root drive");
+            for (int s = 0; s < S; s++) hcheck(launch_plain_substep(P, D, dr, s, S, stream), "substep");
+            hcheck(launch_gather(P, D, stream), "gather");
+        } else {
+            int var = variant;
+            if (var < 0 || !variant_supported(var)) var = default_variant();
+            if (var < 0) throw Invalid("no persistent-kernel variant fits these ropes; use dispatch mode 0");
+            if (cellsFor != var) build_cells(var);
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timing) {
+                hcheck(hipEventCreate(&e0), "event");
+                hcheck(hipEventCreate(&e1), "event");
+                hcheck(hipEventRecord(e0, stream), "event");
+            }
+            hcheck(launch_cell(var, P, D, dr, d_cells, d_order, (int)cells.size(), S, countHits ? d_hits : nullptr,
+                               stream),
+                   "frame kernel");
+            if (timing) {
+                hcheck(hipEventRecord(e1, stream), "event");
+                events.emplace_back(e0, e1);
+            }
+        }
+    }
+    void frame(const Drive& in, float dt, int S) {
+        if (!(dt > 0.f)) throw Invalid("dt must be > 0");
+        if (S < 1) throw Invalid("substeps must be >= 1");
+        Drive dr;
+        stage_inputs(in, dr);
+        dr.dt = dt;
+        begin_step();
+        prepare(dt / float(S));
+        stepped = true;
+        run_substeps(dr, S);
+        time = time + dt;
+        check_errors();
+        end_step(S);
+    }
+    void check_errors() {  // Sim::checkErrors (YB/YarnBall.cu:311-339): errors come only from rope-rope detection
+        hcheck(hipGetLastError(), "launch");
+        if (!ropeContact) return;
+        int e[2] = {0, 0};
+        hcheck(hipMemcpyAsync(e, D.err, sizeof(e), hipMemcpyDeviceToHost, stream), "read error word");
+        hcheck(hipStreamSynchronize(stream), "frame");
+        if (e[0] != 0 || e[1] != 0) hcheck(hipMemsetAsync(D.err, 0, sizeof(e), stream), "reset error word");
+        if (e[1] != 0) lastWarningCode = e[1];
+        if (e[0] != 0) {
+            lastErrorCode = e[0];
+            throw SolverErr(e[0] == ERR_MAX_COLLISIONS ? "MAX_COLLISIONS_PER_SEGMENT exceeded" : "Indescript error");
+        }
+    }
+    void gather_nolock() { hcheck(launch_gather(params(), D, stream), "gather"); }
+};
+
+// ==================================================================== C ABI
+
+extern "C" {
+
+const char* rs_last_error_message(void) { return g_msg.c_str(); }
+int rs_abi_version(void) { return 1; }
+
+int rs_create(const rs_desc* desc, rs_batch** out) {
+    if (!desc || !out) {
+        g_msg = "null argument";
+        return RS_INVALID;
+    }
+    *out = nullptr;
+    rs_batch* b = new rs_batch();
+    const int rc = guarded([&] { b->build(*desc); });
+    if (rc != RS_OK) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return RS_OK;
+}
+
+void rs_destroy(rs_batch* b) { delete b; }
+int rs_num_vertices(const rs_batch* b) { return b ? b->nv : 0; }
+int rs_num_ropes(const rs_batch* b) { return b ? b->N : 0; }
+int rs_offsets(const rs_batch* b, int* out) {
+    return guarded([&] { std::copy(b->roff.begin(), b->roff.end(), out); });
+}
+
+int rs_get_param(const rs_batch* b, int which, double* out) {
+    return guarded([&] {
+        switch (which) {
+            case RS_P_DRAG: *out = b->drag; break;
+            case RS_P_DAMPING: *out = b->damping; break;
+            case RS_P_DRAG_QUAD: *out = b->dragQuad; break;
+            case RS_P_DRAG_QUAD_T: *out = b->dragQuadT; break;
+            case RS_P_NUM_ITR: *out = b->numItr; break;
+            case RS_P_ACCEL: *out = b->accel; break;
+            case RS_P_MAX_H: *out = b->maxH; break;
+            case RS_P_TIME: *out = b->time; break;
+            case RS_P_STEPPED: *out = b->stepped ? 1 : 0; break;
+            default: throw Invalid("unknown parameter");
+        }
+    });
+}
+
+int rs_set_param(rs_batch* b, int which, double value) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        const float v = (float)value;
+        switch (which) {
+            case RS_P_DRAG: b->drag = v; break;
+            case RS_P_DAMPING: b->damping = v; break;
+            case RS_P_DRAG_QUAD:
+                if (!std::isfinite(v) || v < 0) throw Invalid("drag_quad must be finite and >= 0");
+                b->dragQuad = v;
+                break;
+            case RS_P_DRAG_QUAD_T:
+                if (!std::isfinite(v) || v < 0) throw Invalid("drag_quad_t must be finite and >= 0");
+                b->dragQuadT = v;
+                break;
+            case RS_P_NUM_ITR:
+                if ((int)value < 1) throw Invalid("num_itr must be >= 1");
+                b->numItr = (int)value;
+                break;
+            case RS_P_ACCEL: b->accel = v; break;
+            case RS_P_MAX_H:
+                if (!(v > 0)) throw Invalid("max_h must be > 0");
+                b->maxH = v;
+                break;
+            default: throw Invalid("parameter is read only or unknown");
+        }
+    });
+}
+
+int rs_set_gravity(rs_batch* b, const float* g3) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        b->gx = g3[0];
+        b->gy = g3[1];
+        b->gz = g3[2];
+    });
+}
+int rs_get_gravity(const rs_batch* b, float* g3) {
+    g3[0] = b->gx;
+    g3[1] = b->gy;
+    g3[2] = b->gz;
+    return RS_OK;
+}
+
+int rs_get_vertex(rs_batch* b, int which, float* out) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        const int n = b->nv;
+        if (which == RS_V_POSITIONS || which == RS_V_VELOCITIES || which == RS_V_QUATERNIONS ||
+            which == RS_V_LAST_VELOCITIES) {
+            b->download();
+            if (which == RS_V_LAST_VELOCITIES) {
+                std::vector<float4> t(n);
+                hcheck(hipMemcpy(t.data(), b->D.vlast, n * sizeof(float4), hipMemcpyDeviceToHost), "download vlast");
+                for (int i = 0; i < n; i++) {
+                    out[3 * i] = t[i].x;
+                    out[3 * i + 1] = t[i].y;
+                    out[3 * i + 2] = t[i].z;
+                }
+                return;
+            }
+            const std::vector<float4>& src = which == RS_V_POSITIONS ? b->hx : (which == RS_V_VELOCITIES ? b->hv : b->hq);
+            const int w = which == RS_V_QUATERNIONS ? 4 : 3;
+            for (int i = 0; i < n; i++) {
+                out[w * i] = src[i].x;
+                out[w * i + 1] = src[i].y;
+                out[w * i + 2] = src[i].z;
+                if (w == 4) out[w * i + 3] = src[i].w;
+            }
+        } else if (which == RS_V_INV_MASSES) {
+            for (int i = 0; i < n; i++) out[i] = b->hprm[i].x;
+        } else if (which == RS_V_REST_LENGTHS) {
+            for (int i = 0; i < n; i++) out[i] = b->hprm[i].y;
+        } else {
+            throw Invalid("field is not readable");
+        }
+    });
+}
+
+int rs_set_vertex(rs_batch* b, int which, const float* in) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        const int n = b->nv;
+        if (which == RS_V_VELOCITIES) {  // set_velocities (csrc/bindings.cpp:1000-1013)
+            for (int i = 0; i < 3 * n; i++)
+                if (!std::isfinite(in[i])) throw Invalid("velocities must be finite");
+            b->download();
+            for (int i = 0; i < n; i++) b->hv[i] = make_float4(in[3 * i], in[3 * i + 1], in[3 * i + 2], 0.f);
+            hcheck(hipMemcpy(b->D.v, b->hv.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload v");
+            hcheck(launch_gather(b->params(), b->D, b->stream), "gather");
+            return;
+        }
+        if (which == RS_V_INV_MASSES) {  // raw inverse masses (RopeSim.unpin_root, csrc/bindings.cpp:520-526)
+            for (int i = 0; i < n; i++)
+                if (!std::isfinite(in[i]) || in[i] < 0) throw Invalid("inverse masses must be finite and >= 0");
+            for (int i = 0; i < n; i++) b->hprm[i].x = in[i];
+            hcheck(hipMemcpy(b->D.prm, b->hprm.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload prm");
+            return;
+        }
+        if (which == RS_V_MASSES || which == RS_V_EA || which == RS_V_EI) {
+            for (int r = 0; r < b->N; r++)
+                for (int i = b->roff[r]; i < b->roff[r + 1]; i++) {
+                    const bool seg = i + 1 < b->roff[r + 1];
+                    if (which == RS_V_MASSES) {
+                        if (b->hprm[i].x != 0.f && (!std::isfinite(in[i]) || !(in[i] > 0)))
+                            throw Invalid("node masses must be finite and > 0 (pinned columns ignored)");
+                    } else if (seg && (!std::isfinite(in[i]) || !(in[i] > 0))) {
+                        throw Invalid(which == RS_V_EA ? "EA values must be finite and > 0"
+                                                       : "EI values must be finite and > 0");
+                    }
+                }
+            for (int r = 0; r < b->N; r++)
+                for (int i = b->roff[r]; i < b->roff[r + 1]; i++) {
+                    const bool seg = i + 1 < b->roff[r + 1];
+                    float4& p = b->hprm[i];
+                    if (which == RS_V_MASSES) {
+                        if (p.x != 0.f) p.x = 1.f / in[i];
+                    } else if (which == RS_V_EA) {
+                        if (seg) p.z = in[i] * p.y;
+                    } else if (seg) {
+                        const float k = 4.f * in[i] / p.y;
+                        const float4 qr = b->hqrest[i];
+                        const q4 nq = qscale(k, qnormalize(q4{qr.x, qr.y, qr.z, qr.w}));
+                        b->hqrest[i] = make_float4(nq.x, nq.y, nq.z, nq.w);
+                    }
+                }
+            hcheck(hipMemcpy(b->D.prm, b->hprm.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload prm");
+            hcheck(hipMemcpy(b->D.qrest, b->hqrest.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload qrest");
+            return;
+        }
+        throw Invalid("field is not settable");
+    });
+}
+
+int rs_set_root_host(rs_batch* b, int rope, const float* pos3, const float* vel3, const float* q) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (rope < 0 || rope >= b->N) throw Invalid("rope index out of range");
+        const int i = b->roff[rope];
+        hcheck(hipStreamSynchronize(b->stream), "sync");
+        const float4 x = make_float4(pos3[0], pos3[1], pos3[2], 0.f);
+        const float4 v = make_float4(vel3[0], vel3[1], vel3[2], 0.f);
+        hcheck(hipMemcpy(b->D.x + i, &x, sizeof(float4), hipMemcpyHostToDevice), "set root x");
+        hcheck(hipMemcpy(b->D.v + i, &v, sizeof(float4), hipMemcpyHostToDevice), "set root v");
+        if (q) {
+            const q4 u = qunit_or_identity(q4{q[0], q[1], q[2], q[3]});
+            const float4 qq = make_float4(u.x, u.y, u.z, u.w);
+            hcheck(hipMemcpy(b->D.q + i, &qq, sizeof(float4), hipMemcpyHostToDevice), "set root q");
+        }
+        hcheck(launch_gather(b->params(), b->D, b->stream), "gather");
+        hcheck(hipStreamSynchronize(b->stream), "sync");
+    });
+}
+
+int rs_step_const(rs_batch* b, const float* pos, const float* vel, const float* q, float dt, int substeps) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (!pos || !vel) throw Invalid("root_pos and root_vel are required");
+        Drive d{};
+        d.mode = DRIVE_CONST;
+        d.p0 = pos;
+        d.v0 = vel;
+        d.q0 = q;
+        b->frame(d, dt, substeps);
+    });
+}
+
+int rs_step_program(rs_batch* b, const float* p0, const float* v0, const float* p1, const float* q0, const float* q1,
+                    float dt, int substeps) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (!p0 || !v0 || !p1) throw Invalid("root_pos, root_vel and root_pos_end are required");
+        if (q1 && !q0) throw Invalid("root_q_end requires root_q");
+        Drive d{};
+        d.mode = DRIVE_PROGRAM;
+        d.p0 = p0;
+        d.v0 = v0;
+        d.p1 = p1;
+        d.q0 = q0;
+        d.q1 = q1;
+        b->frame(d, dt, substeps);
+    });
+}
+
+int rs_step_free(rs_batch* b, float dt, int substeps) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        Drive d{};
+        d.mode = DRIVE_FREE;
+        b->frame(d, dt, substeps);
+    });
+}
+
+int rs_reset_envs(rs_batch* b, const int* ids, int count) {
+    return guarded([&] {
+        if (count <= 0) return;
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (count > b->N) throw Invalid("ids longer than num_envs");
+        for (int i = 0; i < count; i++)
+            if (ids[i] < 0 || ids[i] >= b->N) throw Invalid("env id out of range");
+        hcheck(hipMemcpyAsync(b->d_ids, ids, sizeof(int) * count, hipMemcpyHostToDevice, b->stream), "upload ids");
+        hcheck(launch_reset(b->params(), b->D, b->d_initX, b->d_initQ, b->d_ids, count, b->stream), "reset");
+        b->gather_nolock();
+        hcheck(hipStreamSynchronize(b->stream), "reset");
+    });
+}
+
+int rs_gather(rs_batch* b) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        b->gather_nolock();
+    });
+}
+
+int rs_views(rs_batch* b, rs_views_t* out) {
+    return guarded([&] {
+        out->positions = b->D.outPos;
+        out->velocities = b->D.outVel;
+        out->quaternions = b->D.outQuat;
+        out->contact_forces = b->forcesOn ? b->D.force : nullptr;
+        out->wrenches = b->forcesOn ? b->D.wrench : nullptr;
+        out->num_vertices = b->nv;
+        out->num_ropes = b->N;
+    });
+}
+
+static void check_material(float radius, float mu_s, float mu_k, float v_slip, float kappa, const char* who) {
+    if (!(radius > 0) || !(kappa > 0) || !(mu_s >= 0) || !(mu_k >= 0) || !(v_slip > 0))
+        throw Invalid(std::string(who) + " needs radius > 0, kappa > 0, mu_s >= 0, mu_k >= 0, v_slip > 0");
+}
+
+int rs_set_planes(rs_batch* b, int n, const float* planes, float radius, float mu_s, float mu_k, float v_slip,
+                  float kappa) {
+    return guarded([&] {
+        if (n < 0 || n > 8) throw Invalid("planes must be [P<=8, 4] rows (nx, ny, nz, w): half-spaces n.x >= w");
+        if (n > 0) check_material(radius, mu_s, mu_k, v_slip, kappa, "plane contact");
+        float rows[8][4];
+        for (int i = 0; i < n; i++) {
+            const f3 nrm = mk3(planes[4 * i], planes[4 * i + 1], planes[4 * i + 2]);
+            const float l = len(nrm);
+            if (!(l > 0) || !std::isfinite(l) || !std::isfinite(planes[4 * i + 3]))
+                throw Invalid("plane rows need a finite, nonzero normal and a finite offset");
+            rows[i][0] = nrm.x / l;
+            rows[i][1] = nrm.y / l;
+            rows[i][2] = nrm.z / l;
+            rows[i][3] = planes[4 * i + 3] / l;
+        }
+        std::lock_guard<std::mutex> g(b->mtx);
+        for (int i = 0; i < n; i++)
+            for (int c = 0; c < 4; c++) b->planes[i][c] = rows[i][c];
+        b->numPlanes = n;
+        if (n > 0) {
+            b->cRadius = radius;
+            b->cMuS = mu_s;
+            b->cMuK = mu_k;
+            b->cVSlip = v_slip;
+            b->cKappa = kappa;
+        }
+    });
+}
+
+int rs_set_contact(rs_batch* b, float radius, float mu_s, float mu_k, float v_slip, float kappa) {
+    return guarded([&] {
+        check_material(radius, mu_s, mu_k, v_slip, kappa, "surface contact");
+        std::lock_guard<std::mutex> g(b->mtx);
+        b->cRadius = radius;
+        b->cMuS = mu_s;
+        b->cMuK = mu_k;
+        b->cVSlip = v_slip;
+        b->cKappa = kappa;
+    });
+}
+
+int rs_get_contact(const rs_batch* b, float* o) {
+    o[0] = b->cRadius;
+    o[1] = b->cMuS;
+    o[2] = b->cMuK;
+    o[3] = b->cVSlip;
+    o[4] = b->cKappa;
+    return RS_OK;
+}
+
+int rs_set_rope_contact(rs_batch* b, int enabled, float radius, float barrier, float k_collision, float mu,
+                        float k_friction, float detection_scaler, int group_ropes) {
+    return guarded([&] {
+        if (enabled && (!(radius > 0) || !(barrier > 0) || !(k_collision > 0) || !(mu >= 0) || !(k_friction >= 0) ||
+                        !(detection_scaler > 1) || group_ropes < 1))
+            throw Invalid("rope contact needs radius, barrier, k_collision > 0, mu, k_friction >= 0, "
+                          "detection_scaler > 1, group_ropes >= 1");
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (enabled) {
+            if (2.f * b->minSegLen < 2.f * radius + barrier)
+                throw Invalid("rope contact needs 2 * radius + barrier <= 2 * shortest segment");
+            if (!b->D.coll) b->D.coll = dalloc<int>((size_t)b->nv * MAX_COLL, "collision lists");
+            b->groupRopes = group_ropes;
+            b->radius = radius;
+            b->barrier = barrier;
+            b->kCollision = k_collision;
+            b->fricMu = mu;
+            b->kFriction = k_friction;
+            b->detScaler = detection_scaler;
+        } else {
+            hcheck(launch_fill_i32(b->D.ncol, 0, b->nv, b->stream), "clear collision counts");
+        }
+        b->ropeContact = enabled != 0;
+        b->cellsFor = -2;
+    });
+}
+
+int rs_add_mesh(rs_batch* b, const float* values, int nx, int ny, int nz, const float* origin3, float spacing,
+                float mu_s, float mu_k, const float* pos, const float* quat, int rows, int* index_out) {
+    return guarded([&] {
+        if (nx < 2 || ny < 2 || nz < 2) throw Invalid("sdf values must be [nx, ny, nz] with every dim >= 2");
+        if (!(spacing > 0) || !std::isfinite(spacing)) throw Invalid("sdf spacing must be > 0");
+        const size_t n = (size_t)nx * ny * nz;
+        for (size_t i = 0; i < n; i++)
+            if (!std::isfinite(values[i])) throw Invalid("sdf values must be finite");
+        if (rows != 1 && rows != b->N) throw Invalid("add_mesh: pose rows must be 1 or num_envs");
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (!(b->cRadius > 0))
+            throw Invalid("set the contact material first: set_contact(radius, ...) or set_planes(..., radius=)");
+        if ((int)b->grids.size() >= MAX_SDF) throw std::invalid_argument("at most 64 meshes per sim");
+        b->ensure_mesh_state();
+        float* d = dalloc<float>(n, "sdf grid");
+        const hipError_t e = hipMemcpy(d, values, n * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(d);
+            hcheck(e, "upload sdf grid");
+        }
+        b->gridMem.push_back(d);
+        Grid G{};
+        G.v = d;
+        G.nx = nx;
+        G.ny = ny;
+        G.nz = nz;
+        G.ox = origin3[0];
+        G.oy = origin3[1];
+        G.oz = origin3[2];
+        G.hx = origin3[0] + float(nx - 1) * spacing;
+        G.hy = origin3[1] + float(ny - 1) * spacing;
+        G.hz = origin3[2] + float(nz - 1) * spacing;
+        G.h = spacing;
+        G.invH = 1.f / spacing;
+        G.muS = mu_s;
+        G.muK = mu_k;
+        b->grids.push_back(G);
+        b->upload_grids();
+        const int idx = (int)b->grids.size() - 1;
+        const bool pending = b->pendingSet;  // other meshes' targets stay pending
+        b->set_targets_host(idx, pos, quat, rows, nullptr, true);
+        hcheck(launch_commit(b->d_cur, b->d_tgt, b->d_tele, b->D.poses, b->d_grids, (int)b->grids.size(), idx, b->N,
+                             b->stream),
+               "commit mesh");
+        hcheck(hipStreamSynchronize(b->stream), "add_mesh");
+        b->pendingSet = pending;
+        *index_out = idx;
+    });
+}
+
+int rs_set_mesh_friction(rs_batch* b, int index, float mu_s, float mu_k) {
+    return guarded([&] {
+        if (!(mu_s >= 0) || !(mu_k >= 0)) throw Invalid("mu_s / mu_k must be >= 0");
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (index < 0 || index >= (int)b->grids.size()) throw Invalid("mesh index out of range");
+        b->grids[index].muS = mu_s;
+        b->grids[index].muK = mu_k;
+        b->upload_grids();
+    });
+}
+
+int rs_set_mesh_targets(rs_batch* b, int index, const float* pos, const float* quat, int rows, const unsigned char* tele,
+                        int tele_all, int on_device) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        const int M = (int)b->grids.size();
+        if (index >= M || (index < 0 && M == 0)) throw Invalid("mesh index out of range");
+        if (on_device)
+            b->set_targets_dev(index, pos, quat, rows, tele_all ? nullptr : tele, tele_all != 0);
+        else
+            b->set_targets_host(index, pos, quat, rows, tele, tele_all != 0);
+    });
+}
+
+int rs_clear_meshes(rs_batch* b) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        hcheck(hipStreamSynchronize(b->stream), "sync");
+        for (float* p : b->gridMem) hipFree(p);
+        b->gridMem.clear();
+        b->grids.clear();
+        b->pendingSet = false;
+        b->moving = false;
+        if (b->meshesTouched) b->upload_grids();
+    });
+}
+
+int rs_num_meshes(const rs_batch* b) { return (int)b->grids.size(); }
+
+int rs_enable_contact_forces(rs_batch* b, int on) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        b->ensure_mesh_state();
+        b->forcesOn = on != 0;
+    });
+}
+
+int rs_contact_forces_host(rs_batch* b, float* out) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (!b->forcesOn) throw Invalid("enable_contact_forces(True) first");
+        hcheck(hipStreamSynchronize(b->stream), "sync");
+        hcheck(hipMemcpy(out, b->D.force, 3 * (size_t)b->nv * sizeof(float), hipMemcpyDeviceToHost), "download forces");
+    });
+}
+
+int rs_mesh_wrenches_host(rs_batch* b, float* out) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (!b->forcesOn) throw Invalid("enable_contact_forces(True) first");
+        const size_t total = (size_t)b->N * MAX_SDF * 6;
+        std::vector<float> w(total);
+        hcheck(hipStreamSynchronize(b->stream), "sync");
+        hcheck(hipMemcpy(w.data(), b->D.wrench, total * sizeof(float), hipMemcpyDeviceToHost), "download wrenches");
+        const int M = (int)b->grids.size();
+        for (int e = 0; e < b->N; e++)
+            for (int m = 0; m < M; m++)
+                for (int c = 0; c < 6; c++) out[((size_t)e * M + m) * 6 + c] = w[((size_t)e * MAX_SDF + m) * 6 + c];
+    });
+}
+
+int rs_reset_clock(rs_batch* b) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        b->time = 0.f;
+        b->h = b->maxH;
+        b->lastErrorCode = 0;
+        b->lastWarningCode = 0;
+    });
+}
+
+int rs_error_codes(const rs_batch* b, int* last_error, int* last_warning) {
+    *last_error = b->lastErrorCode;
+    *last_warning = b->lastWarningCode;
+    return RS_OK;
+}
+int rs_clear_error_codes(rs_batch* b) {
+    b->lastErrorCode = 0;
+    b->lastWarningCode = 0;
+    return RS_OK;
+}
+
+int rs_set_dispatch(rs_batch* b, int mode, int variant) {
+    return guarded([&] {
+        if (mode != 0 && mode != 1) throw Invalid("dispatch mode must be 0 or 1");
+        if (variant >= kNumCellVariants || variant < -1) throw Invalid("unknown persistent-kernel variant");
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (mode == 1 && variant >= 0 && !b->variant_supported(variant))
+            throw Invalid("persistent-kernel variant too small for these ropes / groups or for the device's LDS");
+        b->mode = mode;
+        b->variant = variant;
+    });
+}
+int rs_get_dispatch(const rs_batch* b, int* mode, int* variant) {
+    *mode = b->mode;
+    *variant = b->variant >= 0 ? b->variant : (b->cellsFor >= 0 ? b->cellsFor : -1);
+    return RS_OK;
+}
+int rs_num_cell_variants(void) { return kNumCellVariants; }
+int rs_cell_variant_info(int variant, int* block, int* vpt, int* supported) {
+    if (variant < 0 || variant >= kNumCellVariants) return RS_INVALID;
+    *block = kCellVariants[variant].block;
+    *vpt = kCellVariants[variant].vpt;
+    if (supported) *supported = cell_lds_bytes(*block, *vpt) <= device_max_lds() ? 1 : 0;
+    return RS_OK;
+}
+
+int rs_autotune(rs_batch* b, float dt, int substeps, int program, float* best_ms) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        const size_t n = b->nv;
+        // snapshot what a frame changes
+        float4 *sx = dalloc<float4>(n, "tune x"), *sv = dalloc<float4>(n, "tune v"), *svl = dalloc<float4>(n, "tune vl"),
+               *sq = dalloc<float4>(n, "tune q");
+        hcheck(hipMemcpyAsync(sx, b->D.x, n * 16, hipMemcpyDeviceToDevice, b->stream), "snapshot");
+        hcheck(hipMemcpyAsync(sv, b->D.v, n * 16, hipMemcpyDeviceToDevice, b->stream), "snapshot");
+        hcheck(hipMemcpyAsync(svl, b->D.vlast, n * 16, hipMemcpyDeviceToDevice, b->stream), "snapshot");
+        hcheck(hipMemcpyAsync(sq, b->D.q, n * 16, hipMemcpyDeviceToDevice, b->stream), "snapshot");
+        const float sh = b->h, slh = b->lastH;
+        // a representative drive: roots held where they are (program with p1 = p0, v0 = 0)
+        std::vector<float> root(3 * b->N);
+        {
+            std::vector<float4> hx(n);
+            hcheck(hipMemcpyAsync(hx.data(), b->D.x, n * 16, hipMemcpyDeviceToHost, b->stream), "roots");
+            hcheck(hipStreamSynchronize(b->stream), "roots");
+            for (int e = 0; e < b->N; e++) {
+                root[3 * e] = hx[b->roff[e]].x;
+                root[3 * e + 1] = hx[b->roff[e]].y;
+                root[3 * e + 2] = hx[b->roff[e]].z;
+            }
+        }
+        float* dRoot = dalloc<float>(3 * b->N, "tune roots");
+        float* dZero = dalloc<float>(3 * b->N, "tune zero");
+        hcheck(hipMemcpy(dRoot, root.data(), 12 * (size_t)b->N, hipMemcpyHostToDevice), "tune roots");
+        hcheck(hipMemset(dZero, 0, 12 * (size_t)b->N), "tune zero");
+        Drive in{};
+        in.mode = program ? DRIVE_PROGRAM : DRIVE_FREE;
+        in.p0 = dRoot;
+        in.v0 = dZero;
+        in.p1 = dRoot;
+        Drive dr;
+        b->stage_inputs(in, dr);
+        dr.dt = dt;
+        b->lastH = b->h;
+        b->h = dt / float(substeps);
+        const int oldMode = b->mode, oldVar = b->variant;
+        b->mode = 1;
+        hipEvent_t e0, e1;
+        hcheck(hipEventCreate(&e0), "event");
+        hcheck(hipEventCreate(&e1), "event");
+        int best = -1;
+        float bestMs = 1e30f;
+        std::string err;
+        for (int v = 0; v < kNumCellVariants; v++) {
+            if (!b->variant_supported(v)) continue;
+            b->variant = v;
+            b->build_cells(v);
+            const Params P = b->params();
+            float ms = 0.f;
+            for (int rep = 0; rep < 2; rep++) {
+                hcheck(hipEventRecord(e0, b->stream), "event");
+                const hipError_t le = launch_cell(v, P, b->D, dr, b->d_cells, b->d_order, (int)b->cells.size(),
+                                                  substeps, nullptr, b->stream);
+                hcheck(hipEventRecord(e1, b->stream), "event");
+                hcheck(hipEventSynchronize(e1), "event");
+                if (le != hipSuccess) {
+                    ms = 1e30f;
+                    break;
+                }
+                hcheck(hipEventElapsedTime(&ms, e0, e1), "event");
+                // restore the state for the next timing
+                hcheck(hipMemcpyAsync(b->D.x, sx, n * 16, hipMemcpyDeviceToDevice, b->stream), "restore");
+                hcheck(hipMemcpyAsync(b->D.v, sv, n * 16, hipMemcpyDeviceToDevice, b->stream), "restore");
+                hcheck(hipMemcpyAsync(b->D.vlast, svl, n * 16, hipMemcpyDeviceToDevice, b->stream), "restore");
+                hcheck(hipMemcpyAsync(b->D.q, sq, n * 16, hipMemcpyDeviceToDevice, b->stream), "restore");
+            }
+            if (ms < bestMs) {
+                bestMs = ms;
+                best = v;
+            }
+        }
+        hcheck(hipMemsetAsync(b->D.err, 0, 2 * sizeof(int), b->stream), "reset error word");
+        hcheck(hipStreamSynchronize(b->stream), "autotune");
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        for (void* p : {(void*)sx, (void*)sv, (void*)svl, (void*)sq, (void*)dRoot, (void*)dZero}) hipFree(p);
+        b->h = sh;
+        b->lastH = slh;
+        b->gather_nolock();
+        b->mode = oldMode == 0 ? 0 : 1;
+        b->variant = best >= 0 ? best : oldVar;
+        if (b->cellsFor != b->variant && b->variant >= 0) b->build_cells(b->variant);
+        if (best_ms) *best_ms = bestMs;
+    });
+}
+
+int rs_check_detect(rs_batch* b, const float* pos, const float* pads, int* counts, int* lists, float* max_step) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (!b->D.coll) throw Invalid("enable rope contact first");
+        const size_t n = b->nv;
+        std::vector<float4> x(n);
+        for (size_t i = 0; i < n; i++) x[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.f);
+        Dev D = b->D;
+        D.x = dalloc<float4>(n, "check x");
+        hcheck(hipMemcpy(D.x, x.data(), n * 16, hipMemcpyHostToDevice), "check x");
+        hcheck(hipMemcpy(D.pad, pads, n * 4, hipMemcpyHostToDevice), "check pads");
+        hcheck(launch_fill_i32(D.coll, -1, (long)n * MAX_COLL, b->stream), "clear lists");
+        hcheck(launch_check_detect(b->params(), D, b->stream), "check detect");
+        hcheck(hipStreamSynchronize(b->stream), "check detect");
+        hcheck(hipMemcpy(counts, D.ncol, n * 4, hipMemcpyDeviceToHost), "counts");
+        hcheck(hipMemcpy(lists, D.coll, n * MAX_COLL * 4, hipMemcpyDeviceToHost), "lists");
+        hcheck(hipMemcpy(max_step, D.maxStep, n * 4, hipMemcpyDeviceToHost), "step limits");
+        for (size_t i = 0; i < n; i++)
+            for (int c = counts[i]; c < MAX_COLL; c++) lists[i * MAX_COLL + c] = -1;
+        hipFree(D.x);
+        hcheck(hipMemsetAsync(D.err, 0, 2 * sizeof(int), b->stream), "reset error word");
+        hcheck(launch_fill_i32(D.ncol, 0, (long)n, b->stream), "clear counts");
+        hcheck(hipStreamSynchronize(b->stream), "check detect");
+    });
+}
+
+int rs_check_broadphase(rs_batch* b, const float* pos, const float* dx, int* counts, int* cand) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        b->ensure_mesh_state();
+        const size_t n = b->nv;
+        std::vector<float4> x(n), d(n);
+        for (size_t i = 0; i < n; i++) {
+            x[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.f);
+            d[i] = make_float4(dx[3 * i], dx[3 * i + 1], dx[3 * i + 2], 0.f);
+        }
+        float4* dX = dalloc<float4>(n, "check x");
+        float4* dD = dalloc<float4>(n, "check dx");
+        int* dC = dalloc<int>(n, "check counts");
+        int* dL = dalloc<int>(n * MAX_CAND, "check cand");
+        hcheck(hipMemcpy(dX, x.data(), n * 16, hipMemcpyHostToDevice), "check x");
+        hcheck(hipMemcpy(dD, d.data(), n * 16, hipMemcpyHostToDevice), "check dx");
+        Params P = b->params();
+        P.sdfMoving = 0;
+        hcheck(launch_check_broadphase(P, b->D, dX, dD, dC, dL, b->stream), "check broadphase");
+        hcheck(hipStreamSynchronize(b->stream), "check broadphase");
+        hcheck(hipMemcpy(counts, dC, n * 4, hipMemcpyDeviceToHost), "counts");
+        hcheck(hipMemcpy(cand, dL, n * MAX_CAND * 4, hipMemcpyDeviceToHost), "cand");
+        for (void* p : {(void*)dX, (void*)dD, (void*)dC, (void*)dL}) hipFree(p);
+    });
+}
+
+int rs_set_kernel_timing(rs_batch* b, int on) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        b->timing = on != 0;
+    });
+}
+
+int rs_kernel_time(rs_batch* b, double* total_ms, int* launches) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        double t = 0.0;
+        for (auto& e : b->events) {
+            hcheck(hipEventSynchronize(e.second), "event");
+            float ms = 0.f;
+            hcheck(hipEventElapsedTime(&ms, e.first, e.second), "event");
+            t += ms;
+            hipEventDestroy(e.first);
+            hipEventDestroy(e.second);
+        }
+        *launches = (int)b->events.size();
+        *total_ms = t;
+        b->events.clear();
+    });
+}
+
+int rs_set_chunk_stats(rs_batch* b, int on, long long* hits_out) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        int h = 0;
+        hcheck(hipStreamSynchronize(b->stream), "sync");
+        hcheck(hipMemcpy(&h, b->d_hits, sizeof(int), hipMemcpyDeviceToHost), "hits");
+        hcheck(hipMemset(b->d_hits, 0, sizeof(int)), "hits");
+        if (hits_out) *hits_out = h;
+        b->countHits = on != 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,888 @@
+// HIP kernels for gfx950 of the batched Cosserat-rod substep.
+//
+// Two dispatch forms over the same per-vertex routines (rs_device.hpp):
+//  * per-substep kernels: root program, init (+ kinematic mesh poses), rope-rope detection (+ step
+//    limit), numItr x (position sweep, orientation sweep), end (+ broad phase), one launch each per
+//    substep, state in HBM between launches (the reference's launch structure,
+//    csrc/batch.cu:351-358 and sim/step.cpp:14-29, in snapshot / Jacobi semantics);
+//  * the persistent frame kernel: one workgroup per cell (whole ropes / contact groups), all
+//    substeps of a frame with the rope state in LDS and registers, one state round trip per frame.
+// Both give bit-identical results (same arithmetic, no FP contraction, same list order).
+#include <hip/hip_runtime.h>
+
+#include "rs_device.hpp"
+#include "rs_kernels.hpp"
+
+namespace rs {
+
+// ------------------------------------------------------------------ small kernels
+
+__global__ void k_fill_i32(int* p, int v, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f32(float* p, float v, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_scale_f32(float* p, float s, long n) {  // sdf.cu:89-92
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = p[i] * s;
+}
+
+// Constant-velocity root drive once per frame (csrc/batch.cu:90-104).
+__global__ void k_root_const(Params P, Dev D, Drive Dr) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= P.nropes) return;
+    const int root = D.roff[e];
+    D.x[root] = make_float4(Dr.p0[3 * e], Dr.p0[3 * e + 1], Dr.p0[3 * e + 2], 0.f);
+    D.v[root] = make_float4(Dr.v0[3 * e], Dr.v0[3 * e + 1], Dr.v0[3 * e + 2], 0.f);
+    if (Dr.hasQ0)
+        D.q[root] = stq(qunit_or_identity(q4{Dr.q0[4 * e], Dr.q0[4 * e + 1], Dr.q0[4 * e + 2], Dr.q0[4 * e + 3]}));
+}
+
+// Second-order root program at substep s (csrc/batch.cu:122-159).
+__global__ void k_root_program(Params P, Dev D, Drive Dr, float4* q, int s, int S) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= P.nropes) return;
+    const int root = D.roff[e];
+    f3 x, v;
+    q4 qq;
+    bool setQ;
+    root_program(Dr, e, s, S, P.h, x, v, qq, setQ);
+    D.x[root] = st3(x);
+    D.v[root] = st3(v);
+    if (setQ) q[root] = stq(qq);
+}
+
+// initItr (YB/sim/iteration.cu:10-69): kinematic mesh poses of this substep, then per vertex the
+// displacement guess, the inertial target (stored in the velocity slot) and the detection padding.
+__global__ void k_init(Params P, Dev D, int sub, int nSub, float4* dxOut) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P.sdfMoving && tid < P.nropes * MAX_SDF) {
+        const int m = tid % MAX_SDF;
+        if (m < P.numSdf) store_pose(D.poses + tid, substep_pose(D.motion[tid], D.grids[m], sub, nSub, P.h));
+    }
+    if (tid >= P.nv) return;
+    const float4 prm = D.prm[tid];
+    const f3 v = ld3(D.v[tid]);
+    const f3 vl = ld3(D.vlast[tid]);
+    D.vlast[tid] = st3(v);
+    f3 d, y;
+    float padding;
+    init_vertex(P, prm.x, v, vl, d, y, padding);
+    if (prm.x != 0.f) D.v[tid] = st3(y);
+    D.pad[tid] = padding;
+    dxOut[tid] = st3(d);
+}
+
+// Rope-rope detection, one thread per segment scanning its contact group in ascending order
+// (csrc/batch.cu:19-49), fused with the step limit (YB/sim/collision.cu:103-136).
+__global__ void k_detect(Params P, Dev D) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nv) return;
+    const uint32_t fl = flags_of(D.prm[i]);
+    if (!(fl & HAS_NEXT)) {
+        D.ncol[i] = 0;
+        D.maxStep[i] = INFINITY;
+        return;
+    }
+    const int rope = D.ropeOf[i];
+    const int G = P.groupRopes;
+    const int g = rope / G;
+    const int g0 = D.roff[min(g * G, P.nropes)], g1 = D.roff[min((g + 1) * G, P.nropes)];
+    const f3 a0 = ld3(D.x[i]);
+    const f3 a1 = ld3(D.x[i + 1]) - a0;
+    const float ri = fmaxf(D.pad[i], D.pad[i + 1]);
+    float minDist = D.pad[i] - P.detR;
+    int cnt = 0;
+    for (int j = g0; j < g1; j++) {
+        if (!(flags_of(D.prm[j]) & HAS_NEXT)) continue;
+        if (D.ropeOf[j] == rope && abs(j - i) <= 2) continue;
+        const f3 b0 = ld3(D.x[j]) - a0, b1 = ld3(D.x[j + 1]) - a0;
+        const float r = ri + fmaxf(D.pad[j], D.pad[j + 1]);
+        float d2;
+        if (pair_hit(a1, b0, b1, r, d2)) {
+            if (cnt < MAX_COLL) {
+                D.coll[(size_t)i * MAX_COLL + cnt] = j;
+                minDist = fminf(minDist, 0.45f * sqrtf(d2));
+            }
+            cnt++;
+        }
+    }
+    if (cnt > MAX_COLL) D.err[0] = ERR_MAX_COLLISIONS;
+    D.ncol[i] = min(cnt, MAX_COLL);
+    D.maxStep[i] = minDist;
+}
+
+__device__ __forceinline__ uint64_t load_cand(const Dev& D, int i, int n) {
+    uint64_t packed = 0;
+    for (int c = 0; c < n && c < MAX_CAND; c++) packed |= (uint64_t)D.cand[(size_t)i * MAX_CAND + c] << (6 * c);
+    return packed;
+}
+
+// Position sweep (YB/sim/cosserat.cu:23-225, snapshot mode): a block owns PLAIN_OWN vertices plus one
+// halo vertex on its left whose segment terms it assembles for its first owned vertex.
+constexpr int PLAIN_BLOCK = 256;
+constexpr int PLAIN_OWN = PLAIN_BLOCK - 1;
+template <bool LIMIT>
+__global__ __launch_bounds__(PLAIN_BLOCK) void k_pos(Params P, Dev D, const float4* dxIn, const float4* qIn,
+                                                     float4* dxOut) {
+    __shared__ float sF[10][PLAIN_BLOCK];
+    const int lt = threadIdx.x;
+    const int i = blockIdx.x * PLAIN_OWN + lt - 1;
+    const bool valid = i >= 0 && i < P.nv;
+    f3 f = mk3(0.f, 0.f, 0.f), f2 = f, d0 = f;
+    s3 H = szero(), H2 = szero();
+    float4 prm = make_float4(0.f, 0.f, 0.f, 0.f);
+    float stepLimit = INFINITY;
+    if (valid) {
+        prm = D.prm[i];
+        const uint32_t fl = flags_of(prm);
+        const f3 p0 = ld3(D.x[i]);
+        d0 = ld3(dxIn[i]);
+        f3 p1 = p0, p1dx = d0;
+        if (fl & HAS_NEXT) {
+            p1 = ld3(D.x[i + 1]);
+            p1dx = ld3(dxIn[i + 1]);
+        }
+        int nc = -1;
+        uint64_t cand = 0;
+        const Pose* envPoses = D.poses + (size_t)D.ropeOf[i] * MAX_SDF;
+        if (P.numSdf > 0) {
+            nc = D.ncand[i];
+            if (nc > 0) cand = load_cand(D, i, nc);
+        }
+        vertex_terms(P, D.grids, envPoses, nc, cand, prm.x, prm.y, prm.z, fl, p0, d0, ld3(D.v[i]), ldq(qIn[i]), p1,
+                     p1dx, f, H, f2, H2);
+        if (LIMIT && (fl & HAS_NEXT)) {
+            const float dmp = P.damping / P.h;
+            const int n = D.ncol[i];
+            for (int c = 0; c < n; c++) {
+                const int j = D.coll[(size_t)i * MAX_COLL + c];
+                collision_term(P, dmp, p0, p1, d0, p1dx, ld3(D.x[j]), ld3(D.x[j + 1]), ld3(dxIn[j]), ld3(dxIn[j + 1]),
+                               f, H, f2, H2);
+            }
+        }
+        if (LIMIT) stepLimit = D.maxStep[i];
+    }
+    sF[0][lt] = f2.x;
+    sF[1][lt] = f2.y;
+    sF[2][lt] = f2.z;
+#pragma unroll
+    for (int c = 0; c < 6; c++) sF[3 + c][lt] = H2.m[c];
+    sF[9][lt] = stepLimit;
+    __syncthreads();
+    if (!valid || lt == 0) return;
+    const uint32_t fl = flags_of(prm);
+    if (fl & HAS_PREV) {
+        if (LIMIT) stepLimit = fminf(stepLimit, sF[9][lt - 1]);
+        f = f + mk3(sF[0][lt - 1], sF[1][lt - 1], sF[2][lt - 1]);
+        s3 Hp;
+#pragma unroll
+        for (int c = 0; c < 6; c++) Hp.m[c] = sF[3 + c][lt - 1];
+        sadd(H, Hp);
+    }
+    f3 d = d0;
+    if (prm.x != 0.f) d = solve_update(P, d0, H, f, LIMIT, stepLimit);
+    dxOut[i] = st3(d);
+}
+
+// Orientation sweep (YB/sim/cosserat.cu:227-273, snapshot mode): reads the new displacements and the
+// previous frames, writes the next frames (frames of other segments are copied through).
+__global__ void k_orient(Params P, Dev D, const float4* dxNew, const float4* qIn, float4* qOut) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nv) return;
+    const float4 prm = D.prm[i];
+    const uint32_t fl = flags_of(prm);
+    const q4 q0 = ldq(qIn[i]);
+    if (!(fl & FIX_ORIENT) && (fl & HAS_NEXT)) {
+        const q4 qp = (fl & HAS_PREV) ? ldq(qIn[i - 1]) : q0;
+        const q4 qn = (fl & HAS_NEXT_ORIENT) ? ldq(qIn[i + 1]) : q0;
+        const q4 qrp = (fl & HAS_PREV) ? ldq(D.qrest[i - 1]) : q0;
+        qOut[i] = stq(orient_update(fl, prm.y, prm.z, ld3(D.x[i]), ld3(D.x[i + 1]), ld3(dxNew[i]), ld3(dxNew[i + 1]),
+                                    qp, q0, qn, qrp, ldq(D.qrest[i])));
+    } else {
+        qOut[i] = stq(q0);
+    }
+}
+
+// endItr (YB/sim/iteration.cu:77-129): velocity with drag, optional force readout, position commit,
+// broad phase for the next substep.
+__global__ void k_end(Params P, Dev D, const float4* dxIn, const float4* qCur) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nv) return;
+    const float4 prm = D.prm[i];
+    const uint32_t fl = flags_of(prm);
+    const f3 d = ld3(dxIn[i]);
+    const f3 x = ld3(D.x[i]);
+    const int rope = D.ropeOf[i];
+    const Pose* envPoses = D.poses + (size_t)rope * MAX_SDF;
+    int nc = -1;
+    uint64_t cand = 0;
+    if (P.numSdf > 0) {
+        nc = D.ncand[i];
+        if (nc > 0) cand = load_cand(D, i, nc);
+    }
+    if (prm.x != 0.f) {
+        f3 t = mk3(1.f, 0.f, 0.f);
+        if (P.dragQuad > 0.f) t = qrot(ldq((fl & HAS_NEXT) ? qCur[i] : qCur[i - 1]), mk3(1.f, 0.f, 0.f));
+        D.v[i] = st3(end_velocity(P, d, t));
+        if (P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0)) {
+            const float kap = P.cKappa / P.h;
+            const f3 fv = surface_readout(P, D.grids, envPoses, D.wrench + (size_t)rope * MAX_SDF * 6, nc, cand, x, d,
+                                          kap * kap / prm.x);
+            float* fo = D.force + 3 * (size_t)i;
+            fo[0] = fo[0] + fv.x;
+            fo[1] = fo[1] + fv.y;
+            fo[2] = fo[2] + fv.z;
+        }
+    }
+    const f3 xn = x + d;
+    D.x[i] = st3(xn);
+    if (P.numSdf > 0) {
+        if (prm.x != 0.f) {
+            uint64_t packed;
+            const int n = build_candidates(P, envPoses, xn, 2.f * len(d) + 0.02f, packed);
+            D.ncand[i] = n;
+            for (int c = 0; c < MAX_CAND; c++) D.cand[(size_t)i * MAX_CAND + c] = (n < 0 || c >= n) ? -1 : cand_get(packed, c);
+        } else {
+            D.ncand[i] = 0;
+        }
+    }
+}
+
+// State -> contiguous float views (csrc/batch.cu:163-175).
+__global__ void k_gather(Params P, Dev D) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nv) return;
+    const float4 x = D.x[i], v = D.v[i], q = D.q[i];
+    D.outPos[3 * i] = x.x;
+    D.outPos[3 * i + 1] = x.y;
+    D.outPos[3 * i + 2] = x.z;
+    D.outVel[3 * i] = v.x;
+    D.outVel[3 * i + 1] = v.y;
+    D.outVel[3 * i + 2] = v.z;
+    D.outQuat[4 * i] = q.x;
+    D.outQuat[4 * i + 1] = q.y;
+    D.outQuat[4 * i + 2] = q.z;
+    D.outQuat[4 * i + 3] = q.w;
+}
+
+// Restore construction state of the selected ropes (csrc/batch.cu:178-189): one block per rope.
+__global__ void k_reset(Params P, Dev D, const float4* initX, const float4* initQ, const int* ids, int count) {
+    const int c = blockIdx.x;
+    if (c >= count) return;
+    const int e = ids[c];
+    for (int i = D.roff[e] + threadIdx.x; i < D.roff[e + 1]; i += blockDim.x) {
+        D.x[i] = initX[i];
+        D.q[i] = initQ[i];
+        D.v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        D.vlast[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// ------------------------------------------------------------------ mesh pose state machine (csrc/sdf.cu)
+
+__device__ __forceinline__ f3 rotvec(q4 q0, q4 q1) {  // sdf.cu:25-33
+    const f3 a = qvec(q1), b = -qvec(q0);
+    f3 v = (q1.w * b + q0.w * a) + cross(a, b);
+    float w = q1.w * q0.w - dot(a, b);
+    if (w < 0.f) {
+        v = -v;
+        w = -w;
+    }
+    const float s = len(v);
+    if (!(s > 1e-9f)) return mk3(0.f, 0.f, 0.f);
+    return v * (2.f * atan2f(s, w) / s);
+}
+
+// Targets for mesh `index` (rows 1 or nropes) or, with index < 0, a dense [nropes, M] block (sdf.cu:37-51).
+__global__ void k_scatter_targets(Pose7* tgt, uint8_t* tele, int index, const float* pos, const float* q, int rows,
+                                  int M, const uint8_t* teleFlags, int teleAll, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = index < 0 ? M : 1;
+    if (i >= N * per) return;
+    const int e = i / per, m = index < 0 ? i % per : index;
+    const int r = index < 0 ? i : (rows == 1 ? 0 : e);
+    q4 qq{q[4 * r], q[4 * r + 1], q[4 * r + 2], q[4 * r + 3]};
+    const float l = qlen(qq);
+    qq = l > 0.f ? q4{qq.x / l, qq.y / l, qq.z / l, qq.w / l} : q4{0.f, 0.f, 0.f, 1.f};
+    const size_t slot = (size_t)e * MAX_SDF + m;
+    tgt[slot] = Pose7{pos[3 * r], pos[3 * r + 1], pos[3 * r + 2], qq.x, qq.y, qq.z, qq.w, 0.f};
+    tele[slot] = teleAll ? 1 : (teleFlags ? teleFlags[e] : 0);
+}
+
+// This frame's motion from the current pose to the target; teleported slots jump (sdf.cu:55-63).
+__global__ void k_stage_motion(Pose7* cur, const Pose7* tgt, uint8_t* tele, Motion* motion, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    if (tele[i]) {
+        cur[i] = tgt[i];
+        tele[i] = 0;
+    }
+    const Pose7 c = cur[i], t = tgt[i];
+    const f3 rv = rotvec(q4{c.qx, c.qy, c.qz, c.qw}, q4{t.qx, t.qy, t.qz, t.qw});
+    motion[i] = Motion{c.px, c.py, c.pz, c.qx, c.qy, c.qz, c.qw, t.px - c.px, t.py - c.py, t.pz - c.pz,
+                       rv.x, rv.y, rv.z, 0.f};
+}
+
+// Commit targets as static current poses with their world AABBs (sdf.cu:67-87). index >= 0: that mesh only.
+__global__ void k_commit(Pose7* cur, const Pose7* tgt, uint8_t* tele, Pose* poses, const Grid* grids, int numSdf,
+                         int index, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int slot, m;
+    if (index >= 0) {
+        if (i >= N) return;
+        m = index;
+        slot = i * MAX_SDF + index;
+    } else {
+        if (i >= N * MAX_SDF) return;
+        m = i % MAX_SDF;
+        slot = i;
+        if (m >= numSdf) return;
+    }
+    cur[slot] = tgt[slot];
+    tele[slot] = 0;
+    const Pose7 c = cur[slot];
+    const f3 pos = mk3(c.px, c.py, c.pz);
+    const q4 q{c.qx, c.qy, c.qz, c.qw};
+    f3 lo, hi;
+    world_box(grids[m], pos, q, lo, hi);
+    store_pose(poses + slot, Pose{pos.x, pos.y, pos.z, q.x, q.y, q.z, q.w, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, lo.x, lo.y,
+                                  lo.z, hi.x, hi.y, hi.z, 0.f});
+}
+
+// ------------------------------------------------------------------ function-level check kernels
+
+// Broad phase on given end-of-substep positions / displacements with the current poses.
+__global__ void k_check_broadphase(Params P, Dev D, const float4* xs, const float4* dxs, int* counts, int* cand) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nv) return;
+    int n = 0;
+    uint64_t packed = 0;
+    if (D.prm[i].x != 0.f) {
+        const f3 d = ld3(dxs[i]);
+        n = build_candidates(P, D.poses + (size_t)D.ropeOf[i] * MAX_SDF, ld3(xs[i]), 2.f * len(d) + 0.02f, packed);
+    }
+    counts[i] = n;
+    for (int c = 0; c < MAX_CAND; c++) cand[(size_t)i * MAX_CAND + c] = (n > 0 && c < n) ? cand_get(packed, c) : -1;
+}
+
+// ------------------------------------------------------------------ the persistent frame kernel
+//
+// One workgroup advances one cell (a contiguous run of whole ropes; a rope-rope contact group never
+// straddles cells) through all S substeps of a frame. Thread t owns local vertex slots t + k * BLOCK,
+// k < VPT. LDS holds what neighbours and contact partners read: positions, the displacement iterate
+// (ping-pong), the frames (ping-pong), the segment terms exchanged with the next vertex, the detection
+// paddings, the step limits and per-chunk boxes for the rope-rope broad phase. Per-vertex constants,
+// velocities, inertial targets and broad-phase candidates stay in registers. Per iteration there are
+// two barriers: [orientation(k-1) + terms(k)] | [solve(k)] | ...
+template <int BLOCK, int VPT>
+__global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const Cell* cells, const int* order, int S,
+                                                int* chunkHits) {
+    constexpr int CAP = BLOCK * VPT;
+    constexpr int NCH = CAP / 16;
+    extern __shared__ float4 smem4[];
+    float* sX = reinterpret_cast<float*>(smem4);  // [3][CAP]
+    float* sDA = sX + 3 * CAP;                    // [3][CAP]
+    float* sDB = sDA + 3 * CAP;                   // [3][CAP]
+    float4* sQA = reinterpret_cast<float4*>(sDB + 3 * CAP);
+    float4* sQB = sQA + CAP;
+    float* sF = reinterpret_cast<float*>(sQB + CAP);  // [9][CAP]
+    float* sPad = sF + 9 * CAP;                       // [CAP]
+    float* sLim = sPad + CAP;                         // [CAP]
+    float* sBox = sLim + CAP;                         // [6][NCH]
+    uint32_t* sFl = reinterpret_cast<uint32_t*>(sBox + 6 * NCH);  // [CAP]
+
+    const Cell C = cells[order[blockIdx.x]];
+    const int t = threadIdx.x;
+
+    int gi[VPT], rope[VPT], nc[VPT], ncol[VPT];
+    float4 prm[VPT];
+    q4 qr[VPT], qrp[VPT];
+    f3 vel[VPT], vl[VPT], y[VPT];
+    float limEff[VPT];
+    uint64_t cand[VPT];
+    bool isRoot[VPT];
+
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        const int sl = t + k * BLOCK;
+        gi[k] = sl < C.nv ? C.v0 + sl : -1;
+        nc[k] = P.numSdf > 0 ? -1 : 0;  // a frame starts by visiting every mesh (sdf.cu:226-227)
+        cand[k] = 0;
+        ncol[k] = 0;
+        limEff[k] = INFINITY;
+        if (gi[k] >= 0) {
+            const int i = gi[k];
+            prm[k] = D.prm[i];
+            qr[k] = ldq(D.qrest[i]);
+            const uint32_t fl = flags_of(prm[k]);
+            qrp[k] = (fl & HAS_PREV) ? ldq(D.qrest[i - 1]) : qr[k];
+            rope[k] = D.ropeOf[i];
+            isRoot[k] = D.roff[rope[k]] == i;
+            f3 x = ld3(D.x[i]);
+            q4 q = ldq(D.q[i]);
+            vel[k] = ld3(D.v[i]);
+            vl[k] = ld3(D.vlast[i]);
+            if (Dr.mode == DRIVE_CONST && isRoot[k]) {
+                const int e = rope[k];
+                x = mk3(Dr.p0[3 * e], Dr.p0[3 * e + 1], Dr.p0[3 * e + 2]);
+                vel[k] = mk3(Dr.v0[3 * e], Dr.v0[3 * e + 1], Dr.v0[3 * e + 2]);
+                if (Dr.hasQ0) q = qunit_or_identity(q4{Dr.q0[4 * e], Dr.q0[4 * e + 1], Dr.q0[4 * e + 2], Dr.q0[4 * e + 3]});
+            }
+            sX[sl] = x.x;
+            sX[CAP + sl] = x.y;
+            sX[2 * CAP + sl] = x.z;
+            sQA[sl] = stq(q);
+            sFl[sl] = fl;
+        } else {
+            prm[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            rope[k] = 0;
+            isRoot[k] = false;
+            vel[k] = vl[k] = y[k] = mk3(0.f, 0.f, 0.f);
+            qr[k] = qrp[k] = q4{0.f, 0.f, 0.f, 1.f};
+            sX[sl] = sX[CAP + sl] = sX[2 * CAP + sl] = 0.f;
+            sQA[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
+            sFl[sl] = 0u;
+        }
+    }
+    float* dCur = sDA;
+    float* dNxt = sDB;
+    float4* qCur = sQA;
+    float4* qNxt = sQB;
+    const float h = P.h;
+
+    for (int s = 0; s < S; s++) {
+        // root program of this substep and kinematic mesh poses
+        if (Dr.mode == DRIVE_PROGRAM) {
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                if (gi[k] >= 0 && isRoot[k]) {
+                    const int sl = t + k * BLOCK;
+                    f3 x, v;
+                    q4 q;
+                    bool setQ;
+                    root_program(Dr, rope[k], s, S, h, x, v, q, setQ);
+                    sX[sl] = x.x;
+                    sX[CAP + sl] = x.y;
+                    sX[2 * CAP + sl] = x.z;
+                    vel[k] = v;
+                    if (setQ) qCur[sl] = stq(q);
+                }
+            }
+        }
+        if (P.numSdf > 0 && P.sdfMoving) {
+            for (int idx = t; idx < C.nropes * P.numSdf; idx += BLOCK) {
+                const int r = C.rope0 + idx / P.numSdf, m = idx % P.numSdf;
+                const size_t slot = (size_t)r * MAX_SDF + m;
+                store_pose(D.poses + slot, substep_pose(D.motion[slot], D.grids[m], s, S, h));
+            }
+        }
+        // init
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            if (gi[k] >= 0) {
+                const int sl = t + k * BLOCK;
+                f3 d;
+                float padding;
+                init_vertex(P, prm[k].x, vel[k], vl[k], d, y[k], padding);
+                vl[k] = vel[k];
+                dCur[sl] = d.x;
+                dCur[CAP + sl] = d.y;
+                dCur[2 * CAP + sl] = d.z;
+                sPad[sl] = padding;
+            }
+        }
+        __syncthreads();
+
+        if (P.ropeContact) {
+            // per-chunk boxes of 16 consecutive segments, padded by each segment's detection radius
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int sl = t + k * BLOCK;
+                f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = mk3(-INFINITY, -INFINITY, -INFINITY);
+                if (gi[k] >= 0 && (flags_of(prm[k]) & HAS_NEXT)) {
+                    const f3 a = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+                    const f3 b = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
+                    const float r = fmaxf(sPad[sl], sPad[sl + 1]);
+                    lo = vmin(a, b) - mk3(r, r, r);
+                    hi = vmax(a, b) + mk3(r, r, r);
+                }
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) {
+                    lo = vmin(lo, mk3(__shfl_xor(lo.x, o), __shfl_xor(lo.y, o), __shfl_xor(lo.z, o)));
+                    hi = vmax(hi, mk3(__shfl_xor(hi.x, o), __shfl_xor(hi.y, o), __shfl_xor(hi.z, o)));
+                }
+                if ((t & 15) == 0) {
+                    const int ch = sl >> 4;
+                    sBox[ch] = lo.x;
+                    sBox[NCH + ch] = lo.y;
+                    sBox[2 * NCH + ch] = lo.z;
+                    sBox[3 * NCH + ch] = hi.x;
+                    sBox[4 * NCH + ch] = hi.y;
+                    sBox[5 * NCH + ch] = hi.z;
+                }
+            }
+            __syncthreads();
+            // exact tests in the chunks whose box meets the segment's own padded box (conservative:
+            // the exact test decides membership; lists stay in ascending partner order)
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int sl = t + k * BLOCK;
+                float minDist = INFINITY;
+                ncol[k] = 0;
+                if (gi[k] >= 0 && (flags_of(prm[k]) & HAS_NEXT)) {
+                    const int i = gi[k];
+                    const int G = P.groupRopes;
+                    const int g = rope[k] / G;
+                    const int g0 = D.roff[min(g * G, P.nropes)] - C.v0, g1 = D.roff[min((g + 1) * G, P.nropes)] - C.v0;
+                    const int rs0 = D.roff[rope[k]] - C.v0, rs1 = D.roff[rope[k] + 1] - C.v0;
+                    const f3 a0 = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+                    const f3 x1 = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
+                    const f3 a1 = x1 - a0;
+                    const float ri = fmaxf(sPad[sl], sPad[sl + 1]);
+                    minDist = sPad[sl] - P.detR;
+                    const float mg = ri + 1e-4f + 2e-6f * fmaxf(fabsf(a0.x), fmaxf(fabsf(a0.y), fabsf(a0.z)));
+                    const f3 lo = vmin(a0, x1) - mk3(mg, mg, mg), hi = vmax(a0, x1) + mk3(mg, mg, mg);
+                    int cnt = 0;
+                    int hits = 0;
+                    for (int ch = g0 >> 4; ch <= (g1 - 1) >> 4; ch++) {
+                        if (sBox[ch] > hi.x || sBox[NCH + ch] > hi.y || sBox[2 * NCH + ch] > hi.z ||
+                            sBox[3 * NCH + ch] < lo.x || sBox[4 * NCH + ch] < lo.y || sBox[5 * NCH + ch] < lo.z)
+                            continue;
+                        hits++;
+                        const int jb = max(ch << 4, g0), je = min((ch << 4) + 16, g1);
+                        for (int j = jb; j < je; j++) {
+                            if (!(sFl[j] & HAS_NEXT)) continue;
+                            if (j >= rs0 && j < rs1 && abs(j - sl) <= 2) continue;
+                            const f3 b0 = mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]) - a0;
+                            const f3 b1 = mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]) - a0;
+                            const float r = ri + fmaxf(sPad[j], sPad[j + 1]);
+                            float d2;
+                            if (pair_hit(a1, b0, b1, r, d2)) {
+                                if (cnt < MAX_COLL) {
+                                    D.coll[(size_t)i * MAX_COLL + cnt] = C.v0 + j;
+                                    minDist = fminf(minDist, 0.45f * sqrtf(d2));
+                                }
+                                cnt++;
+                            }
+                        }
+                    }
+                    if (chunkHits) atomicAdd(chunkHits, hits);
+                    if (cnt > MAX_COLL) D.err[0] = ERR_MAX_COLLISIONS;
+                    ncol[k] = min(cnt, MAX_COLL);
+                }
+                sLim[sl] = minDist;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int sl = t + k * BLOCK;
+                if (gi[k] >= 0) {
+                    limEff[k] = sLim[sl];
+                    if (flags_of(prm[k]) & HAS_PREV) limEff[k] = fminf(limEff[k], sLim[sl - 1]);
+                }
+            }
+        }
+
+        // numItr x (position sweep, orientation sweep)
+        for (int it = 0; it < P.numItr; it++) {
+            f3 fk[VPT];
+            s3 Hk[VPT];
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int sl = t + k * BLOCK;
+                f3 f2 = mk3(0.f, 0.f, 0.f);
+                s3 H2 = szero();
+                fk[k] = f2;
+                Hk[k] = H2;
+                if (gi[k] >= 0) {
+                    const uint32_t fl = flags_of(prm[k]);
+                    const f3 p0 = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+                    const f3 d0 = mk3(dCur[sl], dCur[CAP + sl], dCur[2 * CAP + sl]);
+                    f3 p1 = p0, p1dx = d0;
+                    if (fl & HAS_NEXT) {
+                        p1 = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
+                        p1dx = mk3(dCur[sl + 1], dCur[CAP + sl + 1], dCur[2 * CAP + sl + 1]);
+                    }
+                    vertex_terms(P, D.grids, D.poses + (size_t)rope[k] * MAX_SDF, nc[k], cand[k], prm[k].x, prm[k].y,
+                                 prm[k].z, fl, p0, d0, y[k], ldq(qCur[sl]), p1, p1dx, fk[k], Hk[k], f2, H2);
+                    if (P.ropeContact && (fl & HAS_NEXT)) {
+                        const float dmp = P.damping / h;
+                        const int* lst = D.coll + (size_t)gi[k] * MAX_COLL;
+                        for (int c = 0; c < ncol[k]; c++) {
+                            const int j = lst[c] - C.v0;
+                            collision_term(P, dmp, p0, p1, d0, p1dx, mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]),
+                                           mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]),
+                                           mk3(dCur[j], dCur[CAP + j], dCur[2 * CAP + j]),
+                                           mk3(dCur[j + 1], dCur[CAP + j + 1], dCur[2 * CAP + j + 1]), fk[k], Hk[k], f2,
+                                           H2);
+                        }
+                    }
+                }
+                sF[sl] = f2.x;
+                sF[CAP + sl] = f2.y;
+                sF[2 * CAP + sl] = f2.z;
+#pragma unroll
+                for (int c = 0; c < 6; c++) sF[(3 + c) * CAP + sl] = H2.m[c];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int sl = t + k * BLOCK;
+                if (gi[k] >= 0) {
+                    const uint32_t fl = flags_of(prm[k]);
+                    f3 f = fk[k];
+                    s3 H = Hk[k];
+                    if (fl & HAS_PREV) {
+                        f = f + mk3(sF[sl - 1], sF[CAP + sl - 1], sF[2 * CAP + sl - 1]);
+                        s3 Hp;
+#pragma unroll
+                        for (int c = 0; c < 6; c++) Hp.m[c] = sF[(3 + c) * CAP + sl - 1];
+                        sadd(H, Hp);
+                    }
+                    f3 d = mk3(dCur[sl], dCur[CAP + sl], dCur[2 * CAP + sl]);
+                    if (prm[k].x != 0.f) d = solve_update(P, d, H, f, P.ropeContact != 0, limEff[k]);
+                    dNxt[sl] = d.x;
+                    dNxt[CAP + sl] = d.y;
+                    dNxt[2 * CAP + sl] = d.z;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int sl = t + k * BLOCK;
+                if (gi[k] >= 0) {
+                    const uint32_t fl = flags_of(prm[k]);
+                    const q4 q0 = ldq(qCur[sl]);
+                    if (!(fl & FIX_ORIENT) && (fl & HAS_NEXT)) {
+                        const q4 qp = (fl & HAS_PREV) ? ldq(qCur[sl - 1]) : q0;
+                        const q4 qn = (fl & HAS_NEXT_ORIENT) ? ldq(qCur[sl + 1]) : q0;
+                        qNxt[sl] = stq(orient_update(
+                            fl, prm[k].y, prm[k].z, mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
+                            mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]),
+                            mk3(dNxt[sl], dNxt[CAP + sl], dNxt[2 * CAP + sl]),
+                            mk3(dNxt[sl + 1], dNxt[CAP + sl + 1], dNxt[2 * CAP + sl + 1]), qp, q0, qn, qrp[k], qr[k]));
+                    } else {
+                        qNxt[sl] = stq(q0);
+                    }
+                }
+            }
+            float* td = dCur;
+            dCur = dNxt;
+            dNxt = td;
+            float4* tq = qCur;
+            qCur = qNxt;
+            qNxt = tq;
+        }
+        __syncthreads();
+
+        // end of substep
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            if (gi[k] >= 0) {
+                const int sl = t + k * BLOCK;
+                const uint32_t fl = flags_of(prm[k]);
+                const f3 d = mk3(dCur[sl], dCur[CAP + sl], dCur[2 * CAP + sl]);
+                const f3 x = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+                const Pose* envPoses = D.poses + (size_t)rope[k] * MAX_SDF;
+                if (prm[k].x != 0.f) {
+                    f3 tg = mk3(1.f, 0.f, 0.f);
+                    if (P.dragQuad > 0.f) tg = qrot(ldq((fl & HAS_NEXT) ? qCur[sl] : qCur[sl - 1]), mk3(1.f, 0.f, 0.f));
+                    vel[k] = end_velocity(P, d, tg);
+                    if (P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0)) {
+                        const float kap = P.cKappa / h;
+                        const f3 fv = surface_readout(P, D.grids, envPoses, D.wrench + (size_t)rope[k] * MAX_SDF * 6,
+                                                      nc[k], cand[k], x, d, kap * kap / prm[k].x);
+                        float* fo = D.force + 3 * (size_t)gi[k];
+                        fo[0] = fo[0] + fv.x;
+                        fo[1] = fo[1] + fv.y;
+                        fo[2] = fo[2] + fv.z;
+                    }
+                }
+                const f3 xn = x + d;
+                sX[sl] = xn.x;
+                sX[CAP + sl] = xn.y;
+                sX[2 * CAP + sl] = xn.z;
+                if (P.numSdf > 0) {
+                    if (prm[k].x != 0.f)
+                        nc[k] = build_candidates(P, envPoses, xn, 2.f * len(d) + 0.02f, cand[k]);
+                    else
+                        nc[k] = 0;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // write the state back, fill the views
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        if (gi[k] >= 0) {
+            const int sl = t + k * BLOCK;
+            const int i = gi[k];
+            const f3 x = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+            const float4 q = qCur[sl];
+            D.x[i] = st3(x);
+            D.v[i] = st3(vel[k]);
+            D.vlast[i] = st3(vl[k]);
+            D.q[i] = q;
+            D.outPos[3 * i] = x.x;
+            D.outPos[3 * i + 1] = x.y;
+            D.outPos[3 * i + 2] = x.z;
+            D.outVel[3 * i] = vel[k].x;
+            D.outVel[3 * i + 1] = vel[k].y;
+            D.outVel[3 * i + 2] = vel[k].z;
+            D.outQuat[4 * i] = q.x;
+            D.outQuat[4 * i + 1] = q.y;
+            D.outQuat[4 * i + 2] = q.z;
+            D.outQuat[4 * i + 3] = q.w;
+            if (P.numSdf > 0) {
+                D.ncand[i] = nc[k];
+                for (int c = 0; c < MAX_CAND; c++)
+                    D.cand[(size_t)i * MAX_CAND + c] = (nc[k] > 0 && c < nc[k]) ? cand_get(cand[k], c) : -1;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ host launchers
+
+static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+hipError_t launch_fill_f32(float* p, float v, long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblk(n, 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+hipError_t launch_scale_f32(float* p, float v, long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale_f32, dim3(nblk(n, 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+hipError_t launch_root_const(const Params& P, const Dev& D, const Drive& Dr, hipStream_t s) {
+    hipLaunchKernelGGL(k_root_const, dim3(nblk(P.nropes, 128)), dim3(128), 0, s, P, D, Dr);
+    return hipGetLastError();
+}
+hipError_t launch_gather(const Params& P, const Dev& D, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather, dim3(nblk(P.nv, 256)), dim3(256), 0, s, P, D);
+    return hipGetLastError();
+}
+hipError_t launch_reset(const Params& P, const Dev& D, const float4* initX, const float4* initQ, const int* ids,
+                        int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reset, dim3(count), dim3(256), 0, s, P, D, initX, initQ, ids, count);
+    return hipGetLastError();
+}
+
+// One substep of the per-substep method. dx / q ping-pong through D.dx/D.dx2 and D.q/D.q2; on return
+// D.q holds the current frames (the pointers in D are swapped as needed).
+hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s, int S, hipStream_t st) {
+    const int nvb = nblk(P.nv, 256);
+    if (Dr.mode == DRIVE_PROGRAM)
+        hipLaunchKernelGGL(k_root_program, dim3(nblk(P.nropes, 128)), dim3(128), 0, st, P, D, Dr, D.q, s, S);
+    const int initThreads = P.sdfMoving ? max(P.nv, P.nropes * MAX_SDF) : P.nv;
+    hipLaunchKernelGGL(k_init, dim3(nblk(initThreads, 256)), dim3(256), 0, st, P, D, s, S, D.dx);
+    if (P.ropeContact) hipLaunchKernelGGL(k_detect, dim3(nblk(P.nv, 128)), dim3(128), 0, st, P, D);
+    float4* dIn = D.dx;
+    float4* dOut = D.dx2;
+    float4* qIn = D.q;
+    float4* qOut = D.q2;
+    const int posBlocks = nblk(P.nv + 1, PLAIN_OWN);
+    for (int it = 0; it < P.numItr; it++) {
+        if (P.ropeContact)
+            hipLaunchKernelGGL(k_pos<true>, dim3(posBlocks), dim3(PLAIN_BLOCK), 0, st, P, D, dIn, qIn, dOut);
+        else
+            hipLaunchKernelGGL(k_pos<false>, dim3(posBlocks), dim3(PLAIN_BLOCK), 0, st, P, D, dIn, qIn, dOut);
+        hipLaunchKernelGGL(k_orient, dim3(nvb), dim3(256), 0, st, P, D, dOut, qIn, qOut);
+        float4* t = dIn;
+        dIn = dOut;
+        dOut = t;
+        t = qIn;
+        qIn = qOut;
+        qOut = t;
+    }
+    hipLaunchKernelGGL(k_end, dim3(nvb), dim3(256), 0, st, P, D, dIn, qIn);
+    if (qIn != D.q) {
+        D.q2 = D.q;
+        D.q = qIn;
+    }
+    return hipGetLastError();
+}
+
+size_t cell_lds_bytes(int block, int vpt) {
+    const size_t cap = (size_t)block * vpt;
+    return cap * (3 + 3 + 3 + 4 + 4 + 9 + 1 + 1 + 1) * sizeof(float) + (cap / 16) * 6 * sizeof(float);
+}
+
+template <int B, int V>
+static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
+                                int ncells, int S, int* chunkHits, hipStream_t st) {
+    const size_t lds = cell_lds_bytes(B, V);
+    static bool attrSet = false;
+    if (!attrSet) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cell<B, V>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attrSet = true;
+    }
+    hipLaunchKernelGGL((k_cell<B, V>), dim3(ncells), dim3(B), lds, st, P, D, Dr, cells, order, S, chunkHits);
+    return hipGetLastError();
+}
+
+const CellVariant kCellVariants[] = {{64, 1}, {128, 1}, {256, 1}, {512, 1}, {1024, 1}, {256, 2}, {256, 4}, {512, 2}};
+const int kNumCellVariants = sizeof(kCellVariants) / sizeof(kCellVariants[0]);
+
+hipError_t launch_cell(int variant, const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
+                       int ncells, int S, int* chunkHits, hipStream_t st) {
+    if (ncells <= 0) return hipSuccess;
+    switch (variant) {
+        case 0: return launch_cell_t<64, 1>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
+        case 1: return launch_cell_t<128, 1>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
+        case 2: return launch_cell_t<256, 1>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
+        case 3: return launch_cell_t<512, 1>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
+        case 4: return launch_cell_t<1024, 1>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
+        case 5: return launch_cell_t<256, 2>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
+        case 6: return launch_cell_t<256, 4>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
+        case 7: return launch_cell_t<512, 2>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_scatter_targets(Pose7* tgt, uint8_t* tele, int index, const float* pos, const float* q, int rows,
+                                  int M, const uint8_t* teleFlags, int teleAll, int N, hipStream_t s) {
+    const int threads = index < 0 ? N * M : N;
+    if (threads <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_targets, dim3(nblk(threads, 256)), dim3(256), 0, s, tgt, tele, index, pos, q, rows, M,
+                       teleFlags, teleAll, N);
+    return hipGetLastError();
+}
+hipError_t launch_stage_motion(Pose7* cur, const Pose7* tgt, uint8_t* tele, Motion* motion, int total, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_motion, dim3(nblk(total, 256)), dim3(256), 0, s, cur, tgt, tele, motion, total);
+    return hipGetLastError();
+}
+hipError_t launch_commit(Pose7* cur, const Pose7* tgt, uint8_t* tele, Pose* poses, const Grid* grids, int numSdf,
+                         int index, int N, hipStream_t s) {
+    const int threads = index >= 0 ? N : N * MAX_SDF;
+    hipLaunchKernelGGL(k_commit, dim3(nblk(threads, 256)), dim3(256), 0, s, cur, tgt, tele, poses, grids, numSdf, index,
+                       N);
+    return hipGetLastError();
+}
+hipError_t launch_check_detect(const Params& P, const Dev& D, hipStream_t s) {
+    hipLaunchKernelGGL(k_detect, dim3(nblk(P.nv, 128)), dim3(128), 0, s, P, D);
+    return hipGetLastError();
+}
+hipError_t launch_check_broadphase(const Params& P, const Dev& D, const float4* xs, const float4* dxs, int* counts,
+                                   int* cand, hipStream_t s) {
+    hipLaunchKernelGGL(k_check_broadphase, dim3(nblk(P.nv, 256)), dim3(256), 0, s, P, D, xs, dxs, counts, cand);
+    return hipGetLastError();
+}
+
+}  // namespace rs

@@ -1,0 +1,36 @@
+// Host-side launchers of the rope kernels (rs_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rs_types.hpp"
+
+namespace rs {
+
+struct CellVariant {
+    int block, vpt;  // workgroup size and vertex slots per thread: a cell holds up to block * vpt vertices
+};
+extern const CellVariant kCellVariants[];
+extern const int kNumCellVariants;
+size_t cell_lds_bytes(int block, int vpt);
+
+hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s);
+hipError_t launch_fill_f32(float* p, float v, long n, hipStream_t s);
+hipError_t launch_scale_f32(float* p, float v, long n, hipStream_t s);
+hipError_t launch_root_const(const Params& P, const Dev& D, const Drive& Dr, hipStream_t s);
+hipError_t launch_gather(const Params& P, const Dev& D, hipStream_t s);
+hipError_t launch_reset(const Params& P, const Dev& D, const float4* initX, const float4* initQ, const int* ids,
+                        int count, hipStream_t s);
+hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s, int S, hipStream_t st);
+hipError_t launch_cell(int variant, const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
+                       int ncells, int S, int* chunkHits, hipStream_t st);
+hipError_t launch_scatter_targets(Pose7* tgt, uint8_t* tele, int index, const float* pos, const float* q, int rows,
+                                  int M, const uint8_t* teleFlags, int teleAll, int N, hipStream_t s);
+hipError_t launch_stage_motion(Pose7* cur, const Pose7* tgt, uint8_t* tele, Motion* motion, int total, hipStream_t s);
+hipError_t launch_commit(Pose7* cur, const Pose7* tgt, uint8_t* tele, Pose* poses, const Grid* grids, int numSdf,
+                         int index, int N, hipStream_t s);
+hipError_t launch_check_detect(const Params& P, const Dev& D, hipStream_t s);
+hipError_t launch_check_broadphase(const Params& P, const Dev& D, const float4* xs, const float4* dxs, int* counts,
+                                   int* cand, hipStream_t s);
+
+}  // namespace rs

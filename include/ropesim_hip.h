@@ -1,0 +1,175 @@
+/* C ABI of the MI355X rope solver (librope_hip.so).
+ *
+ * The drop-in boundary for the reference's RopeBatch / RopeSim hot path. Plain pointers and sizes
+ * only; "device" pointers are HIP device addresses (for example a torch tensor's data_ptr()),
+ * "host" pointers are ordinary host memory. Every entry point returns an rs_status; the message of
+ * the last failure on the calling thread is rs_last_error_message().
+ *
+ * Reference interfaces each entry point replaces are cited as path:line in the reference repository
+ * (csrc/ = the ropesim extension, YB = third_party/yarnball/KittenEngine/YarnBall).
+ */
+#ifndef ROPESIM_HIP_H
+#define ROPESIM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RS_OK = 0,
+    RS_INVALID = 1,  /* bad argument or call order -> ValueError */
+    RS_DEVICE = 2,   /* HIP runtime failure -> RuntimeError */
+    RS_SOLVER = 3    /* solver error (rope-rope candidate overflow) -> RuntimeError */
+} rs_status;
+
+typedef struct rs_batch rs_batch;
+
+/* Construction input: ropes as a CSR over flat vertices (uneven lengths allowed). */
+typedef struct {
+    int num_ropes;
+    const int* offsets;          /* host [num_ropes + 1], offsets[0] == 0, every rope >= 3 vertices */
+    const float* positions;      /* host [offsets[num_ropes] * 3] initial (and rest) shape */
+    const unsigned char* pinned; /* host [num_vertices] kinematic vertices, or NULL = each rope's vertex 0 */
+    float density;               /* rho [kg/m] */
+    float k_stretch;             /* EA [N] */
+    float k_bend;                /* EI [N m^2], > 0 */
+    float max_h;                 /* seeds the previous substep length of the first frame */
+    int num_itr;
+    int fix_root_orientation;    /* 1: every rope's vertex-0 frame is kinematic */
+    int straight_rest;           /* 1: straight untwisted rest shape */
+    float min_seg;               /* > 0: length that scales the inactive rope-rope radii (uniform
+                                    constructor); <= 0: the shortest given segment */
+} rs_desc;
+
+/* Solver knobs (rs_get_param / rs_set_param). */
+typedef enum {
+    RS_P_DRAG = 0,        /* linear drag [1/s] */
+    RS_P_DAMPING = 1,
+    RS_P_DRAG_QUAD = 2,   /* quadratic normal drag [1/m], >= 0 */
+    RS_P_DRAG_QUAD_T = 3, /* quadratic tangential drag [1/m], >= 0 */
+    RS_P_NUM_ITR = 4,
+    RS_P_ACCEL = 5,       /* acceleration (over-relaxation) ratio */
+    RS_P_MAX_H = 6,
+    RS_P_TIME = 7,        /* read only */
+    RS_P_STEPPED = 8      /* read only: 1 after the first step */
+} rs_param;
+
+/* Host-side per-vertex arrays (rs_get_vertex / rs_set_vertex), all [num_vertices * width]. */
+typedef enum {
+    RS_V_POSITIONS = 0,    /* width 3 */
+    RS_V_VELOCITIES = 1,   /* width 3 */
+    RS_V_QUATERNIONS = 2,  /* width 4, (x, y, z, w) */
+    RS_V_INV_MASSES = 3,   /* width 1 (set: raw inverse masses, 0 = kinematic) */
+    RS_V_REST_LENGTHS = 4, /* width 1 (entry at a rope's last vertex is padding) */
+    RS_V_MASSES = 5,       /* width 1, set only (pinned vertices ignored) */
+    RS_V_EA = 6,           /* width 1 per segment at its first vertex, set only */
+    RS_V_EI = 7,           /* width 1 per segment at its first vertex, set only */
+    RS_V_LAST_VELOCITIES = 8 /* width 3, read only */
+} rs_vertex_field;
+
+typedef struct {
+    float* positions;      /* device [num_vertices * 3] */
+    float* velocities;     /* device [num_vertices * 3] */
+    float* quaternions;    /* device [num_vertices * 4] */
+    float* contact_forces; /* device [num_vertices * 3] or NULL before rs_enable_contact_forces */
+    float* wrenches;       /* device [num_ropes * 64 * 6] or NULL */
+    int num_vertices, num_ropes;
+} rs_views_t;
+
+const char* rs_last_error_message(void);
+int rs_abi_version(void);
+
+/* RopeBatch ctor / from_positions (csrc/batch.cu:193-276; csrc/bindings.cpp:735-767, 889-934). */
+int rs_create(const rs_desc* desc, rs_batch** out);
+void rs_destroy(rs_batch* b);
+int rs_num_vertices(const rs_batch* b);
+int rs_num_ropes(const rs_batch* b);
+int rs_offsets(const rs_batch* b, int* out); /* host [num_ropes + 1] */
+
+/* Knobs (csrc/bindings.cpp:1161-1191 properties; YB/YarnBall.h:103-192). */
+int rs_get_param(const rs_batch* b, int which, double* out);
+int rs_set_param(rs_batch* b, int which, double value);
+int rs_set_gravity(rs_batch* b, const float* g3);
+int rs_get_gravity(const rs_batch* b, float* g3);
+
+/* Per-vertex host access (csrc/bindings.cpp:1121-1201 setters; RopeSim accessors :354-400). Setters of
+ * masses / EA / EI are legal at any time here; RopeBatch's Python layer refuses them after the first step as the
+ * reference does. */
+int rs_get_vertex(rs_batch* b, int which, float* host_out);
+int rs_set_vertex(rs_batch* b, int which, const float* host_in);
+/* Overwrite one rope's root position / velocity / frame from the host (RopeSim.set_root,
+ * csrc/bindings.cpp:318-350). q may be NULL. */
+int rs_set_root_host(rs_batch* b, int rope, const float* pos3, const float* vel3, const float* q4);
+
+/* One control frame (csrc/batch.cu:288-398). Device arrays [num_ropes * 3] / [num_ropes * 4]; q pointers
+ * may be NULL. step_const: constant-velocity sweep; step_program: quadratic position program to p1 with
+ * SLERP q0 -> q1; step_free: no root drive. Inputs are consumed before the call returns (staged). */
+int rs_step_const(rs_batch* b, const float* pos, const float* vel, const float* q, float dt, int substeps);
+int rs_step_program(rs_batch* b, const float* p0, const float* v0, const float* p1, const float* q0, const float* q1,
+                    float dt, int substeps);
+int rs_step_free(rs_batch* b, float dt, int substeps);
+
+/* RopeBatch.reset_envs / gather (csrc/batch.cu:400-425). ids are host ints. */
+int rs_reset_envs(rs_batch* b, const int* ids, int count);
+int rs_gather(rs_batch* b);
+int rs_views(rs_batch* b, rs_views_t* out);
+
+/* Surface contact (csrc/bindings.cpp:51-80 write_contact / write_planes). planes host [n * 4]. */
+int rs_set_planes(rs_batch* b, int n, const float* planes, float radius, float mu_s, float mu_k, float v_slip,
+                  float kappa);
+int rs_set_contact(rs_batch* b, float radius, float mu_s, float mu_k, float v_slip, float kappa);
+int rs_get_contact(const rs_batch* b, float* radius_mus_muk_vslip_kappa5);
+
+/* Rope-rope contact (csrc/batch.cu:367-385). */
+int rs_set_rope_contact(rs_batch* b, int enabled, float radius, float barrier, float k_collision, float mu,
+                        float k_friction, float detection_scaler, int group_ropes);
+
+/* Meshes (csrc/sdf.cu:137-296). values host [nx*ny*nz] C order; pos/quat host rows (1 or num_ropes). */
+int rs_add_mesh(rs_batch* b, const float* values, int nx, int ny, int nz, const float* origin3, float spacing,
+                float mu_s, float mu_k, const float* pos, const float* quat, int rows, int* index_out);
+int rs_set_mesh_friction(rs_batch* b, int index, float mu_s, float mu_k);
+/* Targets reached at the END of the next step. index < 0: every mesh from dense [num_ropes, M] rows.
+ * on_device: 1 when pos/quat/tele are device pointers. tele may be NULL; tele_all replaces every pose. */
+int rs_set_mesh_targets(rs_batch* b, int index, const float* pos, const float* quat, int rows,
+                        const unsigned char* tele, int tele_all, int on_device);
+int rs_clear_meshes(rs_batch* b);
+int rs_num_meshes(const rs_batch* b);
+int rs_enable_contact_forces(rs_batch* b, int on);
+int rs_contact_forces_host(rs_batch* b, float* out); /* [num_vertices * 3] averaged over the last step */
+int rs_mesh_wrenches_host(rs_batch* b, float* out);  /* [num_ropes * num_meshes * 6] */
+
+/* RopeSim.reset bookkeeping (csrc/bindings.cpp:303-316): time 0, previous substep = max_h, codes cleared. */
+int rs_reset_clock(rs_batch* b);
+
+/* last_error / last_warning codes (YB/YarnBall.h:204-208). */
+int rs_error_codes(const rs_batch* b, int* last_error, int* last_warning);
+int rs_clear_error_codes(rs_batch* b);
+
+/* Dispatch: mode 0 = one launch per kernel per substep, mode 1 = one persistent kernel per frame
+ * (variant -1: autotuned on the first frame). No counterpart in the reference. */
+int rs_set_dispatch(rs_batch* b, int mode, int variant);
+int rs_get_dispatch(const rs_batch* b, int* mode, int* variant);
+int rs_num_cell_variants(void);
+int rs_cell_variant_info(int variant, int* block, int* vpt, int* supported);
+/* Time one frame per supported variant on the current state (restored afterwards) and keep the fastest. */
+int rs_autotune(rs_batch* b, float dt, int substeps, int program, float* best_ms);
+
+/* Function-level checks, no counterpart in the reference: run the device rope-rope detection on given
+ * substep-start positions and paddings (host [nv*3], [nv]) -> counts [nv], lists [nv*128] (-1 padded),
+ * step limits [nv]; and the mesh broad phase on given positions / displacements -> counts [nv],
+ * candidates [nv*8] (-1 padded) with the current poses. */
+int rs_check_detect(rs_batch* b, const float* pos, const float* pads, int* counts, int* lists, float* max_step);
+int rs_check_broadphase(rs_batch* b, const float* pos, const float* dx, int* counts, int* cand);
+/* Timing of the persistent frame kernel with HIP events on the solver stream (no counterpart in the
+ * reference): rs_kernel_time returns the summed duration and count of the frame kernels launched since
+ * the last query, synchronising on them. */
+int rs_set_kernel_timing(rs_batch* b, int on);
+int rs_kernel_time(rs_batch* b, double* total_ms, int* launches);
+/* Rope-rope chunk-pair box hits counted during the next frames (0 disables). Statistic only. */
+int rs_set_chunk_stats(rs_batch* b, int on, long long* hits_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROPESIM_HIP_H */

@@ -1,0 +1,48 @@
+"""Shared test setup: import paths, the `gpu` marker, and small helpers.
+
+`-m "not gpu"` runs the checker (CPU oracle) against analytic answers and golden fixtures, the host
+logic and the C-ABI symbol table; `-m gpu` runs the HIP path against the checker on a real GPU.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "cosserat-rod-solver-cuda_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP GPU and the built extension")
+    config.addinivalue_line("markers", "slow: long-running")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import pyoracle
+    pyoracle.lib()
+    return pyoracle
+
+
+@pytest.fixture(scope="session")
+def rs():
+    """The HIP package; on a GPU box a missing extension is an error, never a fallback."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("gpu test without a GPU")
+    import ropesim
+    return ropesim
+
+
+def to_dev(a):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a, np.float32), device="cuda")
+
+
+def host(view):
+    import torch
+    return torch.as_tensor(view).cpu().numpy()

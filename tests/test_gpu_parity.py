@@ -1,0 +1,232 @@
+"""HIP path vs the CPU oracle on identical seeded inputs (parity tests proper).
+
+Bars: positions within 1e-5 of rope length of the float64 oracle after one frame (and per frame for the
+single-rope config); rope-rope candidate lists, step limits' inputs and mesh broad-phase lists bit-exact
+given identical positions; the per-substep kernels and the persistent frame kernel bit-identical.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import host, to_dev
+
+pytestmark = pytest.mark.gpu
+
+TOL_L = 1e-5  # of rope length
+
+
+def _lib(rs):
+    return C.CDLL(os.path.join(os.path.dirname(rs.__file__), "librope_hip.so"))
+
+
+def build_pair(rs, oracle, scene, prec=64, mode=1):
+    b = scene.build(rs)
+    b.set_dispatch(mode)
+    o = oracle.OracleBatch(scene.positions, scene.density, scene.k_stretch, scene.k_bend, scene.max_h, scene.num_itr,
+                           pinned=scene.pinned, fix_root_orientation=scene.fix_root_orientation, prec=prec,
+                           offsets=scene.offsets)
+    scene.apply(o)
+    return b, o
+
+
+def rope_lengths(scene):
+    p = scene.positions.astype(np.float64)
+    L = np.zeros(scene.num_ropes)
+    for r in range(scene.num_ropes):
+        a, e = scene.offsets[r], scene.offsets[r + 1]
+        L[r] = np.linalg.norm(np.diff(p[a:e], axis=0), axis=1).sum()
+    return L
+
+
+def rel_err(scene, x_gpu, x_ref):
+    """Max over ropes of (max vertex position error / rope length)."""
+    L = rope_lengths(scene)
+    err = np.linalg.norm(x_gpu.reshape(-1, 3) - x_ref.reshape(-1, 3), axis=1)
+    rope = np.repeat(np.arange(scene.num_ropes), np.diff(scene.offsets))
+    per = np.zeros(scene.num_ropes)
+    np.maximum.at(per, rope, err)
+    return float((per / L).max())
+
+
+def test_views_are_device_arrays_sharing_memory(rs):
+    import torch
+    from ropesim import scenes
+    s = scenes.config2(n_ropes=2, V=10, frames=1)
+    b = s.build(rs)
+    v = b.positions()
+    t = torch.as_tensor(v, device="cuda")
+    assert t.is_cuda and t.dtype == torch.float32 and tuple(t.shape) == (2, 10, 3)
+    assert t.data_ptr() == v.ptr
+    t0 = t.clone()
+    fi = s.frame_inputs(1)[0]
+    s.step(b, fi, to_dev)
+    torch.cuda.synchronize()
+    assert not torch.equal(t, t0)
+    q = torch.as_tensor(b.quaternions(), device="cuda")
+    n = torch.linalg.vector_norm(q[:, :-1, :], dim=-1)
+    assert torch.allclose(n, torch.ones_like(n), atol=1e-5)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_config1_every_frame_within_1e5_of_length(rs, oracle, mode):
+    from ropesim import scenes
+    s = scenes.config1(frames=60)
+    b, o = build_pair(rs, oracle, s, 64, mode)
+    L = rope_lengths(s)[0]
+    worst = 0.0
+    for f, fi in enumerate(s.frame_inputs()):
+        s.step(b, fi, to_dev)
+        s.step(o, fi)
+        e = np.abs(host(b.positions()).reshape(-1, 3) - o.positions()).max() / L
+        worst = max(worst, e)
+        assert e < TOL_L, f"frame {f}: {e:.3e} of rope length"
+    print(f"\n  config1 mode {mode}: worst per-frame error {worst:.2e} of rope length")
+
+
+def test_config1_fp32_oracle_tracks_device(rs, oracle):
+    """Same float32 arithmetic in the same order: the device follows the float32 oracle to rounding."""
+    from ropesim import scenes
+    s = scenes.config1(frames=10)
+    b, o = build_pair(rs, oracle, s, 32, 1)
+    for fi in s.frame_inputs():
+        s.step(b, fi, to_dev)
+        s.step(o, fi)
+    d = np.abs(host(b.positions()).reshape(-1, 3) - o.positions()).max()
+    print(f"\n  config1 device vs float32 oracle after 10 frames: {d:.3e} m")
+    assert d < 1e-6
+
+
+def _run(rs, scene, mode, nframes, variant=-1):
+    b = scene.build(rs)
+    b.set_dispatch(mode, variant)
+    for fi in scene.frame_inputs(nframes):
+        scene.step(b, fi, to_dev)
+    return b
+
+
+@pytest.mark.parametrize("which", ["config2", "config3"])
+def test_per_substep_kernels_equal_persistent_kernel_bitwise(rs, which):
+    from ropesim import scenes
+    s = scenes.config2(n_ropes=12, V=65, frames=3) if which == "config2" else \
+        scenes.config3(n_ropes=8, frames=2, vmax=96, grid_n=24)
+    a = _run(rs, s, 0, s.frames)
+    for var in range(rs.num_cell_variants()):
+        blk, vpt, ok = rs.cell_variant_info(var)
+        if not ok or blk * vpt < (96 * 4 if which == "config3" else 65):
+            continue
+        b = _run(rs, s, 1, s.frames, var)
+        for get in ("positions", "velocities", "quaternions"):
+            x, y = host(getattr(a, get)()), host(getattr(b, get)())
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), f"{get} differs (variant {var})"
+
+
+def test_config2_one_frame_vs_fp64_oracle(rs, oracle):
+    from ropesim import scenes
+    s = scenes.config2(n_ropes=16, V=65, frames=2)
+    b, o = build_pair(rs, oracle, s, 64, 1)
+    for f, fi in enumerate(s.frame_inputs()):
+        s.step(b, fi, to_dev)
+        s.step(o, fi)
+        e = rel_err(s, host(b.positions()), o.positions())
+        print(f"\n  config2 frame {f}: {e:.2e} of rope length")
+        assert e < TOL_L
+
+
+def test_config3_one_frame_vs_fp64_oracle(rs, oracle):
+    from ropesim import scenes
+    s = scenes.config3(n_ropes=8, frames=1, vmax=96, grid_n=32)
+    b, o = build_pair(rs, oracle, s, 64, 1)
+    fi = s.frame_inputs(1)[0]
+    s.step(b, fi, to_dev)
+    s.step(o, fi)
+    e = rel_err(s, host(b.positions()), o.positions())
+    print(f"\n  config3 (8 ropes) one frame: {e:.2e} of rope length")
+    assert e < TOL_L
+
+
+def test_program_lands_on_endpoint_and_slerp_end(rs):
+    """The root program lands exactly on p1 and the frame on q1 (csrc/batch.cu:122-159 closed form)."""
+    N, V, H = 3, 10, 1 / 60
+    roots = np.zeros((N, 3), np.float32)
+    roots[:, 0] = np.arange(N) * 1.5
+    b = rs.RopeBatch(num_envs=N, verts_per_rope=V, root_pos=roots, fix_root_orientation=True)
+    b.damping = 0.0
+    p1 = roots + np.array([0.04, -0.02, 0.03], np.float32)
+    v0 = np.tile(np.array([0.5, 0.2, -0.1], np.float32), (N, 1))
+    th = np.deg2rad([10.0, 20.0, 30.0])
+    q0 = np.zeros((N, 4), np.float32)
+    q0[:, 3] = 1
+    q1 = np.zeros((N, 4), np.float32)
+    q1[:, 2] = np.sin(th / 2)
+    q1[:, 3] = np.cos(th / 2)
+    b.step(to_dev(roots), to_dev(v0), H, 50, root_q=to_dev(q0), root_pos_end=to_dev(p1), root_q_end=to_dev(q1))
+    np.testing.assert_allclose(host(b.positions())[:, 0], p1, atol=2e-6)
+    np.testing.assert_allclose(host(b.quaternions())[:, 0], q1, atol=1e-5)
+    a = 2.0 * ((p1 - roots) - v0 * H) / H ** 2
+    np.testing.assert_allclose(host(b.velocities())[:, 0], v0 + a * (H - (H / 50) / 2), rtol=1e-4, atol=1e-4)
+
+
+def _tangle(rng, n_ropes=6, V=40):
+    """Ropes crossing each other inside one small region (many close segment pairs)."""
+    ropes = []
+    for r in range(n_ropes):
+        c = rng.uniform(-0.05, 0.05, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        w = rng.normal(size=3) * 0.02
+        t = np.arange(V)[:, None] * 0.015
+        ropes.append(c + (t - t.mean()) * d + w * np.sin(t * 40))
+    return ropes
+
+
+def test_rope_rope_lists_bitexact_vs_oracle(rs, oracle):
+    rng = np.random.default_rng(7)
+    ropes = _tangle(rng)
+    b = rs.RopeBatch.from_positions(ropes, 0.117, 11473.0, 4.48e-3, 1 / 6000, 8, pinned=None)
+    b.set_rope_contact(True, 0.0036, 0.0008, 0.025, 0.4, 1e7, 1.2, 3)
+    o = oracle.OracleBatch(ropes, 0.117, 11473.0, 4.48e-3, 1 / 6000, 8, prec=32)
+    o.set_rope_contact(True, 0.0036, 0.0008, 0.025, 0.4, 1e7, 1.2, 3)
+    nv = b.num_vertices
+    x = np.concatenate(ropes).astype(np.float32)
+    x += rng.normal(0, 0.002, x.shape).astype(np.float32)
+    pads = rng.uniform(0.004, 0.012, nv).astype(np.float32)
+    L = _lib(rs)
+    counts = np.zeros(nv, np.int32)
+    lists = np.zeros(nv * 128, np.int32)
+    lim = np.zeros(nv, np.float32)
+    fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int)
+    rc = L.rs_check_detect(C.c_void_p(b._handle), x.ctypes.data_as(fp), pads.ctypes.data_as(fp),
+                           counts.ctypes.data_as(ip), lists.ctypes.data_as(ip), lim.ctypes.data_as(fp))
+    assert rc == 0
+    oc, ol, olim = o.detect(x.astype(np.float64), pads.astype(np.float64))
+    assert counts.sum() > 100, "the scene must produce many candidates"
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(lists.reshape(nv, 128), ol)
+    np.testing.assert_array_equal(lim, olim.astype(np.float32))
+
+
+def test_mesh_broadphase_bitexact_vs_oracle(rs, oracle):
+    from ropesim import scenes
+    s = scenes.config3(n_ropes=8, frames=1, vmax=64, grid_n=16)
+    b = s.build(rs)
+    o = oracle.OracleBatch(s.positions, s.density, s.k_stretch, s.k_bend, s.max_h, s.num_itr, prec=32,
+                           offsets=s.offsets)
+    s.apply(o)
+    rng = np.random.default_rng(3)
+    nv = s.num_vertices
+    # probe points scattered over the meshes' neighbourhood, displacements of a few mm
+    x = rng.uniform([-1.4, -1.4, 0.1], [1.4, 1.4, 1.7], (nv, 3)).astype(np.float32)
+    dx = rng.normal(0, 0.003, (nv, 3)).astype(np.float32)
+    counts = np.zeros(nv, np.int32)
+    cand = np.zeros(nv * 8, np.int32)
+    L = _lib(rs)
+    fp, ip = C.POINTER(C.c_float), C.POINTER(C.c_int)
+    assert L.rs_check_broadphase(C.c_void_p(b._handle), x.ctypes.data_as(fp), dx.ctypes.data_as(fp),
+                                 counts.ctypes.data_as(ip), cand.ctypes.data_as(ip)) == 0
+    oc, ocand = o.broadphase(x.astype(np.float64), dx.astype(np.float64))
+    assert (counts > 0).sum() > 10
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(cand.reshape(nv, 8), ocand)
