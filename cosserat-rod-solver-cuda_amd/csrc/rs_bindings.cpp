@@ -811,9 +811,9 @@ PYBIND11_MODULE(_ropesim, m) {
              "the solver stream.")
         .def("chunk_stats",
              [](RopeBatch& b, bool on) {
-                 long long h = 0;
-                 check(rs_set_chunk_stats(b.b(), on ? 1 : 0, &h));
-                 return h;
+                 long long h[4] = {0, 0, 0, 0};
+                 check(rs_set_chunk_stats(b.b(), on ? 1 : 0, h));
+                 return py::make_tuple(h[0], h[1], h[2], h[3]);
              },
              py::arg("on") = true)
         .def_property("drag", [](RopeBatch& b) { return b.param(RS_P_DRAG); }, [](RopeBatch& b, float v) { b.set_param(RS_P_DRAG, v); })

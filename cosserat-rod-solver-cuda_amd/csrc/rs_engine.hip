@@ -120,7 +120,7 @@ struct rs_batch {
     std::vector<Cell> cells;
     Cell* d_cells = nullptr;
     int* d_order = nullptr;
-    int* d_hits = nullptr;
+    unsigned long long* d_hits = nullptr;  // rope-rope statistics [4]
     bool countHits = false;
     // optional timing of the frame kernels on the solver stream (event pairs, summed on query)
     bool timing = false;
@@ -315,11 +315,11 @@ struct rs_batch {
         d_initQ = dalloc<float4>(n, "initQ");
         d_ids = dalloc<int>(N, "ids");
         d_prog = dalloc<float>(17 * (size_t)N, "prog");
-        d_hits = dalloc<int>(1, "hits");
+        d_hits = dalloc<unsigned long long>(4, "stats");
         hcheck(hipMemset(D.err, 0, 2 * sizeof(int)), "memset err");
         hcheck(hipMemset(D.vlast, 0, n * sizeof(float4)), "memset vlast");
         hcheck(hipMemset(D.ncol, 0, n * sizeof(int)), "memset ncol");
-        hcheck(hipMemset(d_hits, 0, sizeof(int)), "memset hits");
+        hcheck(hipMemset(d_hits, 0, 4 * sizeof(unsigned long long)), "memset stats");
     }
 
     void upload_all() {
@@ -1297,14 +1297,15 @@ int rs_kernel_time(rs_batch* b, double* total_ms, int* launches) {
     });
 }
 
-int rs_set_chunk_stats(rs_batch* b, int on, long long* hits_out) {
+int rs_set_chunk_stats(rs_batch* b, int on, long long* out4) {
     return guarded([&] {
         std::lock_guard<std::mutex> g(b->mtx);
-        int h = 0;
+        unsigned long long h[4] = {0, 0, 0, 0};
         hcheck(hipStreamSynchronize(b->stream), "sync");
-        hcheck(hipMemcpy(&h, b->d_hits, sizeof(int), hipMemcpyDeviceToHost), "hits");
-        hcheck(hipMemset(b->d_hits, 0, sizeof(int)), "hits");
-        if (hits_out) *hits_out = h;
+        hcheck(hipMemcpy(h, b->d_hits, sizeof(h), hipMemcpyDeviceToHost), "stats");
+        hcheck(hipMemset(b->d_hits, 0, sizeof(h)), "stats");
+        if (out4)
+            for (int i = 0; i < 4; i++) out4[i] = (long long)h[i];
         b->countHits = on != 0;
     });
 }

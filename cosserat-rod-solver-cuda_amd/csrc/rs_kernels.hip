@@ -373,34 +373,37 @@ __global__ void k_check_broadphase(Params P, Dev D, const float4* xs, const floa
 //
 // One workgroup advances one cell (a contiguous run of whole ropes; a rope-rope contact group never
 // straddles cells) through all S substeps of a frame. Thread t owns local vertex slots t + k * BLOCK,
-// k < VPT. LDS holds what neighbours and contact partners read: positions, the displacement iterate
-// (ping-pong), the frames (ping-pong), the segment terms exchanged with the next vertex, the detection
-// paddings, the step limits and per-chunk boxes for the rope-rope broad phase. Per-vertex constants,
-// velocities, inertial targets and broad-phase candidates stay in registers. Per iteration there are
-// two barriers: [orientation(k-1) + terms(k)] | [solve(k)] | ...
+// k < VPT. LDS holds what neighbours and contact partners read: positions, the displacement iterate, the
+// frames, the segment terms exchanged with the next vertex, the detection paddings, the step limits, the
+// flags and per-chunk boxes for the rope-rope broad phase. Per-vertex constants, velocities, inertial
+// targets, the vertex's own frame and broad-phase candidates stay in registers.
+//
+// Jacobi semantics with one displacement and one frame buffer: per iteration
+//   [terms(k): read x, dx^k, partners; write segment terms] | [solve(k): dx^k -> dx^(k+1) in place; publish
+//   q^k of the previous orientation sweep] | [orientation(k): read dx^(k+1), neighbours' q^k -> own q^(k+1)
+//   kept in a register] | ...
+// so nobody reads a slot while its owner overwrites it, and the results equal the per-substep kernels'.
 template <int BLOCK, int VPT>
 __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const Cell* cells, const int* order, int S,
-                                                int* chunkHits) {
+                                                unsigned long long* stats) {
     constexpr int CAP = BLOCK * VPT;
     constexpr int NCH = CAP / 16;
     extern __shared__ float4 smem4[];
-    float* sX = reinterpret_cast<float*>(smem4);  // [3][CAP]
-    float* sDA = sX + 3 * CAP;                    // [3][CAP]
-    float* sDB = sDA + 3 * CAP;                   // [3][CAP]
-    float4* sQA = reinterpret_cast<float4*>(sDB + 3 * CAP);
-    float4* sQB = sQA + CAP;
-    float* sF = reinterpret_cast<float*>(sQB + CAP);  // [9][CAP]
-    float* sPad = sF + 9 * CAP;                       // [CAP]
-    float* sLim = sPad + CAP;                         // [CAP]
-    float* sBox = sLim + CAP;                         // [6][NCH]
-    uint32_t* sFl = reinterpret_cast<uint32_t*>(sBox + 6 * NCH);  // [CAP]
+    float4* sQ = smem4;                                  // [CAP] frames
+    float* sX = reinterpret_cast<float*>(sQ + CAP);     // [3][CAP] positions at the substep start
+    float* sD = sX + 3 * CAP;                            // [3][CAP] displacement iterate
+    float* sF = sD + 3 * CAP;                            // [9][CAP] segment terms for the next vertex
+    float* sPad = sF + 9 * CAP;                          // [CAP]
+    float* sLim = sPad + CAP;                            // [CAP]
+    uint32_t* sFl = reinterpret_cast<uint32_t*>(sLim + CAP);  // [CAP]
+    float* sBox = reinterpret_cast<float*>(sFl + CAP);   // [6][NCH]
 
     const Cell C = cells[order[blockIdx.x]];
     const int t = threadIdx.x;
 
     int gi[VPT], rope[VPT], nc[VPT], ncol[VPT];
     float4 prm[VPT];
-    q4 qr[VPT], qrp[VPT];
+    q4 qr[VPT], qrp[VPT], qo[VPT];
     f3 vel[VPT], vl[VPT], y[VPT];
     float limEff[VPT];
     uint64_t cand[VPT];
@@ -435,23 +438,20 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             sX[sl] = x.x;
             sX[CAP + sl] = x.y;
             sX[2 * CAP + sl] = x.z;
-            sQA[sl] = stq(q);
+            qo[k] = q;
+            sQ[sl] = stq(q);
             sFl[sl] = fl;
         } else {
             prm[k] = make_float4(0.f, 0.f, 0.f, 0.f);
             rope[k] = 0;
             isRoot[k] = false;
             vel[k] = vl[k] = y[k] = mk3(0.f, 0.f, 0.f);
-            qr[k] = qrp[k] = q4{0.f, 0.f, 0.f, 1.f};
+            qr[k] = qrp[k] = qo[k] = q4{0.f, 0.f, 0.f, 1.f};
             sX[sl] = sX[CAP + sl] = sX[2 * CAP + sl] = 0.f;
-            sQA[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
+            sQ[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
             sFl[sl] = 0u;
         }
     }
-    float* dCur = sDA;
-    float* dNxt = sDB;
-    float4* qCur = sQA;
-    float4* qNxt = sQB;
     const float h = P.h;
 
     for (int s = 0; s < S; s++) {
@@ -469,7 +469,10 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     sX[CAP + sl] = x.y;
                     sX[2 * CAP + sl] = x.z;
                     vel[k] = v;
-                    if (setQ) qCur[sl] = stq(q);
+                    if (setQ) {
+                        qo[k] = q;
+                        sQ[sl] = stq(q);
+                    }
                 }
             }
         }
@@ -489,9 +492,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 float padding;
                 init_vertex(P, prm[k].x, vel[k], vl[k], d, y[k], padding);
                 vl[k] = vel[k];
-                dCur[sl] = d.x;
-                dCur[CAP + sl] = d.y;
-                dCur[2 * CAP + sl] = d.z;
+                sD[sl] = d.x;
+                sD[CAP + sl] = d.y;
+                sD[2 * CAP + sl] = d.z;
                 sPad[sl] = padding;
             }
         }
@@ -526,8 +529,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 }
             }
             __syncthreads();
-            // exact tests in the chunks whose box meets the segment's own padded box (conservative:
-            // the exact test decides membership; lists stay in ascending partner order)
+            // Exact tests only where the chunk box and then the partner segment's own padded box meet this
+            // segment's padded box. Both filters are conservative (a margin far above float rounding of the
+            // coordinates), so the exact test alone decides membership and lists keep ascending order.
 #pragma unroll
             for (int k = 0; k < VPT; k++) {
                 const int sl = t + k * BLOCK;
@@ -544,10 +548,10 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     const f3 a1 = x1 - a0;
                     const float ri = fmaxf(sPad[sl], sPad[sl + 1]);
                     minDist = sPad[sl] - P.detR;
-                    const float mg = ri + 1e-4f + 2e-6f * fmaxf(fabsf(a0.x), fmaxf(fabsf(a0.y), fabsf(a0.z)));
-                    const f3 lo = vmin(a0, x1) - mk3(mg, mg, mg), hi = vmax(a0, x1) + mk3(mg, mg, mg);
-                    int cnt = 0;
-                    int hits = 0;
+                    const float mg = 1e-4f + 2e-6f * fmaxf(fabsf(a0.x), fmaxf(fabsf(a0.y), fabsf(a0.z)));
+                    const float rm = ri + mg;
+                    const f3 lo = vmin(a0, x1) - mk3(rm, rm, rm), hi = vmax(a0, x1) + mk3(rm, rm, rm);
+                    int cnt = 0, hits = 0, tests = 0;
                     for (int ch = g0 >> 4; ch <= (g1 - 1) >> 4; ch++) {
                         if (sBox[ch] > hi.x || sBox[NCH + ch] > hi.y || sBox[2 * NCH + ch] > hi.z ||
                             sBox[3 * NCH + ch] < lo.x || sBox[4 * NCH + ch] < lo.y || sBox[5 * NCH + ch] < lo.z)
@@ -557,11 +561,16 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         for (int j = jb; j < je; j++) {
                             if (!(sFl[j] & HAS_NEXT)) continue;
                             if (j >= rs0 && j < rs1 && abs(j - sl) <= 2) continue;
-                            const f3 b0 = mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]) - a0;
-                            const f3 b1 = mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]) - a0;
-                            const float r = ri + fmaxf(sPad[j], sPad[j + 1]);
+                            const f3 c0 = mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]);
+                            const f3 c1 = mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]);
+                            const float rj = fmaxf(sPad[j], sPad[j + 1]);
+                            const f3 blo = vmin(c0, c1), bhi = vmax(c0, c1);
+                            if (blo.x - rj > hi.x || blo.y - rj > hi.y || blo.z - rj > hi.z || bhi.x + rj < lo.x ||
+                                bhi.y + rj < lo.y || bhi.z + rj < lo.z)
+                                continue;
+                            tests++;
                             float d2;
-                            if (pair_hit(a1, b0, b1, r, d2)) {
+                            if (pair_hit(a1, c0 - a0, c1 - a0, ri + rj, d2)) {
                                 if (cnt < MAX_COLL) {
                                     D.coll[(size_t)i * MAX_COLL + cnt] = C.v0 + j;
                                     minDist = fminf(minDist, 0.45f * sqrtf(d2));
@@ -570,7 +579,12 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                             }
                         }
                     }
-                    if (chunkHits) atomicAdd(chunkHits, hits);
+                    if (stats) {  // statistics only: chunk overlaps, exact tests, list entries, segments
+                        atomicAdd(stats, (unsigned long long)hits);
+                        atomicAdd(stats + 1, (unsigned long long)tests);
+                        atomicAdd(stats + 2, (unsigned long long)cnt);
+                        atomicAdd(stats + 3, 1ull);
+                    }
                     if (cnt > MAX_COLL) D.err[0] = ERR_MAX_COLLISIONS;
                     ncol[k] = min(cnt, MAX_COLL);
                 }
@@ -601,14 +615,14 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 if (gi[k] >= 0) {
                     const uint32_t fl = flags_of(prm[k]);
                     const f3 p0 = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-                    const f3 d0 = mk3(dCur[sl], dCur[CAP + sl], dCur[2 * CAP + sl]);
+                    const f3 d0 = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
                     f3 p1 = p0, p1dx = d0;
                     if (fl & HAS_NEXT) {
                         p1 = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
-                        p1dx = mk3(dCur[sl + 1], dCur[CAP + sl + 1], dCur[2 * CAP + sl + 1]);
+                        p1dx = mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]);
                     }
                     vertex_terms(P, D.grids, D.poses + (size_t)rope[k] * MAX_SDF, nc[k], cand[k], prm[k].x, prm[k].y,
-                                 prm[k].z, fl, p0, d0, y[k], ldq(qCur[sl]), p1, p1dx, fk[k], Hk[k], f2, H2);
+                                 prm[k].z, fl, p0, d0, y[k], qo[k], p1, p1dx, fk[k], Hk[k], f2, H2);
                     if (P.ropeContact && (fl & HAS_NEXT)) {
                         const float dmp = P.damping / h;
                         const int* lst = D.coll + (size_t)gi[k] * MAX_COLL;
@@ -616,9 +630,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                             const int j = lst[c] - C.v0;
                             collision_term(P, dmp, p0, p1, d0, p1dx, mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]),
                                            mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]),
-                                           mk3(dCur[j], dCur[CAP + j], dCur[2 * CAP + j]),
-                                           mk3(dCur[j + 1], dCur[CAP + j + 1], dCur[2 * CAP + j + 1]), fk[k], Hk[k], f2,
-                                           H2);
+                                           mk3(sD[j], sD[CAP + j], sD[2 * CAP + j]),
+                                           mk3(sD[j + 1], sD[CAP + j + 1], sD[2 * CAP + j + 1]), fk[k], Hk[k], f2, H2);
                         }
                     }
                 }
@@ -643,11 +656,14 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         for (int c = 0; c < 6; c++) Hp.m[c] = sF[(3 + c) * CAP + sl - 1];
                         sadd(H, Hp);
                     }
-                    f3 d = mk3(dCur[sl], dCur[CAP + sl], dCur[2 * CAP + sl]);
-                    if (prm[k].x != 0.f) d = solve_update(P, d, H, f, P.ropeContact != 0, limEff[k]);
-                    dNxt[sl] = d.x;
-                    dNxt[CAP + sl] = d.y;
-                    dNxt[2 * CAP + sl] = d.z;
+                    if (prm[k].x != 0.f) {
+                        f3 d = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
+                        d = solve_update(P, d, H, f, P.ropeContact != 0, limEff[k]);
+                        sD[sl] = d.x;
+                        sD[CAP + sl] = d.y;
+                        sD[2 * CAP + sl] = d.z;
+                    }
+                    if (it > 0) sQ[sl] = stq(qo[k]);  // publish the frames of the previous orientation sweep
                 }
             }
             __syncthreads();
@@ -656,27 +672,23 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 const int sl = t + k * BLOCK;
                 if (gi[k] >= 0) {
                     const uint32_t fl = flags_of(prm[k]);
-                    const q4 q0 = ldq(qCur[sl]);
                     if (!(fl & FIX_ORIENT) && (fl & HAS_NEXT)) {
-                        const q4 qp = (fl & HAS_PREV) ? ldq(qCur[sl - 1]) : q0;
-                        const q4 qn = (fl & HAS_NEXT_ORIENT) ? ldq(qCur[sl + 1]) : q0;
-                        qNxt[sl] = stq(orient_update(
-                            fl, prm[k].y, prm[k].z, mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
-                            mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]),
-                            mk3(dNxt[sl], dNxt[CAP + sl], dNxt[2 * CAP + sl]),
-                            mk3(dNxt[sl + 1], dNxt[CAP + sl + 1], dNxt[2 * CAP + sl + 1]), qp, q0, qn, qrp[k], qr[k]));
-                    } else {
-                        qNxt[sl] = stq(q0);
+                        const q4 q0 = ldq(sQ[sl]);
+                        const q4 qp = (fl & HAS_PREV) ? ldq(sQ[sl - 1]) : q0;
+                        const q4 qn = (fl & HAS_NEXT_ORIENT) ? ldq(sQ[sl + 1]) : q0;
+                        qo[k] = orient_update(fl, prm[k].y, prm[k].z, mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
+                                              mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]),
+                                              mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]),
+                                              mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]), qp, q0, qn, qrp[k],
+                                              qr[k]);
                     }
                 }
             }
-            float* td = dCur;
-            dCur = dNxt;
-            dNxt = td;
-            float4* tq = qCur;
-            qCur = qNxt;
-            qNxt = tq;
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < VPT; k++)
+            if (gi[k] >= 0) sQ[t + k * BLOCK] = stq(qo[k]);  // frames of the last orientation sweep
         __syncthreads();
 
         // end of substep
@@ -685,12 +697,12 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             if (gi[k] >= 0) {
                 const int sl = t + k * BLOCK;
                 const uint32_t fl = flags_of(prm[k]);
-                const f3 d = mk3(dCur[sl], dCur[CAP + sl], dCur[2 * CAP + sl]);
+                const f3 d = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
                 const f3 x = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
                 const Pose* envPoses = D.poses + (size_t)rope[k] * MAX_SDF;
                 if (prm[k].x != 0.f) {
                     f3 tg = mk3(1.f, 0.f, 0.f);
-                    if (P.dragQuad > 0.f) tg = qrot(ldq((fl & HAS_NEXT) ? qCur[sl] : qCur[sl - 1]), mk3(1.f, 0.f, 0.f));
+                    if (P.dragQuad > 0.f) tg = qrot((fl & HAS_NEXT) ? qo[k] : ldq(sQ[sl - 1]), mk3(1.f, 0.f, 0.f));
                     vel[k] = end_velocity(P, d, tg);
                     if (P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0)) {
                         const float kap = P.cKappa / h;
@@ -724,11 +736,11 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             const int sl = t + k * BLOCK;
             const int i = gi[k];
             const f3 x = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-            const float4 q = qCur[sl];
+            const q4 q = qo[k];
             D.x[i] = st3(x);
             D.v[i] = st3(vel[k]);
             D.vlast[i] = st3(vl[k]);
-            D.q[i] = q;
+            D.q[i] = stq(q);
             D.outPos[3 * i] = x.x;
             D.outPos[3 * i + 1] = x.y;
             D.outPos[3 * i + 2] = x.z;
@@ -819,12 +831,12 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
 
 size_t cell_lds_bytes(int block, int vpt) {
     const size_t cap = (size_t)block * vpt;
-    return cap * (3 + 3 + 3 + 4 + 4 + 9 + 1 + 1 + 1) * sizeof(float) + (cap / 16) * 6 * sizeof(float);
+    return cap * (4 + 3 + 3 + 9 + 1 + 1 + 1) * sizeof(float) + (cap / 16) * 6 * sizeof(float);
 }
 
 template <int B, int V>
 static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
-                                int ncells, int S, int* chunkHits, hipStream_t st) {
+                                int ncells, int S, unsigned long long* stats, hipStream_t st) {
     const size_t lds = cell_lds_bytes(B, V);
     static bool attrSet = false;
     if (!attrSet) {
@@ -833,7 +845,7 @@ static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, 
         if (e != hipSuccess) return e;
         attrSet = true;
     }
-    hipLaunchKernelGGL((k_cell<B, V>), dim3(ncells), dim3(B), lds, st, P, D, Dr, cells, order, S, chunkHits);
+    hipLaunchKernelGGL((k_cell<B, V>), dim3(ncells), dim3(B), lds, st, P, D, Dr, cells, order, S, stats);
     return hipGetLastError();
 }
 
@@ -841,17 +853,17 @@ const CellVariant kCellVariants[] = {{64, 1}, {128, 1}, {256, 1}, {512, 1}, {102
 const int kNumCellVariants = sizeof(kCellVariants) / sizeof(kCellVariants[0]);
 
 hipError_t launch_cell(int variant, const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
-                       int ncells, int S, int* chunkHits, hipStream_t st) {
+                       int ncells, int S, unsigned long long* stats, hipStream_t st) {
     if (ncells <= 0) return hipSuccess;
     switch (variant) {
-        case 0: return launch_cell_t<64, 1>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
-        case 1: return launch_cell_t<128, 1>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
-        case 2: return launch_cell_t<256, 1>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
-        case 3: return launch_cell_t<512, 1>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
-        case 4: return launch_cell_t<1024, 1>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
-        case 5: return launch_cell_t<256, 2>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
-        case 6: return launch_cell_t<256, 4>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
-        case 7: return launch_cell_t<512, 2>(P, D, Dr, cells, order, ncells, S, chunkHits, st);
+        case 0: return launch_cell_t<64, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 1: return launch_cell_t<128, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 2: return launch_cell_t<256, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 3: return launch_cell_t<512, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 4: return launch_cell_t<1024, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 5: return launch_cell_t<256, 2>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 6: return launch_cell_t<256, 4>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 7: return launch_cell_t<512, 2>(P, D, Dr, cells, order, ncells, S, stats, st);
         default: return hipErrorInvalidValue;
     }
 }

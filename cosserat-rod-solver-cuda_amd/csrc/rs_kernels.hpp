@@ -23,7 +23,7 @@ hipError_t launch_reset(const Params& P, const Dev& D, const float4* initX, cons
                         int count, hipStream_t s);
 hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s, int S, hipStream_t st);
 hipError_t launch_cell(int variant, const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
-                       int ncells, int S, int* chunkHits, hipStream_t st);
+                       int ncells, int S, unsigned long long* stats, hipStream_t st);
 hipError_t launch_scatter_targets(Pose7* tgt, uint8_t* tele, int index, const float* pos, const float* q, int rows,
                                   int M, const uint8_t* teleFlags, int teleAll, int N, hipStream_t s);
 hipError_t launch_stage_motion(Pose7* cur, const Pose7* tgt, uint8_t* tele, Motion* motion, int total, hipStream_t s);

@@ -165,8 +165,10 @@ int rs_check_broadphase(rs_batch* b, const float* pos, const float* dx, int* cou
  * the last query, synchronising on them. */
 int rs_set_kernel_timing(rs_batch* b, int on);
 int rs_kernel_time(rs_batch* b, double* total_ms, int* launches);
-/* Rope-rope chunk-pair box hits counted during the next frames (0 disables). Statistic only. */
-int rs_set_chunk_stats(rs_batch* b, int on, long long* hits_out);
+/* Rope-rope broad-phase statistics of the persistent kernel (no counterpart in the reference): returns
+ * and clears [chunk-box overlaps, exact segment-pair tests, list entries, segment-substeps] counted since the
+ * last call, and turns counting on or off for the next frames. */
+int rs_set_chunk_stats(rs_batch* b, int on, long long* out4);
 
 #ifdef __cplusplus
 }
