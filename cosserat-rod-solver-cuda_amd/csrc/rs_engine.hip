@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ropesim_hip.h"
+#include "ropesim_hip.h"
 #include "rs_kernels.hpp"
 #include "rs_math.hpp"
 #include "rs_types.hpp"
@@ -144,7 +144,7 @@ struct rs_batch {
             hipEventDestroy(e.second);
         }
         void* ptrs[] = {D.x, D.v, D.vlast, D.q, D.q2, D.dx, D.dx2, D.prm, D.qrest, D.ropeOf, d_roff, D.pad, D.maxStep,
-                        D.ncol, D.coll, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.outPos,
+                        D.ncol, D.coll, D.vlist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.outPos,
                         D.outVel, D.outQuat, D.err, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_hits,
                         d_grids, d_cur, d_tgt, d_tele, d_stage, d_stageTele};
         for (void* p : ptrs)
@@ -949,6 +949,7 @@ int rs_set_rope_contact(rs_batch* b, int enabled, float radius, float barrier, f
             if (2.f * b->minSegLen < 2.f * radius + barrier)
                 throw Invalid("rope contact needs 2 * radius + barrier <= 2 * shortest segment");
             if (!b->D.coll) b->D.coll = dalloc<int>((size_t)b->nv * MAX_COLL, "collision lists");
+            if (!b->D.vlist) b->D.vlist = dalloc<int>((size_t)b->nv * VERLET_CAP, "skin lists");
             b->groupRopes = group_ropes;
             b->radius = radius;
             b->barrier = barrier;

@@ -388,20 +388,32 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                                                 unsigned long long* stats) {
     constexpr int CAP = BLOCK * VPT;
     constexpr int NCH = CAP / 16;
+    static_assert(NCH <= 64 && BLOCK % 64 == 0, "chunk masks are 64 bits; waves are 64 lanes");
     extern __shared__ float4 smem4[];
     float4* sQ = smem4;                                  // [CAP] frames
     float* sX = reinterpret_cast<float*>(sQ + CAP);     // [3][CAP] positions at the substep start
     float* sD = sX + 3 * CAP;                            // [3][CAP] displacement iterate
     float* sF = sD + 3 * CAP;                            // [9][CAP] segment terms for the next vertex
     float* sPad = sF + 9 * CAP;                          // [CAP]
-    float* sLim = sPad + CAP;                            // [CAP]
-    uint32_t* sFl = reinterpret_cast<uint32_t*>(sLim + CAP);  // [CAP]
+    uint32_t* sFl = reinterpret_cast<uint32_t*>(sPad + CAP);  // [CAP]
+    // during the rope-rope broad phase sF holds each segment's padded box, its radius and step limit
+    float* sBlo = sF;            // [3][CAP]
+    float* sBhi = sF + 3 * CAP;  // [3][CAP]
+    float* sRj = sF + 6 * CAP;   // [CAP]
+    float* sLim = sF + 7 * CAP;  // [CAP]
+    int* sNcol = reinterpret_cast<int*>(sF + 8 * CAP);  // [CAP]
     float* sBox = reinterpret_cast<float*>(sFl + CAP);   // [6][NCH]
+    uint32_t* sOv = reinterpret_cast<uint32_t*>(sBox + 6 * NCH);  // [2][NCH] chunk-overlap bit masks (lo, hi words)
+    int* sChG = reinterpret_cast<int*>(sOv + 2 * NCH);              // [2][NCH] group range of each chunk
+    float* sX0 = reinterpret_cast<float*>(sChG + 2 * NCH);         // [3][CAP] positions at the last skin build
+    float* sR = sX0 + 3 * CAP;                                      // [CAP] detection radius bound of the build
 
     const Cell C = cells[order[blockIdx.x]];
     const int t = threadIdx.x;
 
     int gi[VPT], rope[VPT], nc[VPT], ncol[VPT];
+    int grp[VPT], rng[VPT];  // packed local [begin, end) of the contact group and of the rope (16 bits each)
+    int vcnt[VPT];           // skin-list length of the vertex's segment (-1: overflow, scan the whole group)
     float4 prm[VPT];
     q4 qr[VPT], qrp[VPT], qo[VPT];
     f3 vel[VPT], vl[VPT], y[VPT];
@@ -416,6 +428,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
         nc[k] = P.numSdf > 0 ? -1 : 0;  // a frame starts by visiting every mesh (sdf.cu:226-227)
         cand[k] = 0;
         ncol[k] = 0;
+        vcnt[k] = 0;
         limEff[k] = INFINITY;
         if (gi[k] >= 0) {
             const int i = gi[k];
@@ -425,6 +438,12 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             qrp[k] = (fl & HAS_PREV) ? ldq(D.qrest[i - 1]) : qr[k];
             rope[k] = D.ropeOf[i];
             isRoot[k] = D.roff[rope[k]] == i;
+            {
+                const int G = P.groupRopes;
+                const int g = rope[k] / G;
+                grp[k] = (D.roff[min(g * G, P.nropes)] - C.v0) | ((D.roff[min((g + 1) * G, P.nropes)] - C.v0) << 16);
+                rng[k] = (D.roff[rope[k]] - C.v0) | ((D.roff[rope[k] + 1] - C.v0) << 16);
+            }
             f3 x = ld3(D.x[i]);
             q4 q = ldq(D.q[i]);
             vel[k] = ld3(D.v[i]);
@@ -440,10 +459,11 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             sX[2 * CAP + sl] = x.z;
             qo[k] = q;
             sQ[sl] = stq(q);
-            sFl[sl] = fl;
+            sFl[sl] = fl | ((uint32_t)(grp[k] & 0xFFFF) << 4) | ((uint32_t)(grp[k] >> 16) << 15);
         } else {
             prm[k] = make_float4(0.f, 0.f, 0.f, 0.f);
             rope[k] = 0;
+            grp[k] = rng[k] = 0;
             isRoot[k] = false;
             vel[k] = vl[k] = y[k] = mk3(0.f, 0.f, 0.f);
             qr[k] = qrp[k] = qo[k] = q4{0.f, 0.f, 0.f, 1.f};
@@ -501,37 +521,162 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
         __syncthreads();
 
         if (P.ropeContact) {
-            // per-chunk boxes of 16 consecutive segments, padded by each segment's detection radius
+            // Rope-rope broad phase with skin (Verlet) lists. At a rebuild every segment records, in ascending
+            // order, the partners whose box (padded by the build radius R + skin) meets its own: a superset of
+            // every pair the exact test can accept until some vertex has moved more than the skin from its
+            // build position or some detection radius has grown past R (then the next substep rebuilds).
+            // Each substep the exact test of the reference runs over that list in order, so the candidate
+            // lists are the ones a full ascending scan of the contact group produces.
+            const float skin = 2.f * P.scaledDetR, slackR = 0.25f * P.scaledDetR;
+            bool viol = s == 0;
 #pragma unroll
             for (int k = 0; k < VPT; k++) {
                 const int sl = t + k * BLOCK;
-                f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = mk3(-INFINITY, -INFINITY, -INFINITY);
-                if (gi[k] >= 0 && (flags_of(prm[k]) & HAS_NEXT)) {
-                    const f3 a = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-                    const f3 b = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
-                    const float r = fmaxf(sPad[sl], sPad[sl + 1]);
-                    lo = vmin(a, b) - mk3(r, r, r);
-                    hi = vmax(a, b) + mk3(r, r, r);
-                }
-#pragma unroll
-                for (int o = 1; o < 16; o <<= 1) {
-                    lo = vmin(lo, mk3(__shfl_xor(lo.x, o), __shfl_xor(lo.y, o), __shfl_xor(lo.z, o)));
-                    hi = vmax(hi, mk3(__shfl_xor(hi.x, o), __shfl_xor(hi.y, o), __shfl_xor(hi.z, o)));
-                }
-                if ((t & 15) == 0) {
-                    const int ch = sl >> 4;
-                    sBox[ch] = lo.x;
-                    sBox[NCH + ch] = lo.y;
-                    sBox[2 * NCH + ch] = lo.z;
-                    sBox[3 * NCH + ch] = hi.x;
-                    sBox[4 * NCH + ch] = hi.y;
-                    sBox[5 * NCH + ch] = hi.z;
+                if (gi[k] >= 0 && s > 0) {
+                    const f3 dxv = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]) - mk3(sX0[sl], sX0[CAP + sl], sX0[2 * CAP + sl]);
+                    if (len2(dxv) > skin * skin) viol = true;
+                    if ((flags_of(prm[k]) & HAS_NEXT) && fmaxf(sPad[sl], sPad[sl + 1]) > sR[sl]) viol = true;
                 }
             }
-            __syncthreads();
-            // Exact tests only where the chunk box and then the partner segment's own padded box meet this
-            // segment's padded box. Both filters are conservative (a margin far above float rounding of the
-            // coordinates), so the exact test alone decides membership and lists keep ascending order.
+            if (__syncthreads_or(viol)) {
+                // (1) build boxes padded by the build radius, chunk boxes and chunk group ranges
+#pragma unroll
+                for (int k = 0; k < VPT; k++) {
+                    const int sl = t + k * BLOCK;
+                    f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = mk3(-INFINITY, -INFINITY, -INFINITY);
+                    int cg0 = 1 << 30, cg1 = 0;
+                    if (gi[k] >= 0) {
+                        cg0 = grp[k] & 0xFFFF;
+                        cg1 = grp[k] >> 16;
+                        const f3 a = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+                        sX0[sl] = a.x;
+                        sX0[CAP + sl] = a.y;
+                        sX0[2 * CAP + sl] = a.z;
+                        if (flags_of(prm[k]) & HAS_NEXT) {
+                            const f3 b = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
+                            const float R = fmaxf(sPad[sl], sPad[sl + 1]) + slackR;
+                            sR[sl] = R;
+                            const float rb = R + skin + 1e-4f + 2e-6f * fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z)));
+                            lo = vmin(a, b) - mk3(rb, rb, rb);
+                            hi = vmax(a, b) + mk3(rb, rb, rb);
+                        }
+                    }
+                    sBlo[sl] = lo.x;
+                    sBlo[CAP + sl] = lo.y;
+                    sBlo[2 * CAP + sl] = lo.z;
+                    sBhi[sl] = hi.x;
+                    sBhi[CAP + sl] = hi.y;
+                    sBhi[2 * CAP + sl] = hi.z;
+#pragma unroll
+                    for (int o = 1; o < 16; o <<= 1) {
+                        lo = vmin(lo, mk3(__shfl_xor(lo.x, o), __shfl_xor(lo.y, o), __shfl_xor(lo.z, o)));
+                        hi = vmax(hi, mk3(__shfl_xor(hi.x, o), __shfl_xor(hi.y, o), __shfl_xor(hi.z, o)));
+                        cg0 = min(cg0, __shfl_xor(cg0, o));
+                        cg1 = max(cg1, __shfl_xor(cg1, o));
+                    }
+                    if ((t & 15) == 0) {
+                        const int ch = sl >> 4;
+                        sBox[ch] = lo.x;
+                        sBox[NCH + ch] = lo.y;
+                        sBox[2 * NCH + ch] = lo.z;
+                        sBox[3 * NCH + ch] = hi.x;
+                        sBox[4 * NCH + ch] = hi.y;
+                        sBox[5 * NCH + ch] = hi.z;
+                        sChG[ch] = cg0;
+                        sChG[NCH + ch] = cg1;
+                    }
+                }
+                __syncthreads();
+                const int nch = (C.nv + 15) >> 4;
+                // (2) chunk A meets chunk B (B within A's contact groups): 16 lanes per A, OR-reduced masks
+                for (int base = 0; base < nch * 16; base += BLOCK) {
+                    const int A = (base + t) >> 4, sub = t & 15;
+                    uint32_t mlo = 0, mhi = 0;
+                    if (A < nch && sChG[A] < sChG[NCH + A]) {
+                        const float lx = sBox[A], ly = sBox[NCH + A], lz = sBox[2 * NCH + A];
+                        const float hx = sBox[3 * NCH + A], hy = sBox[4 * NCH + A], hz = sBox[5 * NCH + A];
+                        const int b0 = sChG[A] >> 4, b1 = (sChG[NCH + A] - 1) >> 4;
+                        for (int B = b0 + sub; B <= b1; B += 16) {
+                            const bool ov = !(sBox[B] > hx || sBox[NCH + B] > hy || sBox[2 * NCH + B] > hz ||
+                                              sBox[3 * NCH + B] < lx || sBox[4 * NCH + B] < ly || sBox[5 * NCH + B] < lz);
+                            if (ov) {
+                                if (B < 32)
+                                    mlo |= 1u << B;
+                                else
+                                    mhi |= 1u << (B - 32);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int o = 1; o < 16; o <<= 1) {
+                        mlo |= __shfl_xor(mlo, o);
+                        mhi |= __shfl_xor(mhi, o);
+                    }
+                    if (A < nch && sub == 0) {
+                        sOv[A] = mlo;
+                        sOv[NCH + A] = mhi;
+                    }
+                }
+                __syncthreads();
+                // (3) one wave per chunk A: lane (a, g) holds segment i = 16 A + a and tests partners 4g..4g+3
+                //     of every overlapping chunk B in ascending order; the four lanes of a segment append their
+                //     hits in partner order (cross-lane prefix), so each skin list ascends.
+                {
+                    const int lane = t & 63, wave = t >> 6;
+                    const int a = lane & 15, g = lane >> 4;
+                    for (int A = wave; A < nch; A += BLOCK / 64) {
+                        const int i = A * 16 + a;
+                        const uint32_t fi = i < C.nv ? sFl[i] : 0u;
+                        const bool seg = (fi & HAS_NEXT) != 0;
+                        const int g0 = (fi >> 4) & 0x7FF, g1 = (fi >> 15) & 0x7FF;
+                        const bool nextHN = seg && (sFl[i + 1] & HAS_NEXT);
+                        const bool prevHN = seg && (fi & HAS_PREV);
+                        f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = -lo;
+                        if (seg) {
+                            lo = mk3(sBlo[i], sBlo[CAP + i], sBlo[2 * CAP + i]);
+                            hi = mk3(sBhi[i], sBhi[CAP + i], sBhi[2 * CAP + i]);
+                        }
+                        int cnt = 0;
+                        uint64_t mask = ((uint64_t)sOv[NCH + A] << 32) | sOv[A];
+                        int* vl = D.vlist + (size_t)(C.v0 + i) * VERLET_CAP;
+                        while (mask) {
+                            const int B = __builtin_ctzll(mask);
+                            mask &= mask - 1;
+                            uint32_t hm = 0;
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const int j = B * 16 + 4 * g + u;
+                                const int jc = min(j, C.nv - 1);
+                                const bool ov = !(sBlo[jc] > hi.x || sBlo[CAP + jc] > hi.y || sBlo[2 * CAP + jc] > hi.z ||
+                                                  sBhi[jc] < lo.x || sBhi[CAP + jc] < lo.y || sBhi[2 * CAP + jc] < lo.z);
+                                const int dj = j - i;
+                                const bool self = dj == 0 || dj == 1 || dj == -1 || (dj == 2 && nextHN) || (dj == -2 && prevHN);
+                                if (seg && ov && !self && j < C.nv && j >= g0 && j < g1) hm |= 1u << u;
+                            }
+                            const int c = __builtin_popcount(hm);
+                            const int c0 = __shfl(c, a), c1 = __shfl(c, a + 16), c2 = __shfl(c, a + 32), c3 = __shfl(c, a + 48);
+                            int idx = cnt + (g > 0 ? c0 : 0) + (g > 1 ? c1 : 0) + (g > 2 ? c2 : 0);
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                if (hm & (1u << u)) {
+                                    if (idx < VERLET_CAP) vl[idx] = B * 16 + 4 * g + u;
+                                    idx++;
+                                }
+                            }
+                            cnt += c0 + c1 + c2 + c3;
+                        }
+                        if (g == 0 && i < C.nv) sNcol[i] = cnt <= VERLET_CAP ? cnt : -1;
+                        if (stats && g == 0 && seg) atomicAdd(stats, (unsigned long long)cnt);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < VPT; k++)
+                    if (gi[k] >= 0) vcnt[k] = sNcol[t + k * BLOCK];
+                if (stats && t == 0) atomicAdd(stats + 3, 1ull);
+            }
+            // exact tests over the skin list (or over the whole contact group after a skin-list overflow)
+            float minD[VPT];
 #pragma unroll
             for (int k = 0; k < VPT; k++) {
                 const int sl = t + k * BLOCK;
@@ -539,10 +684,6 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 ncol[k] = 0;
                 if (gi[k] >= 0 && (flags_of(prm[k]) & HAS_NEXT)) {
                     const int i = gi[k];
-                    const int G = P.groupRopes;
-                    const int g = rope[k] / G;
-                    const int g0 = D.roff[min(g * G, P.nropes)] - C.v0, g1 = D.roff[min((g + 1) * G, P.nropes)] - C.v0;
-                    const int rs0 = D.roff[rope[k]] - C.v0, rs1 = D.roff[rope[k] + 1] - C.v0;
                     const f3 a0 = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
                     const f3 x1 = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
                     const f3 a1 = x1 - a0;
@@ -551,45 +692,45 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     const float mg = 1e-4f + 2e-6f * fmaxf(fabsf(a0.x), fmaxf(fabsf(a0.y), fabsf(a0.z)));
                     const float rm = ri + mg;
                     const f3 lo = vmin(a0, x1) - mk3(rm, rm, rm), hi = vmax(a0, x1) + mk3(rm, rm, rm);
-                    int cnt = 0, hits = 0, tests = 0;
-                    for (int ch = g0 >> 4; ch <= (g1 - 1) >> 4; ch++) {
-                        if (sBox[ch] > hi.x || sBox[NCH + ch] > hi.y || sBox[2 * NCH + ch] > hi.z ||
-                            sBox[3 * NCH + ch] < lo.x || sBox[4 * NCH + ch] < lo.y || sBox[5 * NCH + ch] < lo.z)
+                    const int* vl = D.vlist + (size_t)i * VERLET_CAP;
+                    const bool full = vcnt[k] < 0;
+                    const int g0 = grp[k] & 0xFFFF, g1 = grp[k] >> 16;
+                    const int rs0 = rng[k] & 0xFFFF, rs1 = rng[k] >> 16;
+                    const int n = full ? g1 - g0 : vcnt[k];
+                    int cnt = 0, tests = 0;
+                    for (int c = 0; c < n; c++) {
+                        const int j = full ? g0 + c : vl[c];
+                        if (full && (!(sFl[j] & HAS_NEXT) || (j >= rs0 && j < rs1 && abs(j - sl) <= 2))) continue;
+                        const f3 c0 = mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]);
+                        const f3 c1 = mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]);
+                        const float rj = fmaxf(sPad[j], sPad[j + 1]);
+                        const f3 blo = vmin(c0, c1), bhi = vmax(c0, c1);
+                        if (blo.x - rj > hi.x || blo.y - rj > hi.y || blo.z - rj > hi.z || bhi.x + rj < lo.x ||
+                            bhi.y + rj < lo.y || bhi.z + rj < lo.z)
                             continue;
-                        hits++;
-                        const int jb = max(ch << 4, g0), je = min((ch << 4) + 16, g1);
-                        for (int j = jb; j < je; j++) {
-                            if (!(sFl[j] & HAS_NEXT)) continue;
-                            if (j >= rs0 && j < rs1 && abs(j - sl) <= 2) continue;
-                            const f3 c0 = mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]);
-                            const f3 c1 = mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]);
-                            const float rj = fmaxf(sPad[j], sPad[j + 1]);
-                            const f3 blo = vmin(c0, c1), bhi = vmax(c0, c1);
-                            if (blo.x - rj > hi.x || blo.y - rj > hi.y || blo.z - rj > hi.z || bhi.x + rj < lo.x ||
-                                bhi.y + rj < lo.y || bhi.z + rj < lo.z)
-                                continue;
-                            tests++;
-                            float d2;
-                            if (pair_hit(a1, c0 - a0, c1 - a0, ri + rj, d2)) {
-                                if (cnt < MAX_COLL) {
-                                    D.coll[(size_t)i * MAX_COLL + cnt] = C.v0 + j;
-                                    minDist = fminf(minDist, 0.45f * sqrtf(d2));
-                                }
-                                cnt++;
+                        tests++;
+                        float d2;
+                        if (pair_hit(a1, c0 - a0, c1 - a0, ri + rj, d2)) {
+                            if (cnt < MAX_COLL) {
+                                D.coll[(size_t)i * MAX_COLL + cnt] = C.v0 + j;
+                                minDist = fminf(minDist, 0.45f * sqrtf(d2));
                             }
+                            cnt++;
                         }
                     }
-                    if (stats) {  // statistics only: chunk overlaps, exact tests, list entries, segments
-                        atomicAdd(stats, (unsigned long long)hits);
+                    if (stats) {  // statistics only: [skin entries at builds, exact tests, list entries, builds]
                         atomicAdd(stats + 1, (unsigned long long)tests);
                         atomicAdd(stats + 2, (unsigned long long)cnt);
-                        atomicAdd(stats + 3, 1ull);
                     }
                     if (cnt > MAX_COLL) D.err[0] = ERR_MAX_COLLISIONS;
                     ncol[k] = min(cnt, MAX_COLL);
                 }
-                sLim[sl] = minDist;
+                minD[k] = minDist;
             }
+            // sF held the build boxes: wait until every segment has finished with them
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < VPT; k++) sLim[t + k * BLOCK] = minD[k];
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < VPT; k++) {
@@ -599,6 +740,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     if (flags_of(prm[k]) & HAS_PREV) limEff[k] = fminf(limEff[k], sLim[sl - 1]);
                 }
             }
+            __syncthreads();  // the step limits are read before sF takes the segment terms
         }
 
         // numItr x (position sweep, orientation sweep)
@@ -831,7 +973,7 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
 
 size_t cell_lds_bytes(int block, int vpt) {
     const size_t cap = (size_t)block * vpt;
-    return cap * (4 + 3 + 3 + 9 + 1 + 1 + 1) * sizeof(float) + (cap / 16) * 6 * sizeof(float);
+    return cap * (4 + 3 + 3 + 9 + 1 + 1 + 3 + 1) * sizeof(float) + (cap / 16) * (6 + 2 + 2) * sizeof(float);
 }
 
 template <int B, int V>

@@ -14,6 +14,7 @@ enum : uint32_t { HAS_PREV = 1u, HAS_NEXT = 2u, HAS_NEXT_ORIENT = 4u, FIX_ORIENT
 constexpr int MAX_SDF = 64;    // meshes per batch (YarnBall.h:153 kMaxSdf)
 constexpr int MAX_CAND = 8;    // broad-phase candidates per vertex (YarnBall.h:153 kMaxSdfCand)
 constexpr int MAX_COLL = 128;  // rope-rope candidates per segment (YarnBall.h:38)
+constexpr int VERLET_CAP = 64; // skin-list entries per segment in the persistent kernel
 enum { ERR_NONE = 0, ERR_MAX_COLLISIONS = 1 };
 
 // A signed-distance grid: phi on an (nx, ny, nz) node lattice, C order (i slowest), at
@@ -77,6 +78,7 @@ struct Dev {
     float* maxStep;   // step limit of each segment this substep
     int* ncol;        // [nv]
     int* coll;        // [nv * MAX_COLL] row-major, ascending partner index
+    int* vlist;       // [nv * VERLET_CAP] skin lists of the persistent kernel (cell-local partner slots)
     // meshes
     const Grid* grids;     // [MAX_SDF]
     Pose* poses;           // [nropes * MAX_SDF]
