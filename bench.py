@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--variant", type=int, default=-1, help="frame-kernel variant (-1: autotune)")
     ap.add_argument("--cpu-sample-ropes", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: the rope batch runs on a single device (environments would shard "
@@ -84,6 +86,13 @@ def main():
         tuned_ms = b.autotune(s.dt, s.substeps, True)
     setup_s = time.perf_counter() - t_setup
 
+    if args.copy_calibration_mb > 0:  # a plain copy of known size: reads and writes n bytes each
+        n = args.copy_calibration_mb << 20
+        src = torch.ones(n // 4, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for k in range(W):
         s.step(b, dev[k], lambda a: a)
     torch.cuda.synchronize()

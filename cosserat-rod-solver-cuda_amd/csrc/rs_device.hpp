@@ -98,17 +98,18 @@ __device__ __forceinline__ void surface_contact(const Params& P, f3 n, float gap
 }
 
 // Trilinear SDF sample and the exact gradient of the interpolant (YB/sim/contact.cuh:57-76).
-__device__ __forceinline__ bool sdf_sample(const Grid& g, f3 p, float& phi, f3& grad) {
-    const f3 o = mk3(g.ox, g.oy, g.oz);
-    const f3 u = (p - o) * g.invH;
-    if (!(u.x >= 0.f && u.y >= 0.f && u.z >= 0.f && u.x < float(g.nx - 1) && u.y < float(g.ny - 1) &&
-          u.z < float(g.nz - 1)))
+__device__ __forceinline__ bool sdf_sample(const Grid* gp, f3 p, float& phi, f3& grad) {
+    const f3 o = mk3(gp->ox, gp->oy, gp->oz);
+    const float invH = gp->invH;
+    const int nx = gp->nx, ny = gp->ny, nz = gp->nz;
+    const f3 u = (p - o) * invH;
+    if (!(u.x >= 0.f && u.y >= 0.f && u.z >= 0.f && u.x < float(nx - 1) && u.y < float(ny - 1) && u.z < float(nz - 1)))
         return false;
     const int i = (int)u.x, j = (int)u.y, k = (int)u.z;
-    const f3 fr = (p - (o + mk3(float(i), float(j), float(k)) * g.h)) * g.invH;
+    const f3 fr = (p - (o + mk3(float(i), float(j), float(k)) * gp->h)) * invH;
     const float fx = fminf(fmaxf(fr.x, 0.f), 1.f), fy = fminf(fmaxf(fr.y, 0.f), 1.f), fz = fminf(fmaxf(fr.z, 0.f), 1.f);
-    const int sy = g.nz, sx = g.ny * g.nz;
-    const float* c = g.v + (size_t)i * sx + (size_t)j * sy + k;
+    const int sy = nz, sx = ny * nz;
+    const float* c = gp->v + (size_t)i * sx + (size_t)j * sy + k;
     const float c000 = __ldg(c), c001 = __ldg(c + 1), c010 = __ldg(c + sy), c011 = __ldg(c + sy + 1);
     const float c100 = __ldg(c + sx), c101 = __ldg(c + sx + 1), c110 = __ldg(c + sx + sy), c111 = __ldg(c + sx + sy + 1);
     const float dx00 = c100 - c000, dx01 = c101 - c001, dx10 = c110 - c010, dx11 = c111 - c011;
@@ -117,25 +118,31 @@ __device__ __forceinline__ bool sdf_sample(const Grid& g, f3 p, float& phi, f3& 
     const float c0 = c00 + fy * dy0, c1 = c01 + fy * dy1;
     phi = c0 + fz * (c1 - c0);
     const float gx0 = dx00 + fy * (dx10 - dx00), gx1 = dx01 + fy * (dx11 - dx01);
-    grad = mk3(gx0 + fz * (gx1 - gx0), dy0 + fz * (dy1 - dy0), c1 - c0) * g.invH;
+    grad = mk3(gx0 + fz * (gx1 - gx0), dy0 + fz * (dy1 - dy0), c1 - c0) * invH;
     return true;
 }
 
-// Contact of a vertex with one posed mesh at the probe (pos - P.pos) + dx (YB/sim/contact.cuh:94-110).
-__device__ __forceinline__ bool sdf_contact(const Params& Pm, const Grid& g, const Pose& P, f3 p, f3 d, f3& n,
+// Contact of a vertex with one posed mesh at the probe (pos - P.pos) + dx (YB/sim/contact.cuh:94-110). Reads
+// only the pose fields it needs (the velocities only while meshes move).
+__device__ __forceinline__ bool sdf_contact(const Params& Pm, const Grid* gp, const Pose* P, f3 p, f3 d, f3& n,
                                             float& gap, f3& dRel) {
-    const q4 q{P.qx, P.qy, P.qz, P.qw};
-    const f3 pp = mk3(P.px, P.py, P.pz);
+    const float4* s = reinterpret_cast<const float4*>(P);
+    const float4 a = s[0], b = s[1];
+    const q4 q{a.w, b.x, b.y, b.z};
+    const f3 pp = mk3(a.x, a.y, a.z);
     float phi;
     f3 gr;
-    if (!sdf_sample(g, qrot(qconj(q), (p - pp) + d), phi, gr)) return false;
+    if (!sdf_sample(gp, qrot(qconj(q), (p - pp) + d), phi, gr)) return false;
     gap = phi - Pm.cRadius;
     if (gap >= 0.f) return false;
     const float gl = len(gr);
     if (!(gl > 0.f)) return false;
     n = qrot(q, gr / gl);
     dRel = d;
-    if (Pm.sdfMoving) dRel = dRel - Pm.h * (mk3(P.vx, P.vy, P.vz) + cross(mk3(P.wx, P.wy, P.wz), p - pp));
+    if (Pm.sdfMoving) {
+        const float4 c = s[2], e = s[3];
+        dRel = dRel - Pm.h * (mk3(b.w, c.x, c.y) + cross(mk3(c.z, c.w, e.x), p - pp));
+    }
     return true;
 }
 
@@ -155,12 +162,10 @@ __device__ __forceinline__ void surfaces_at(const Params& P, const Grid* grids, 
         const int cnt = ncand < 0 ? P.numSdf : ncand;
         for (int c = 0; c < cnt; c++) {
             const int m = ncand < 0 ? c : cand_get(cand, c);
-            const Grid g = grids[m];
-            const Pose Ps = load_pose(envPoses + m);
             f3 n, dRel;
             float gap;
-            if (!sdf_contact(P, g, Ps, p, d, n, gap, dRel)) continue;
-            surface_contact(P, n, gap, dRel, k, g.muS, g.muK, f, H);
+            if (!sdf_contact(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
+            surface_contact(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, f, H);
         }
     }
 }
@@ -181,15 +186,14 @@ __device__ __forceinline__ f3 surface_readout(const Params& P, const Grid* grids
         const int cnt = ncand < 0 ? P.numSdf : ncand;
         for (int c = 0; c < cnt; c++) {
             const int m = ncand < 0 ? c : cand_get(cand, c);
-            const Grid g = grids[m];
-            const Pose Ps = load_pose(envPoses + m);
             f3 n, dRel;
             float gap;
-            if (!sdf_contact(P, g, Ps, p, d, n, gap, dRel)) continue;
+            if (!sdf_contact(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
             f3 fm = mk3(0.f, 0.f, 0.f);
-            surface_contact(P, n, gap, dRel, k, g.muS, g.muK, fm, H);
+            surface_contact(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, fm, H);
             fv = fv + fm;
-            const f3 r = ((p - mk3(Ps.px, Ps.py, Ps.pz)) + d) - P.cRadius * n;
+            const Pose* Pp = envPoses + m;
+            const f3 r = ((p - mk3(Pp->px, Pp->py, Pp->pz)) + d) - P.cRadius * n;
             const f3 tq = cross(r, -fm);
             float* w = envWrench + m * 6;
             atomicAdd(w + 0, -fm.x);
@@ -209,13 +213,16 @@ __device__ __forceinline__ int build_candidates(const Params& P, const Pose* env
     int n = 0;
     packed = 0;
     for (int m = 0; m < P.numSdf; m++) {
-        const Pose Ps = load_pose(envPoses + m);
-        const float pm = padv + (P.sdfMoving ? P.h * (len(mk3(Ps.vx, Ps.vy, Ps.vz)) +
-                                                      len(mk3(Ps.wx, Ps.wy, Ps.wz)) * 0.5f *
-                                                          len(mk3(Ps.hix, Ps.hiy, Ps.hiz) - mk3(Ps.lox, Ps.loy, Ps.loz)))
-                                             : 0.f);
-        if (p.x >= Ps.lox - pm && p.x <= Ps.hix + pm && p.y >= Ps.loy - pm && p.y <= Ps.hiy + pm &&
-            p.z >= Ps.loz - pm && p.z <= Ps.hiz + pm) {
+        const float4* s = reinterpret_cast<const float4*>(envPoses + m);
+        const float4 e = s[3], f = s[4];
+        const f3 lo = mk3(e.y, e.z, e.w), hi = mk3(f.x, f.y, f.z);
+        float pm = padv;
+        if (P.sdfMoving) {
+            const float4 b = s[1], c = s[2];
+            pm = padv + P.h * (len(mk3(b.w, c.x, c.y)) + len(mk3(c.z, c.w, e.x)) * 0.5f * len(hi - lo));
+        }
+        if (p.x >= lo.x - pm && p.x <= hi.x + pm && p.y >= lo.y - pm && p.y <= hi.y + pm && p.z >= lo.z - pm &&
+            p.z <= hi.z + pm) {
             if (n < MAX_CAND) packed |= (uint64_t)m << (6 * n);
             n++;
         }
