@@ -9,7 +9,10 @@ programs and mesh targets) are resident on the GPU before the timed region.
 
 One JSON line: metric, value, ms_per_step, roofline of the persistent frame kernel (algorithmic bytes
 per launch over its HIP-event-timed duration on the solver stream), a VALU figure, and the CPU
-baseline (the float32 oracle on the host cores, a bounded sample of the same workload).
+baseline (the float32 oracle on the host cores, a bounded sample of the same workload). The
+"differentiation" object is the second half of the metric, on the config-4 workload:
+- forward time of the per-substep kernels and of the persistent frame kernel;
+- a differentiable rollout's forward (with per-frame checkpoints) and adjoint time, and their ratio.
 """
 import argparse
 import json
@@ -50,6 +53,83 @@ def cpu_baseline(sample_ropes, threads):
                       f"environments) of the same generator, 1 frame of {s.substeps} substeps, {dt:.1f} s"}
 
 
+def differentiation_bench(n_ropes, frames):
+    """Config-4 workload: forward throughput of the plain per-substep kernels against the persistent frame
+    kernel, and the differentiable rollout (forward with per-frame checkpoints, then the adjoint over every
+    frame) with its backward/forward time ratio."""
+    import numpy as np
+    import torch
+
+    import ropesim
+    from ropesim import scenes
+    s = scenes.config4(n_ropes=n_ropes, frames=frames)
+    b = s.build(ropesim)
+    fis = s.frame_inputs(frames)
+    dev = lambda k: torch.tensor(np.stack([f[k] for f in fis]), device="cuda")
+    P0, V0, P1 = dev("p0"), dev("v0"), dev("p1")
+    units = s.num_vertices * s.substeps * frames
+    all_envs = np.arange(s.num_ropes, dtype=np.int32)
+
+    def forward_frames(mode):
+        b.reset_envs(all_envs)
+        b.set_dispatch(mode, -1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(frames):
+            b.step(P0[f], V0[f], s.dt, s.substeps, root_pos_end=P1[f])
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    forward_frames(0)  # warm-up (module load, first launches)
+    t_plain = forward_frames(0)
+    b.set_dispatch(1, -1)
+    b.autotune(s.dt, s.substeps, True)
+    variant = b.dispatch
+    forward_frames(1)
+    t_persist = forward_frames(1)
+    b.set_dispatch(*variant)
+
+    w = np.asarray(b.inv_masses(), np.float64).reshape(-1)
+    mass = torch.tensor(np.where(w > 0, 1.0 / np.where(w > 0, w, 1.0), 1.0), dtype=torch.float32, device="cuda")
+    ea = torch.full((s.num_vertices,), s.k_stretch, device="cuda", requires_grad=True)
+    ei = torch.full((s.num_vertices,), s.k_bend, device="cuda", requires_grad=True)
+    mass.requires_grad_(True)
+    drag = torch.tensor(s.drag, device="cuda", requires_grad=True)
+    p1 = P1.clone().requires_grad_(True)
+    tips = torch.as_tensor(s.offsets[1:] - 1, device="cuda", dtype=torch.long)
+    target = torch.as_tensor(s.positions[s.offsets[1:] - 1], device="cuda")[None] + 0.05
+    fwds, bwds = [], []
+    for rep in range(5):  # two warm-ups (first-use allocations of torch's caching allocator), three timed
+        for t in (ea, ei, mass, drag, p1):
+            t.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = ropesim.diff.rollout(b, ea, ei, mass, drag, P0, V0, p1, s.dt, s.substeps)
+        loss = ((x[:, tips] - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if rep >= 2:
+            fwds.append(t1 - t0)
+            bwds.append(t2 - t1)
+    assert all(bool(torch.isfinite(g).all()) for g in (ea.grad, ei.grad, mass.grad, drag.grad, p1.grad))
+    fwd, bwd = float(np.median(fwds)), float(np.median(bwds))
+    return {"workload": f"config4: {n_ropes} ropes x 65 vertices (config-1 material), plane + one static sphere SDF "
+                        f"(48^3) per rope, root program, {frames} frames x {s.substeps} substeps x {s.num_itr} "
+                        "iterations, tip-tracking loss",
+            "frames": frames, "vertex_substeps": units,
+            "plain_kernels_ms_per_frame": 1e3 * t_plain / frames,
+            "persistent_kernel_ms_per_frame": 1e3 * t_persist / frames,
+            "persistent_speedup_vs_plain": t_plain / t_persist,
+            "persistent_dispatch": list(variant),
+            "rollout_forward_ms_per_frame": 1e3 * fwd / frames,
+            "rollout_forward_vertex_substeps_per_s": units / fwd,
+            "adjoint_ms_per_frame": 1e3 * bwd / frames,
+            "adjoint_forward_ratio": bwd / fwd, "timing": "median of 3 rollouts after 2 warm-ups"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +140,8 @@ def main():
     ap.add_argument("--variant", type=int, default=-1, help="frame-kernel variant (-1: autotune)")
     ap.add_argument("--cpu-sample-ropes", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--diff-ropes", type=int, default=256, help="config-4 ropes of the differentiation leg")
+    ap.add_argument("--diff-frames", type=int, default=120, help="config-4 frames (0 skips the leg)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -139,6 +221,10 @@ def main():
         tf = fl * units / per_launch_s / 1e12
         line["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP32_PEAK_TFLOPS,
                         "flop_per_vertex_substep": fl, "note": "solver flops only; rope-rope pair tests excluded"}
+    if args.diff_frames > 0:
+        d = differentiation_bench(args.diff_ropes, args.diff_frames)
+        line["differentiation"] = d
+        line["adjoint_forward_ratio"] = d["adjoint_forward_ratio"]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_ropes, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())))
     print(json.dumps(line), flush=True)

@@ -1,0 +1,55 @@
+// Reverse-mode (adjoint) pass of the substep: device buffers and host launchers (rs_adjoint.hip).
+//
+// The reference has no adjoint. The backward pass differentiates exactly the arithmetic of the
+// forward substep (rs_device.hpp), with the forward active set frozen:
+// - contact on / off
+// - friction stick / slide
+// - the gravity acceleration clamp
+// - the orientation sign tests
+// - the broad-phase candidate lists
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rs_types.hpp"
+
+namespace rs {
+
+struct AdjDev {
+    // frame checkpoint: state at the start of the frame [nv], and at the start of the next one (recompute check)
+    const float4 *cx, *cv, *cvl, *cq;
+    const float4 *nx, *nv, *nvl, *nq;
+    // history of the frame for the ropes [rope0, rope0 + gridDim.x) of a launch, local vertex i - hv0 of hn
+    int rope0, hv0, hn;
+    float4 *hX, *hV, *hVL, *hY;  // [S][hn] substep inputs: x after the root program (w: broad-phase count bits),
+                                 // v after the root program, last velocity before init, inertial target y
+    uint2* hC;                   // [S][hn] packed broad-phase candidate ids
+    float4 *hD, *hQ;             // [S][I + 1][hn] iterates d^k and frames q^k
+    float* hH;                   // [S][I][9][hn] total Hessian (packed 6) and Newton direction H^-1 F (3)
+    // adjoint of the state at the end of the frame on entry, at its start on exit [nv]
+    float4 *ax, *av, *avl, *aq;
+    const float* dlx;  // dL/dx of this frame's output positions [nv * 3], may be null
+    // accumulated parameter adjoints [nv]: k = EA l, rest Darboux vector [nv][4], mass; drag per rope [N]
+    double* gk;
+    double* grho;
+    double* gm;
+    double* gdrag;
+    // this frame's root-program adjoints [N * 3] (p0, v0, p1)
+    float *gp0, *gv0, *gp1;
+    int* mismatch;  // count of vertices whose recomputed end state differs from the next checkpoint
+    const Pose* fposes;     // frame-start poses [N][numSdf]
+    const Motion* fmotion;  // frame motion [N][numSdf] while meshes move, else null
+};
+
+// Largest workgroup (= longest rope) the backward kernels support; history bytes per vertex-substep.
+constexpr int ADJ_MAX_BLOCK = 256;
+size_t adjoint_history_bytes(int numItr);
+int adjoint_block_for(int maxRopeVerts);  // 64 / 128 / 256, or -1
+// Both passes for the ropes [A.rope0, A.rope0 + nropes).
+hipError_t launch_adjoint_frame(int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
+                                int nropes, hipStream_t s);
+hipError_t launch_adjoint_finish(const Params& P, const Dev& D, const float4* rhoUnit, const double* gk,
+                                 const double* grho, const double* gm, float* gea, float* gei, float* gmass,
+                                 hipStream_t s);
+
+}  // namespace rs

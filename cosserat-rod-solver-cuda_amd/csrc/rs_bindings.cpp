@@ -317,6 +317,58 @@ class RopeBatch : public Core {
 public:
     bool stepped = false;
 
+    // Differentiation form (no reference counterpart): material between rollouts, recording, backward pass.
+    void set_material(py::object ea, py::object ei, py::object mass) {
+        const py::ssize_t n = nv;
+        bool dev = false, host = false;
+        for (const py::object* o : {&ea, &ei, &mass})
+            if (!o->is_none()) (py::hasattr(*o, "__cuda_array_interface__") ? dev : host) = true;
+        if (dev && host) throw py::value_error("set_material: pass every array on the GPU or every array on the host");
+        if (dev) {
+            const float* a = ea.is_none() ? nullptr : parse_cai(ea, {n}, "ea");
+            const float* c = ei.is_none() ? nullptr : parse_cai(ei, {n}, "ei");
+            const float* m = mass.is_none() ? nullptr : parse_cai(mass, {n}, "mass");
+            check(rs_set_material(b(), a, c, m, 1));
+            return;
+        }
+        farray A, C, M;
+        auto host_arr = [&](const py::object& o, farray& keep, const char* name) -> const float* {
+            if (o.is_none()) return nullptr;
+            keep = o.cast<farray>();
+            if (keep.size() != n) throw py::value_error(std::string(name) + " must hold num_vertices values");
+            return keep.data();
+        };
+        const float* a = host_arr(ea, A, "ea");
+        const float* c = host_arr(ei, C, "ei");
+        const float* m = host_arr(mass, M, "mass");
+        check(rs_set_material(b(), a, c, m, 0));
+    }
+    void tape_begin(int max_frames) { check(rs_tape_begin(b(), max_frames)); }
+    void tape_end() { check(rs_tape_end(b())); }
+    int tape_frames() const { return rs_tape_frames(b()); }
+    py::tuple adjoint(py::object dl_dx, py::object g_ea, py::object g_ei, py::object g_mass, py::object g_p0,
+                      py::object g_v0, py::object g_p1) {
+        const py::ssize_t F = rs_tape_frames(b()), n = nv, R = N;
+        const float* dl = parse_cai(dl_dx, {F, n, 3}, "dl_dx");
+        auto out = [&](const py::object& o, const std::vector<py::ssize_t>& shape, const char* name) -> float* {
+            return o.is_none() ? nullptr : const_cast<float*>(parse_cai(o, shape, name));
+        };
+        float* ge = out(g_ea, {n}, "g_ea");
+        float* gi = out(g_ei, {n}, "g_ei");
+        float* gm = out(g_mass, {n}, "g_mass");
+        float* gp0 = out(g_p0, {F, R, 3}, "g_p0");
+        float* gv0 = out(g_v0, {F, R, 3}, "g_v0");
+        float* gp1 = out(g_p1, {F, R, 3}, "g_p1");
+        double gd = 0.0;
+        int mis = 0, rc;
+        {
+            py::gil_scoped_release rel;
+            rc = rs_adjoint(b(), dl, ge, gi, gm, &gd, gp0, gv0, gp1, &mis);
+        }
+        check(rc);
+        return py::make_tuple(gd, mis);
+    }
+
     void step(py::object root_pos, py::object root_vel, float dt, int substeps, py::object root_q,
               py::object root_pos_end, py::object root_q_end) {
         if (!(dt > 0)) throw py::value_error("dt must be > 0");
@@ -798,6 +850,19 @@ PYBIND11_MODULE(_ropesim, m) {
              "0: one launch per kernel per substep; 1: one persistent frame kernel (variant -1: automatic).")
         .def_property_readonly("dispatch", &RopeBatch::dispatch)
         .def("autotune", &RopeBatch::autotune, py::arg("dt"), py::arg("substeps"), py::arg("program") = true)
+        .def("set_material", &RopeBatch::set_material, py::arg("ea") = py::none(), py::arg("ei") = py::none(),
+             py::arg("mass") = py::none(),
+             "Per-vertex EA, EI (per segment, last vertex ignored) and node masses (pinned ignored); GPU or host "
+             "arrays of num_vertices float32; allowed between rollouts")
+        .def("tape_begin", &RopeBatch::tape_begin, py::arg("max_frames"),
+             "Record a checkpoint at every following step (up to max_frames) for adjoint()")
+        .def("tape_end", &RopeBatch::tape_end)
+        .def("tape_frames", &RopeBatch::tape_frames)
+        .def("adjoint", &RopeBatch::adjoint, py::arg("dl_dx"), py::arg("g_ea") = py::none(),
+             py::arg("g_ei") = py::none(), py::arg("g_mass") = py::none(), py::arg("g_p0") = py::none(),
+             py::arg("g_v0") = py::none(), py::arg("g_p1") = py::none(),
+             "Backward pass over the recorded frames: dl_dx [frames, num_vertices, 3] in, gradients written into "
+             "the given GPU arrays; returns (dL/d drag, recompute mismatches)")
         .def("set_kernel_timing", [](RopeBatch& b, bool on) { check(rs_set_kernel_timing(b.b(), on ? 1 : 0)); },
              py::arg("on") = true)
         .def("kernel_time",

@@ -97,8 +97,10 @@ __device__ __forceinline__ void surface_contact(const Params& P, f3 n, float gap
     ssub(H, sscale(nn, kt));
 }
 
-// Trilinear SDF sample and the exact gradient of the interpolant (YB/sim/contact.cuh:57-76).
-__device__ __forceinline__ bool sdf_sample(const Grid* gp, f3 p, float& phi, f3& grad) {
+// Trilinear SDF sample and the exact gradient of the interpolant (YB/sim/contact.cuh:57-76). With MIXED the
+// interpolant's mixed second derivatives (d2/dxdy, d2/dxdz, d2/dydz; the pure ones vanish) are returned too.
+template <bool MIXED>
+__device__ __forceinline__ bool sdf_sample_t(const Grid* gp, f3 p, float& phi, f3& grad, f3& mixed) {
     const f3 o = mk3(gp->ox, gp->oy, gp->oz);
     const float invH = gp->invH;
     const int nx = gp->nx, ny = gp->ny, nz = gp->nz;
@@ -119,7 +121,15 @@ __device__ __forceinline__ bool sdf_sample(const Grid* gp, f3 p, float& phi, f3&
     phi = c0 + fz * (c1 - c0);
     const float gx0 = dx00 + fy * (dx10 - dx00), gx1 = dx01 + fy * (dx11 - dx01);
     grad = mk3(gx0 + fz * (gx1 - gx0), dy0 + fz * (dy1 - dy0), c1 - c0) * invH;
+    if (MIXED) {
+        const float i2 = invH * invH;
+        mixed = mk3(((dx10 - dx00) + fz * ((dx11 - dx01) - (dx10 - dx00))) * i2, (gx1 - gx0) * i2, (dy1 - dy0) * i2);
+    }
     return true;
+}
+__device__ __forceinline__ bool sdf_sample(const Grid* gp, f3 p, float& phi, f3& grad) {
+    f3 unused;
+    return sdf_sample_t<false>(gp, p, phi, grad, unused);
 }
 
 // Contact of a vertex with one posed mesh at the probe (pos - P.pos) + dx (YB/sim/contact.cuh:94-110). Reads

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "ropesim_hip.h"
+#include "rs_adjoint.hpp"
 #include "rs_kernels.hpp"
 #include "rs_math.hpp"
 #include "rs_types.hpp"
@@ -22,6 +23,8 @@ using namespace rs;
 namespace {
 
 thread_local std::string g_msg;
+// Device memory for one frame's adjoint history; larger batches are differentiated in chunks of ropes.
+constexpr size_t kAdjointHistoryBudget = size_t(2) << 30;
 
 struct Invalid : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -136,6 +139,28 @@ struct rs_batch {
     uint8_t* d_stageTele = nullptr;
     bool pendingSet = false, moving = false, sdfMoving = false, forcesOn = false, meshesTouched = false;
     int lastSubsteps = 1;
+    // differentiation: unit rest Darboux directions, the recorded rollout (per-frame checkpoints) and the
+    // adjoint's buffers
+    std::vector<float4> hRhoUnit;
+    float4* d_rhoUnit = nullptr;
+    struct Tape {
+        bool on = false, closed = false;
+        int maxFrames = 0, frames = 0, numSdf = 0, maxS = 0;
+        float4* ck = nullptr;      // [(maxFrames + 1)][4][nv]: x, v, v_last, q at each frame start (+ the end)
+        float* prog = nullptr;     // [maxFrames][17 N] staged root programs
+        Pose* poses = nullptr;     // [maxFrames][N][numSdf]
+        Motion* motion = nullptr;  // [maxFrames][N][numSdf]
+        std::vector<Params> P;
+        std::vector<Drive> dr;
+        std::vector<int> S;
+    } tape;
+    float4* d_adj = nullptr;      // [4][nv] state adjoint
+    char* d_hist = nullptr;       // history of one frame (a chunk of ropes at a time)
+    size_t histBytes = 0;
+    double *d_gk = nullptr, *d_gm = nullptr, *d_grho = nullptr;
+    float* d_gp = nullptr;
+    double* d_gdrag = nullptr;
+    int* d_mismatch = nullptr;
 
     ~rs_batch() {
         if (stream) hipStreamSynchronize(stream);
@@ -146,7 +171,9 @@ struct rs_batch {
         void* ptrs[] = {D.x, D.v, D.vlast, D.q, D.q2, D.dx, D.dx2, D.prm, D.qrest, D.ropeOf, d_roff, D.pad, D.maxStep,
                         D.ncol, D.coll, D.vlist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.outPos,
                         D.outVel, D.outQuat, D.err, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_hits,
-                        d_grids, d_cur, d_tgt, d_tele, d_stage, d_stageTele};
+                        d_grids, d_cur, d_tgt, d_tele, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
+                        tape.poses, tape.motion, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
+                        d_gdrag, d_mismatch};
         for (void* p : ptrs)
             if (p) hipFree(p);
         for (float* g : gridMem) hipFree(g);
@@ -285,6 +312,11 @@ struct rs_batch {
         }
         initX = hx;
         initQ = hq;
+        hRhoUnit.resize(nv);
+        for (int i = 0; i < nv; i++) {
+            const q4 u = qunit_or_identity(qRest[i]);
+            hRhoUnit[i] = make_float4(u.x, u.y, u.z, u.w);
+        }
         allocate();
         upload_all();
     }
@@ -316,6 +348,7 @@ struct rs_batch {
         d_ids = dalloc<int>(N, "ids");
         d_prog = dalloc<float>(17 * (size_t)N, "prog");
         d_hits = dalloc<unsigned long long>(4, "stats");
+        d_rhoUnit = dalloc<float4>(n, "rho unit");
         hcheck(hipMemset(D.err, 0, 2 * sizeof(int)), "memset err");
         hcheck(hipMemset(D.vlast, 0, n * sizeof(float4)), "memset vlast");
         hcheck(hipMemset(D.ncol, 0, n * sizeof(int)), "memset ncol");
@@ -333,6 +366,7 @@ struct rs_batch {
         hcheck(hipMemcpy(d_roff, roff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice), "upload roff");
         hcheck(hipMemcpy(d_initX, initX.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload initX");
         hcheck(hipMemcpy(d_initQ, initQ.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload initQ");
+        hcheck(hipMemcpy(d_rhoUnit, hRhoUnit.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload rho unit");
         gather_nolock();
         hcheck(hipStreamSynchronize(stream), "construct");
     }
@@ -586,6 +620,7 @@ struct rs_batch {
         dr.dt = dt;
         begin_step();
         prepare(dt / float(S));
+        if (tape.on) tape_record(dr, S);
         stepped = true;
         run_substeps(dr, S);
         time = time + dt;
@@ -606,6 +641,59 @@ struct rs_batch {
         }
     }
     void gather_nolock() { hcheck(launch_gather(params(), D, stream), "gather"); }
+
+    // ---------------------------------------------------------------- differentiation
+    void tape_check() const {
+        if (ropeContact) throw Invalid("the adjoint does not cover rope-rope contact; disable it while recording");
+        if (dragQuad > 0.f) throw Invalid("the adjoint does not cover quadratic drag; set drag_quad = 0 while recording");
+        if ((int)grids.size() != tape.numSdf) throw Invalid("meshes were added or cleared while recording");
+        int maxV = 0;
+        for (int r = 0; r < N; r++) maxV = std::max(maxV, roff[r + 1] - roff[r]);
+        if (adjoint_block_for(maxV) < 0) throw Invalid("the adjoint supports ropes of at most 256 vertices");
+    }
+    void copy_state(float4* dst) {
+        const size_t n = nv;
+        hcheck(hipMemcpyAsync(dst, D.x, n * 16, hipMemcpyDeviceToDevice, stream), "checkpoint");
+        hcheck(hipMemcpyAsync(dst + n, D.v, n * 16, hipMemcpyDeviceToDevice, stream), "checkpoint");
+        hcheck(hipMemcpyAsync(dst + 2 * n, D.vlast, n * 16, hipMemcpyDeviceToDevice, stream), "checkpoint");
+        hcheck(hipMemcpyAsync(dst + 3 * n, D.q, n * 16, hipMemcpyDeviceToDevice, stream), "checkpoint");
+    }
+    void tape_record(const Drive& dr, int S) {
+        tape_check();
+        if (tape.frames >= tape.maxFrames) throw Invalid("the tape is full (max_frames of rs_tape_begin)");
+        const int f = tape.frames;
+        copy_state(tape.ck + (size_t)f * 4 * nv);
+        hcheck(hipMemcpyAsync(tape.prog + (size_t)f * 17 * N, d_prog, 17 * (size_t)N * 4, hipMemcpyDeviceToDevice,
+                              stream),
+               "record program");
+        const int M = tape.numSdf;
+        if (M > 0) {
+            hcheck(hipMemcpy2DAsync(tape.poses + (size_t)f * N * M, M * sizeof(Pose), D.poses, MAX_SDF * sizeof(Pose),
+                                    M * sizeof(Pose), N, hipMemcpyDeviceToDevice, stream),
+                   "record poses");
+            if (sdfMoving)
+                hcheck(hipMemcpy2DAsync(tape.motion + (size_t)f * N * M, M * sizeof(Motion), D.motion,
+                                        MAX_SDF * sizeof(Motion), M * sizeof(Motion), N, hipMemcpyDeviceToDevice,
+                                        stream),
+                       "record motion");
+        }
+        tape.P.push_back(params());
+        tape.dr.push_back(dr);
+        tape.S.push_back(S);
+        tape.maxS = std::max(tape.maxS, S);
+        tape.frames++;
+    }
+    void tape_close() {
+        if (!tape.on) return;
+        copy_state(tape.ck + (size_t)tape.frames * 4 * nv);  // the state after the last recorded frame
+        tape.on = false;
+        tape.closed = true;
+    }
+    void tape_free() {
+        for (void* p : {(void*)tape.ck, (void*)tape.prog, (void*)tape.poses, (void*)tape.motion})
+            if (p) hipFree(p);
+        tape = Tape{};
+    }
 };
 
 // ==================================================================== C ABI
@@ -1308,6 +1396,201 @@ int rs_set_chunk_stats(rs_batch* b, int on, long long* out4) {
         if (out4)
             for (int i = 0; i < 4; i++) out4[i] = (long long)h[i];
         b->countHits = on != 0;
+    });
+}
+
+// ------------------------------------------------------------------ differentiation form (no reference counterpart)
+
+int rs_set_material(rs_batch* b, const float* ea, const float* ei, const float* mass, int on_device) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        const int n = b->nv;
+        if (b->tape.on) throw Invalid("the material cannot change while a rollout is being recorded");
+        std::vector<float> hea, hei, hm;
+        auto fetch = [&](const float* src, std::vector<float>& dst) -> const float* {
+            if (!src) return nullptr;
+            if (!on_device) return src;
+            dst.resize(n);
+            hcheck(hipStreamSynchronize(b->stream), "sync");
+            hcheck(hipMemcpy(dst.data(), src, n * sizeof(float), hipMemcpyDeviceToHost), "read material");
+            return dst.data();
+        };
+        const float* EA = fetch(ea, hea);
+        const float* EI = fetch(ei, hei);
+        const float* M = fetch(mass, hm);
+        for (int r = 0; r < b->N; r++)
+            for (int i = b->roff[r]; i < b->roff[r + 1]; i++) {
+                const bool seg = i + 1 < b->roff[r + 1];
+                if (M && b->hprm[i].x != 0.f && (!std::isfinite(M[i]) || !(M[i] > 0)))
+                    throw Invalid("node masses must be finite and > 0 (pinned entries ignored)");
+                if (seg && EA && (!std::isfinite(EA[i]) || !(EA[i] > 0))) throw Invalid("EA values must be finite and > 0");
+                if (seg && EI && (!std::isfinite(EI[i]) || !(EI[i] > 0))) throw Invalid("EI values must be finite and > 0");
+            }
+        hcheck(hipStreamSynchronize(b->stream), "sync");
+        for (int r = 0; r < b->N; r++)
+            for (int i = b->roff[r]; i < b->roff[r + 1]; i++) {
+                const bool seg = i + 1 < b->roff[r + 1];
+                float4& p = b->hprm[i];
+                if (M && p.x != 0.f) p.x = 1.f / M[i];
+                if (seg && EA) p.z = EA[i] * p.y;
+                if (seg && EI) {
+                    const float k = 4.f * EI[i] / p.y;
+                    const float4 u = b->hRhoUnit[i];
+                    b->hqrest[i] = make_float4(k * u.x, k * u.y, k * u.z, k * u.w);
+                }
+            }
+        hcheck(hipMemcpy(b->D.prm, b->hprm.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload prm");
+        hcheck(hipMemcpy(b->D.qrest, b->hqrest.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload qrest");
+    });
+}
+
+int rs_tape_begin(rs_batch* b, int max_frames) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (max_frames < 1) throw Invalid("max_frames must be >= 1");
+        b->tape_free();
+        b->tape.numSdf = (int)b->grids.size();
+        b->tape_check();
+        const size_t n = b->nv;
+        b->tape.maxFrames = max_frames;
+        b->tape.ck = dalloc<float4>((size_t)(max_frames + 1) * 4 * n, "tape checkpoints");
+        b->tape.prog = dalloc<float>((size_t)max_frames * 17 * b->N, "tape programs");
+        if (b->tape.numSdf > 0) {
+            const size_t slots = (size_t)max_frames * b->N * b->tape.numSdf;
+            b->tape.poses = dalloc<Pose>(slots, "tape poses");
+            b->tape.motion = dalloc<Motion>(slots, "tape motion");
+        }
+        b->tape.on = true;
+    });
+}
+
+int rs_tape_end(rs_batch* b) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        b->tape_close();
+    });
+}
+
+int rs_tape_frames(const rs_batch* b) { return b ? b->tape.frames : 0; }
+
+int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float* g_mass, double* g_drag, float* g_p0,
+               float* g_v0, float* g_p1, int* mismatches) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        b->tape_close();
+        auto& T = b->tape;
+        if (!T.closed || T.frames < 1) throw Invalid("no recorded rollout: call rs_tape_begin, then step");
+        const int F = T.frames, N = b->N;
+        const size_t n = b->nv;
+        int maxV = 0;
+        for (int r = 0; r < N; r++) maxV = std::max(maxV, b->roff[r + 1] - b->roff[r]);
+        const int blk = adjoint_block_for(maxV);
+        // the history of a frame is kept for a chunk of ropes at a time, within a fixed budget
+        int maxI = 1;
+        for (const Params& P : T.P) maxI = std::max(maxI, P.numItr);
+        const size_t perV = adjoint_history_bytes(maxI) * (size_t)T.maxS;
+        const size_t budget = std::max(kAdjointHistoryBudget, perV * (size_t)maxV);
+        struct Chunk {
+            int rope0, nropes, v0, nv;
+        };
+        std::vector<Chunk> chunks;
+        for (int r = 0; r < N;) {
+            Chunk c{r, 0, b->roff[r], 0};
+            while (r < N && (size_t)(c.nv + b->roff[r + 1] - b->roff[r]) * perV <= budget) {
+                c.nv += b->roff[r + 1] - b->roff[r];
+                c.nropes++;
+                r++;
+            }
+            chunks.push_back(c);
+        }
+        size_t need = 0;
+        for (const Chunk& c : chunks) need = std::max(need, (size_t)c.nv * perV);
+        if (need > b->histBytes) {
+            if (b->d_hist) hipFree(b->d_hist);
+            b->d_hist = dalloc<char>(need, "adjoint history");
+            b->histBytes = need;
+        }
+        if (!b->d_adj) {
+            b->d_adj = dalloc<float4>(4 * n, "adjoint state");
+            b->d_gk = dalloc<double>(n, "dk");
+            b->d_gm = dalloc<double>(n, "dm");
+            b->d_grho = dalloc<double>(4 * n, "drho");
+            b->d_gdrag = dalloc<double>(N, "ddrag");
+            b->d_gp = dalloc<float>(9 * (size_t)N, "droot");
+            b->d_mismatch = dalloc<int>(1, "mismatch");
+        }
+        hipStream_t st = b->stream;
+        hcheck(hipMemsetAsync(b->d_adj, 0, 4 * n * 16, st), "zero adjoint");
+        hcheck(hipMemsetAsync(b->d_gk, 0, n * 8, st), "zero");
+        hcheck(hipMemsetAsync(b->d_gm, 0, n * 8, st), "zero");
+        hcheck(hipMemsetAsync(b->d_grho, 0, n * 32, st), "zero");
+        hcheck(hipMemsetAsync(b->d_gdrag, 0, N * 8, st), "zero");
+        hcheck(hipMemsetAsync(b->d_mismatch, 0, 4, st), "zero");
+        for (int f = F - 1; f >= 0; f--) {
+            const float4* ck = T.ck + (size_t)f * 4 * n;
+            const float4* nk = T.ck + (size_t)(f + 1) * 4 * n;
+            const int S = T.S[f], I = T.P[f].numItr;
+            Drive dr = T.dr[f];
+            const float* prog = T.prog + (size_t)f * 17 * N;
+            dr.p0 = prog;
+            dr.v0 = prog + 3 * N;
+            dr.p1 = prog + 6 * N;
+            dr.q0 = prog + 9 * N;
+            dr.q1 = prog + 13 * N;
+            for (const Chunk& c : chunks) {
+                AdjDev A{};
+                A.cx = ck;
+                A.cv = ck + n;
+                A.cvl = ck + 2 * n;
+                A.cq = ck + 3 * n;
+                A.nx = nk;
+                A.nv = nk + n;
+                A.nvl = nk + 2 * n;
+                A.nq = nk + 3 * n;
+                A.rope0 = c.rope0;
+                A.hv0 = c.v0;
+                A.hn = c.nv;
+                const size_t Sn = (size_t)S * c.nv;
+                char* p = b->d_hist;
+                A.hX = reinterpret_cast<float4*>(p);
+                A.hV = A.hX + Sn;
+                A.hVL = A.hV + Sn;
+                A.hY = A.hVL + Sn;
+                A.hD = A.hY + Sn;
+                A.hQ = A.hD + Sn * (I + 1);
+                A.hC = reinterpret_cast<uint2*>(A.hQ + Sn * (I + 1));
+                A.hH = reinterpret_cast<float*>(A.hC + Sn);
+                A.ax = b->d_adj;
+                A.av = A.ax + n;
+                A.avl = A.av + n;
+                A.aq = A.avl + n;
+                A.dlx = dl_dx ? dl_dx + (size_t)f * n * 3 : nullptr;
+                A.gk = b->d_gk;
+                A.grho = b->d_grho;
+                A.gm = b->d_gm;
+                A.gdrag = b->d_gdrag;
+                A.gp0 = g_p0 ? g_p0 + (size_t)f * 3 * N : b->d_gp;
+                A.gv0 = g_v0 ? g_v0 + (size_t)f * 3 * N : b->d_gp + 3 * N;
+                A.gp1 = g_p1 ? g_p1 + (size_t)f * 3 * N : b->d_gp + 6 * N;
+                A.mismatch = b->d_mismatch;
+                const int M = T.numSdf;
+                A.fposes = M > 0 ? T.poses + (size_t)f * N * M : nullptr;
+                A.fmotion = (M > 0 && T.P[f].sdfMoving) ? T.motion + (size_t)f * N * M : nullptr;
+                hcheck(launch_adjoint_frame(blk, T.P[f], b->D, A, dr, S, c.nropes, st), "adjoint frame");
+            }
+        }
+        hcheck(launch_adjoint_finish(b->params(), b->D, b->d_rhoUnit, b->d_gk, b->d_grho, b->d_gm, g_ea, g_ei, g_mass,
+                                     st),
+               "adjoint finish");
+        std::vector<double> gd(N);
+        int mis = 0;
+        hcheck(hipMemcpyAsync(gd.data(), b->d_gdrag, N * 8, hipMemcpyDeviceToHost, st), "read drag adjoint");
+        hcheck(hipMemcpyAsync(&mis, b->d_mismatch, 4, hipMemcpyDeviceToHost, st), "read mismatch");
+        hcheck(hipStreamSynchronize(st), "adjoint");
+        double sum = 0.0;
+        for (double v : gd) sum += v;
+        if (g_drag) *g_drag = sum;
+        if (mismatches) *mismatches = mis;
     });
 }
 

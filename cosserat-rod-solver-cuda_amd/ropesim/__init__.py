@@ -5,6 +5,7 @@
              positions / velocities / frames exposed zero-copy through __cuda_array_interface__.
 - CudaView:  the device-array view type.
 - ropesim.sdf: signed-distance grids (closed forms, trilinear host sampler).
+- ropesim.diff: differentiable rollouts (torch.autograd.Function over the hand-written adjoint).
 
 The solver is librope_hip.so (hand-written HIP for gfx950) behind a C ABI; _ropesim is its pybind11
 boundary. Importing fails loudly when the built extension is missing: there is no CPU fallback.
@@ -22,4 +23,7 @@ try:
 except ImportError as e:  # pragma: no cover
     raise ImportError("ropesim: the HIP extension is not built (python -m ropesim._build): " + str(e)) from e
 
-__all__ = ["RopeSim", "RopeBatch", "CudaView", "sdf"]
+if _torch is not None:
+    from . import diff  # noqa: F401,E402
+
+__all__ = ["RopeSim", "RopeBatch", "CudaView", "sdf", "diff"]

@@ -272,8 +272,10 @@ def config3(n_ropes=4096, frames=2, seed=1, n_meshes=8, grid_n=64, vmin=16, vmax
 
 
 def config4(n_ropes=256, V=65, frames=120, seed=4, grid_n=48):
-    """Differentiation form: n_ropes x V (config-1 material: rho 0.1, EA 500, EI 1e-4, 0.02 m segments),
-    h 1/3000 (50 substeps), 8 iterations; plane z = 0 and one static sphere SDF (r 0.1 m, grid_n^3) per rope."""
+    """Differentiation form: n_ropes x V with config-1 material (rho 0.1 kg/m, EA 500 N, EI 1e-4 N m^2, linear
+    drag 0.2 1/s) on 0.02 m segments along lower-hemisphere directions, h 1/3000 (50 substeps per 1/60 s frame),
+    8 iterations, roots driven by the root program; plane z = 0 and one static sphere SDF (r 0.1 m on a
+    grid_n^3 lattice) per rope, placed beside the rope's midpoint so the falling rope swings into it."""
     rng = np.random.default_rng(seed)
     d = _lower_hemisphere(rng, n_ropes)
     p0 = np.column_stack([rng.uniform(-0.5, 0.5, (n_ropes, 2)), rng.uniform(0.8, 1.2, n_ropes)])
@@ -285,14 +287,18 @@ def config4(n_ropes=256, V=65, frames=120, seed=4, grid_n=48):
     lo = -(r + margin)
     spacing = 2 * (r + margin) / (grid_n - 1)
     g = _sdf.from_function(_sdf.sdf_sphere(r), (lo, lo, lo), (lo + spacing * (grid_n - 1),) * 3, spacing)
-    # the sphere sits below each rope's midpoint, clear of it
-    mid = pos[:, V // 2].astype(np.float64)
-    c = mid - np.array([0.0, 0.0, r + 0.05])
+    side = np.cross(d, np.array([0.0, 0.0, 1.0]))
+    ns = np.linalg.norm(side, axis=1, keepdims=True)
+    side = np.where(ns > 1e-3, side / np.maximum(ns, 1e-30), np.array([1.0, 0.0, 0.0]))
+    c = pos[:, V // 2].astype(np.float64) + (r + 0.03) * side
     c[:, 2] = np.maximum(c[:, 2], r + 0.01)
+    for i in range(n_ropes):  # keep the start clear of the sphere
+        while np.linalg.norm(pos[i].astype(np.float64) - c[i], axis=1).min() < r + 0.02:
+            c[i] += 0.02 * side[i]
     mpos = c[:, None, :].astype(np.float32)
     mq = np.tile(np.array([0, 0, 0, 1], np.float32), (n_ropes, 1, 1))
     contact = dict(radius=R, mu_s=0.5, mu_k=0.4, v_slip=1e-3, kappa=2.0)
     return Scene("config4", pos.reshape(-1, 3), (np.arange(n_ropes + 1) * V).astype(np.int32), 0.1, 500.0, 1e-4,
-                 1.0 / 3000.0, 8, 1.0 / 60.0, 50, frames, planes=np.array([[0, 0, 1, 0]], np.float32),
+                 1.0 / 3000.0, 8, 1.0 / 60.0, 50, frames, drag=0.2, planes=np.array([[0, 0, 1, 0]], np.float32),
                  contact=contact, meshes=[Mesh(g.values, g.origin, g.spacing, 0.5, 0.4)], mesh_pos0=mpos,
                  mesh_quat0=mq, seed=seed)

@@ -170,6 +170,34 @@ int rs_kernel_time(rs_batch* b, double* total_ms, int* launches);
  * last call, and turns counting on or off for the next frames. */
 int rs_set_chunk_stats(rs_batch* b, int on, long long* out4);
 
+/* ---- Differentiation form. The reference has no adjoint, so these entry points have no counterpart in it.
+ * A recorded rollout keeps one checkpoint per frame (x, v, v_last, q, the staged root program and the mesh
+ * poses). rs_adjoint recomputes each frame from its checkpoint and runs the hand-written adjoint of the
+ * substep backwards. The forward active set is frozen: contact on/off, friction branch, gravity clamp,
+ * orientation signs and broad-phase lists. Covered: root program (or constant drive), init, solver iterations,
+ * end update with linear drag, plane and SDF contact. Not covered: rope-rope contact and quadratic drag;
+ * recording refuses both.
+ *
+ * rs_set_material sets per-vertex EA [N], EI [N m^2] (per segment, entry at a rope's last vertex ignored) and
+ * node masses [kg] (pinned entries ignored). Any pointer may be null to keep that field. Unlike the
+ * reference's per-vertex setters it is allowed after stepping. The arrays are device memory when on_device is
+ * nonzero, host memory otherwise. */
+int rs_set_material(rs_batch* b, const float* ea, const float* ei, const float* mass, int on_device);
+/* Start recording: each following step records a checkpoint, up to max_frames. */
+int rs_tape_begin(rs_batch* b, int max_frames);
+/* Stop recording and keep the state after the last recorded frame. rs_adjoint does this by itself. */
+int rs_tape_end(rs_batch* b);
+int rs_tape_frames(const rs_batch* b);
+/* Backward pass over the recorded frames.
+ * In (device): dl_dx [frames, num_vertices, 3], dL/d(positions after each frame).
+ * Out (device, each may be null): dL/dEA, dL/dEI, dL/dmass [num_vertices]; dL/dp0, dL/dv0, dL/dp1
+ * [frames, num_ropes, 3] of each frame's root program (constant drive: p0 = position, v0 = velocity).
+ * Out (host): dL/d(linear drag) in *g_drag. In *mismatches, the number of vertices whose recomputed
+ * frame-end state differs from the next checkpoint; it is 0 when the material and meshes are unchanged
+ * since recording. */
+int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float* g_mass, double* g_drag,
+               float* g_p0, float* g_v0, float* g_p1, int* mismatches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,158 @@
+"""Differentiation form: the hand-written HIP adjoint against two independent references.
+
+- torch autograd through the float64 transcription (oracle/torch_rope.py), which is itself pinned
+  to the C++ oracle and to finite differences in tests/test_torch_transcription.py;
+- central finite differences of the float64 C++ oracle, over directions in parameter space.
+
+The scene is config-4 material (rho 0.1, EA 500, EI 1e-4, drag 0.2) on ropes that fall onto a plane
+and onto a sphere SDF within a few frames. So the adjoint's contact, friction, root-program,
+stretch/bend, mass and drag paths are all exercised. Bar, from the differentiation-form target: every
+gradient family within 1e-3 relative (vector 2-norm) of the float64 references.
+"""
+import numpy as np
+import pytest
+
+from adjoint_cases import contact_scene, make_oracle, nominal_material, oracle_loss, program, rel, targets, torch_grads
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-3
+
+
+def hip_grads(rs, s, b, mat, drag, prog, tips, tgt):
+    import torch
+    t = lambda a: torch.tensor(np.asarray(a, np.float32), device="cuda", requires_grad=True)
+    EA, EI, M = (t(a) for a in mat)
+    P0, V0, P1 = (t(a) for a in prog)
+    D = torch.tensor(float(drag), device="cuda", requires_grad=True)
+    x = rs.diff.rollout(b, EA, EI, M, D, P0, V0, P1, s.dt, s.substeps)
+    loss = ((x[:, tips].double() - torch.tensor(tgt, device="cuda")) ** 2).sum()
+    loss.backward()
+    g = dict(ea=EA, ei=EI, m=M, drag=D, p0=P0, v0=V0, p1=P1)
+    return {k: v.grad.detach().cpu().numpy().astype(np.float64) for k, v in g.items()}, float(loss), x.detach()
+
+
+@pytest.fixture(scope="module")
+def case(rs, oracle):
+    s = contact_scene()
+    b = s.build(rs)
+    mat = nominal_material(s, b)
+    prog = program(s)
+    tips, tgt = targets(s)
+    g_hip, l_hip, x = hip_grads(rs, s, b, mat, s.drag, prog, tips, tgt)
+    return dict(s=s, b=b, mat=mat, prog=prog, tips=tips, tgt=tgt, g=g_hip, loss=l_hip, x=x)
+
+
+def test_scene_touches_sphere_and_plane(oracle, case):
+    """The gradient checks below only mean something if contacts were active."""
+    s = case["s"]
+    _, touched = oracle_loss(oracle, s, case["mat"], s.drag, case["prog"], case["tips"], case["tgt"], forces=True)
+    assert touched == 3, f"contact coverage bits {touched:b} (1: sphere, 2: plane)"
+
+
+def test_rollout_forward_matches_plain_step(rs, case):
+    """rollout() drives the same solver as step(): positions equal a plain rebuild run bit for bit."""
+    import torch
+    s, mat, prog = case["s"], case["mat"], case["prog"]
+    b = s.build(rs)
+    b.set_material(*(a.astype(np.float32) for a in mat))
+    b.drag = s.drag
+    for f in range(s.frames):
+        b.step(*(torch.tensor(a[f], dtype=torch.float32, device="cuda") for a in prog[:2]), s.dt, s.substeps,
+               root_pos_end=torch.tensor(prog[2][f], dtype=torch.float32, device="cuda"))
+        xf = torch.as_tensor(b.positions(), device="cuda").reshape(-1, 3)
+        assert torch.equal(xf, case["x"][f]), f"frame {f}"
+
+
+def _floor_check(rs, oracle, s, b, g_hip, mat, prog, tips, tgt, label):
+    """Every gradient family within 1e-3 of float64 autograd. Where float32 arithmetic itself moves the
+    gradient further, the bar is twice that gap (float32 autograd of the same transcription against
+    float64), as in the positions tests."""
+    import torch
+    g64, _ = torch_grads(oracle, s, mat, s.drag, prog, tips, tgt)
+    g32, _ = torch_grads(oracle, s, mat, s.drag, prog, tips, tgt, dtype=torch.float32)
+    rows = []
+    for k in g64:
+        e, gap = rel(g_hip[k], g64[k]), rel(g32[k], g64[k])
+        rows.append((k, e, gap))
+    print(f"\n  {label}: adjoint vs float64 autograd (float32 autograd vs float64):",
+          ", ".join(f"{k} {e:.1e} ({gap:.1e})" for k, e, gap in rows))
+    for k, e, gap in rows:
+        assert e <= max(TOL, 2 * gap), (k, e, gap)
+
+
+def test_adjoint_matches_float64_autograd(rs, oracle, case):
+    """Contact-rich scene. The float32 forward places vertices to ~1.5e-8 m, and over 2 cm segments
+    that makes the strain (~1e-3) uncertain to ~1e-3 relative. The EA and mass gradients scale with
+    strains and penalty gaps, so they carry that uncertainty; float32 autograd of the same method is off
+    by ~1e-2 there."""
+    s = case["s"]
+    _, l_ref = torch_grads(oracle, s, case["mat"], s.drag, case["prog"], case["tips"], case["tgt"])
+    assert abs(case["loss"] - l_ref) <= 1e-4 * abs(l_ref)
+    _floor_check(rs, oracle, s, case["b"], case["g"], case["mat"], case["prog"], case["tips"], case["tgt"], "contact")
+
+
+def test_adjoint_without_contact(rs, oracle):
+    s = contact_scene(with_contact=False)
+    b = s.build(rs)
+    mat = nominal_material(s, b)
+    prog = program(s)
+    tips, tgt = targets(s)
+    g, _, _ = hip_grads(rs, s, b, mat, s.drag, prog, tips, tgt)
+    _floor_check(rs, oracle, s, b, g, mat, prog, tips, tgt, "no contact")
+
+
+@pytest.fixture(scope="module")
+def g32(oracle, case):
+    import torch
+    s = case["s"]
+    return torch_grads(oracle, s, case["mat"], s.drag, case["prog"], case["tips"], case["tgt"], dtype=torch.float32)[0]
+
+
+@pytest.mark.parametrize("which,eps", [("ea", 1e-6), ("ei", 1e-5), ("drag", 1e-6), ("p1", 1e-9), ("v0", 1e-6)])
+def test_adjoint_matches_central_differences(oracle, case, g32, which, eps):
+    """Directional central differences of the float64 oracle along the adjoint's own (scaled) gradient, so
+    the projection does not cancel. Steps are relative to the parameter scale, with the same choices and
+    reasons as tests/test_torch_transcription.py. Bar: 1e-3, or twice the float32 autograd projection's
+    own gap to the central difference where float32 arithmetic alone moves it further."""
+    s = case["s"]
+    mat, prog, tips, tgt, g = case["mat"], case["prog"], case["tips"], case["tgt"], case["g"]
+    if which in ("ea", "ei"):
+        i = 0 if which == "ea" else 1
+        v = mat[i] * (g[which] * mat[i])
+        v = v / np.linalg.norm(g[which] * mat[i])
+
+        def loss_at(h):
+            m = list(mat)
+            m[i] = mat[i] + h * v
+            return oracle_loss(oracle, s, m, s.drag, prog, tips, tgt)[0]
+    elif which == "drag":
+        v = float(np.sign(g[which]))
+
+        def loss_at(h):
+            return oracle_loss(oracle, s, mat, s.drag + h * v, prog, tips, tgt)[0]
+    else:
+        i = 2 if which == "p1" else 1
+        v = g[which] / np.linalg.norm(g[which])
+
+        def loss_at(h):
+            p = list(prog)
+            p[i] = prog[i] + h * v
+            return oracle_loss(oracle, s, mat, s.drag, p, tips, tgt)[0]
+    fd = (loss_at(eps) - loss_at(-eps)) / (2 * eps)
+    ad = float(np.sum(g[which] * v))
+    gap = abs(float(np.sum(g32[which] * v)) - fd) / abs(fd)
+    print(f"\n  {which}: adjoint {ad:.6e}  central difference {fd:.6e}  (float32 autograd gap {gap:.1e})")
+    assert abs(ad - fd) <= max(TOL, 2 * gap) * abs(fd), (ad, fd, gap)
+
+
+def test_recording_refuses_what_the_adjoint_does_not_cover(rs):
+    s = contact_scene(n_ropes=2, frames=1)
+    b = s.build(rs)
+    b.drag_quad = 0.5
+    with pytest.raises(ValueError, match="quadratic drag"):
+        b.tape_begin(2)
+    b.drag_quad = 0.0
+    b.set_rope_contact(True, 0.001, 0.001, 0.01, 0.1, 1.0)
+    with pytest.raises(ValueError, match="rope-rope"):
+        b.tape_begin(2)
