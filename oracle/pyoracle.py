@@ -186,10 +186,13 @@ class OracleBatch:
 
     def set_planes(self, planes, radius=0.0, mu_s=0.5, mu_k=None, v_slip=1e-3, kappa=2.0):
         p = _d(planes).reshape(-1, 4)
+        if len(p) > 0:
+            self._mu = (mu_s, mu_s if mu_k is None else mu_k)
         _check(lib().orc_set_planes(self._h, len(p), _dp(p), radius, mu_s, mu_s if mu_k is None else mu_k, v_slip,
                                     kappa))
 
     def set_contact(self, radius, mu_s=0.5, mu_k=None, v_slip=1e-3, kappa=2.0):
+        self._mu = (mu_s, mu_s if mu_k is None else mu_k)
         _check(lib().orc_set_contact(self._h, radius, mu_s, mu_s if mu_k is None else mu_k, v_slip, kappa))
 
     def set_masses(self, m):
@@ -202,13 +205,17 @@ class OracleBatch:
     def set_bend_stiffnesses(self, ei):
         _check(lib().orc_set_vertex(self._h, 2, _dp(_d(ei).reshape(-1))))
 
-    def add_mesh(self, values, origin, spacing, pos=(0, 0, 0), quat=(0, 0, 0, 1), mu_s=0.5, mu_k=None):
+    def add_mesh(self, values, origin, spacing, pos=(0, 0, 0), quat=(0, 0, 0, 1), mu_s=None, mu_k=None):
+        """Friction defaults to the contact material's, as in the reference (csrc/bindings.cpp:474-480)."""
+        cs, ck = getattr(self, "_mu", (0.0, 0.0))
+        mu_k = (ck if mu_s is None else mu_s) if mu_k is None else mu_k
+        mu_s = cs if mu_s is None else mu_s
         v = np.ascontiguousarray(np.asarray(values, np.float32))
         nx, ny, nz = v.shape
         p, q = _d(pos).reshape(-1, 3), _d(quat).reshape(-1, 4)
         idx = C.c_int(0)
         _check(lib().orc_add_mesh(self._h, v.ctypes.data_as(C.POINTER(C.c_float)), nx, ny, nz, _dp(_d(origin)),
-                                  float(spacing), mu_s, mu_s if mu_k is None else mu_k, _dp(p), _dp(q), len(p),
+                                  float(spacing), mu_s, mu_k, _dp(p), _dp(q), len(p),
                                   C.byref(idx)))
         self.num_meshes += 1
         return idx.value
