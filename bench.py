@@ -82,6 +82,10 @@ def differentiation_bench(n_ropes, frames):
 
     forward_frames(0)  # warm-up (module load, first launches)
     t_plain = forward_frames(0)
+    b.set_graph(True)  # the same per-substep kernels replayed from a captured hipGraph
+    forward_frames(0)
+    t_plain_graph = forward_frames(0)
+    b.set_graph(False)
     b.set_dispatch(1, -1)
     b.autotune(s.dt, s.substeps, True)
     variant = b.dispatch
@@ -121,6 +125,7 @@ def differentiation_bench(n_ropes, frames):
                         "iterations, tip-tracking loss",
             "frames": frames, "vertex_substeps": units,
             "plain_kernels_ms_per_frame": 1e3 * t_plain / frames,
+            "plain_kernels_graph_ms_per_frame": 1e3 * t_plain_graph / frames,
             "persistent_kernel_ms_per_frame": 1e3 * t_persist / frames,
             "persistent_speedup_vs_plain": t_plain / t_persist,
             "persistent_dispatch": list(variant),
@@ -138,7 +143,7 @@ def main():
     ap.add_argument("--ropes", type=int, default=4096)
     ap.add_argument("--mode", type=int, default=1, help="0: per-substep kernels, 1: persistent frame kernel")
     ap.add_argument("--variant", type=int, default=-1, help="frame-kernel variant (-1: autotune)")
-    ap.add_argument("--cpu-sample-ropes", type=int, default=128)
+    ap.add_argument("--cpu-sample-ropes", type=int, default=768, help="config-3 ropes timed on the CPU oracle (about 10 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--diff-ropes", type=int, default=256, help="config-4 ropes of the differentiation leg")
     ap.add_argument("--diff-frames", type=int, default=120, help="config-4 frames (0 skips the leg)")

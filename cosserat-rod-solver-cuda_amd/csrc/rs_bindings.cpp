@@ -343,6 +343,12 @@ public:
         const float* m = host_arr(mass, M, "mass");
         check(rs_set_material(b(), a, c, m, 0));
     }
+    void set_graph(bool on) { check(rs_set_graph(b(), on ? 1 : 0)); }
+    long long graph_captures() const {
+        long long c = 0;
+        check(rs_graph_captures(b(), &c));
+        return c;
+    }
     void tape_begin(int max_frames) { check(rs_tape_begin(b(), max_frames)); }
     void tape_end() { check(rs_tape_end(b())); }
     int tape_frames() const { return rs_tape_frames(b()); }
@@ -854,6 +860,9 @@ PYBIND11_MODULE(_ropesim, m) {
              py::arg("mass") = py::none(),
              "Per-vertex EA, EI (per segment, last vertex ignored) and node masses (pinned ignored); GPU or host "
              "arrays of num_vertices float32; allowed between rollouts")
+        .def("set_graph", &RopeBatch::set_graph, py::arg("on") = true,
+             "Capture each frame's kernel sequence into a hipGraph and replay it while its inputs are unchanged")
+        .def_property_readonly("graph_captures", &RopeBatch::graph_captures)
         .def("tape_begin", &RopeBatch::tape_begin, py::arg("max_frames"),
              "Record a checkpoint at every following step (up to max_frames) for adjoint()")
         .def("tape_end", &RopeBatch::tape_end)

@@ -154,6 +154,12 @@ struct rs_batch {
         std::vector<Drive> dr;
         std::vector<int> S;
     } tape;
+    // optional hipGraph form of the frame's kernel sequence
+    bool graphOn = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<char> gkey;
+    long long graphCaptures = 0;
     float4* d_adj = nullptr;      // [4][nv] state adjoint
     char* d_hist = nullptr;       // history of one frame (a chunk of ropes at a time)
     size_t histBytes = 0;
@@ -164,6 +170,7 @@ struct rs_batch {
 
     ~rs_batch() {
         if (stream) hipStreamSynchronize(stream);
+        drop_graph();
         for (auto& e : events) {
             hipEventDestroy(e.first);
             hipEventDestroy(e.second);
@@ -586,31 +593,77 @@ struct rs_batch {
         out.hasQ0 = in.q0 != nullptr;
         out.hasQ1 = in.q1 != nullptr;
     }
-    void run_substeps(const Drive& dr, int S) {
-        const Params P = params();
+    // The frame's kernel sequence: per-substep launches (mode 0) or the persistent frame kernel (mode 1).
+    void enqueue_frame(const Params& P, const Drive& dr, int S, int var) {
         if (mode == 0) {
             if (dr.mode == DRIVE_CONST) hcheck(launch_root_const(P, D, dr, stream), "root drive");
             for (int s = 0; s < S; s++) hcheck(launch_plain_substep(P, D, dr, s, S, stream), "substep");
             hcheck(launch_gather(P, D, stream), "gather");
         } else {
-            int var = variant;
-            if (var < 0 || !variant_supported(var)) var = default_variant();
-            if (var < 0) throw Invalid("no persistent-kernel variant fits these ropes; use dispatch mode 0");
-            if (cellsFor != var) build_cells(var);
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (timing) {
-                hcheck(hipEventCreate(&e0), "event");
-                hcheck(hipEventCreate(&e1), "event");
-                hcheck(hipEventRecord(e0, stream), "event");
-            }
             hcheck(launch_cell(var, P, D, dr, d_cells, d_order, (int)cells.size(), S, countHits ? d_hits : nullptr,
                                stream),
                    "frame kernel");
-            if (timing) {
-                hcheck(hipEventRecord(e1, stream), "event");
-                events.emplace_back(e0, e1);
-            }
         }
+    }
+    void run_substeps(const Drive& dr, int S) {
+        const Params P = params();
+        int var = -1;
+        if (mode == 1) {
+            var = variant;
+            if (var < 0 || !variant_supported(var)) var = default_variant();
+            if (var < 0) throw Invalid("no persistent-kernel variant fits these ropes; use dispatch mode 0");
+            if (cellsFor != var) build_cells(var);
+        }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timing && mode == 1) {
+            hcheck(hipEventCreate(&e0), "event");
+            hcheck(hipEventCreate(&e1), "event");
+            hcheck(hipEventRecord(e0, stream), "event");
+        }
+        if (graphOn)
+            launch_graph(P, dr, S, var);
+        else
+            enqueue_frame(P, dr, S, var);
+        if (timing && mode == 1) {
+            hcheck(hipEventRecord(e1, stream), "event");
+            events.emplace_back(e0, e1);
+        }
+    }
+    // Graph form of the frame: the kernel sequence is captured once into a hipGraph and replayed while
+    // every argument it was captured with (parameters, buffers, staged drive, substeps, dispatch) is the same.
+    void launch_graph(const Params& P, const Drive& dr, int S, int var) {
+        std::vector<char> key(sizeof(Params) + sizeof(Dev) + sizeof(Drive) + 6 * sizeof(long long));
+        char* k = key.data();
+        std::memcpy(k, &P, sizeof(Params));
+        std::memcpy(k + sizeof(Params), &D, sizeof(Dev));
+        std::memcpy(k + sizeof(Params) + sizeof(Dev), &dr, sizeof(Drive));
+        const long long tail[6] = {mode, var, S, (long long)(uintptr_t)d_cells, (long long)cells.size(),
+                                   countHits ? 1 : 0};
+        std::memcpy(k + sizeof(Params) + sizeof(Dev) + sizeof(Drive), tail, sizeof(tail));
+        if (!gexec || key != gkey) {
+            drop_graph();
+            hcheck(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "begin capture");
+            try {
+                enqueue_frame(P, dr, S, var);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                hipStreamEndCapture(stream, &g);
+                if (g) hipGraphDestroy(g);
+                throw;
+            }
+            hcheck(hipStreamEndCapture(stream, &graph), "end capture");
+            hcheck(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "instantiate graph");
+            gkey.swap(key);
+            graphCaptures++;
+        }
+        hcheck(hipGraphLaunch(gexec, stream), "graph launch");
+    }
+    void drop_graph() {
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        gkey.clear();
     }
     void frame(const Drive& in, float dt, int S) {
         if (!(dt > 0.f)) throw Invalid("dt must be > 0");
@@ -1397,6 +1450,21 @@ int rs_set_chunk_stats(rs_batch* b, int on, long long* out4) {
             for (int i = 0; i < 4; i++) out4[i] = (long long)h[i];
         b->countHits = on != 0;
     });
+}
+
+int rs_set_graph(rs_batch* b, int on) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        hcheck(hipStreamSynchronize(b->stream), "sync");
+        b->graphOn = on != 0;
+        if (!b->graphOn) b->drop_graph();
+    });
+}
+
+int rs_graph_captures(const rs_batch* b, long long* captures) {
+    if (!b || !captures) return RS_INVALID;
+    *captures = b->graphCaptures;
+    return RS_OK;
 }
 
 // ------------------------------------------------------------------ differentiation form (no reference counterpart)

@@ -153,6 +153,13 @@ int rs_num_cell_variants(void);
 int rs_cell_variant_info(int variant, int* block, int* vpt, int* supported);
 /* Time one frame per supported variant on the current state (restored afterwards) and keep the fastest. */
 int rs_autotune(rs_batch* b, float dt, int substeps, int program, float* best_ms);
+/* Graph form of a frame (the role of the reference's rebuildCUDAGraph, YB/sim/step.cpp:5-57). When on,
+ * each frame's kernel sequence (the per-substep kernels of mode 0, the persistent kernel of mode 1) is
+ * captured once into a hipGraph. The graph is replayed while the frame's parameters, buffers, staged drive
+ * and substep count are unchanged, and captured again when they change. rs_graph_captures counts the
+ * captures. */
+int rs_set_graph(rs_batch* b, int on);
+int rs_graph_captures(const rs_batch* b, long long* captures);
 
 /* Function-level checks, no counterpart in the reference: run the device rope-rope detection on given
  * substep-start positions and paddings (host [nv*3], [nv]) -> counts [nv], lists [nv*128] (-1 padded),
