@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -154,6 +155,22 @@ struct rs_batch {
         std::vector<Drive> dr;
         std::vector<int> S;
     } tape;
+    // per-substep inputs of the persistent kernel (root program, moving-mesh poses), sized for the frame
+    size_t rootTrajCap = 0, posesSCap = 0, vlistCap = 0;
+    void ensure_frame_buffers(int S) {
+        const size_t nr = (size_t)S * N * 3;
+        if (nr > rootTrajCap) {
+            if (D.rootTraj) hipFree(const_cast<float4*>(D.rootTraj));
+            D.rootTraj = dalloc<float4>(nr, "root trajectory");
+            rootTrajCap = nr;
+        }
+        const size_t np = (size_t)S * N * grids.size();
+        if (np > posesSCap) {
+            if (D.posesS) hipFree(const_cast<Pose*>(D.posesS));
+            D.posesS = dalloc<Pose>(np, "substep poses");
+            posesSCap = np;
+        }
+    }
     // optional hipGraph form of the frame's kernel sequence
     bool graphOn = false;
     hipGraph_t graph = nullptr;
@@ -177,7 +194,7 @@ struct rs_batch {
         }
         void* ptrs[] = {D.x, D.v, D.vlast, D.q, D.q2, D.dx, D.dx2, D.prm, D.qrest, D.ropeOf, d_roff, D.pad, D.maxStep,
                         D.ncol, D.coll, D.vlist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.outPos,
-                        D.outVel, D.outQuat, D.err, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_hits,
+                        D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_hits,
                         d_grids, d_cur, d_tgt, d_tele, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
                         tape.poses, tape.motion, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
                         d_gdrag, d_mismatch};
@@ -511,33 +528,74 @@ struct rs_batch {
     }
 
     // ---------------------------------------------------------------- cells
+    // Cells of the persistent kernel: whole units (a contact group, or a rope without rope-rope contact) packed
+    // best-fit-decreasing into cells of the variant's capacity, so the fewest workgroups carry the batch. A cell's
+    // units are laid out in vertex order; cellMap gives the vertex of every slot.
     void build_cells(int var) {
         const int cap = kCellVariants[var].block * kCellVariants[var].vpt;
         const int G = ropeContact ? groupRopes : 1;
-        cells.clear();
-        Cell cur{0, 0, 0, 0};
+        struct Unit {
+            int v0, nv;
+        };
+        std::vector<Unit> units;
         for (int g0 = 0; g0 < N; g0 += G) {
             const int g1 = std::min(N, g0 + G);
-            const int unitNv = roff[g1] - roff[g0];
-            if (unitNv > cap) throw Invalid("a rope / contact group does not fit the cell");
-            if (cur.nv + unitNv > cap && cur.nropes > 0) {
-                cells.push_back(cur);
-                cur = Cell{roff[g0], 0, g0, 0};
-            }
-            if (cur.nropes == 0) cur = Cell{roff[g0], 0, g0, 0};
-            cur.nv += unitNv;
-            cur.nropes += g1 - g0;
+            const Unit u{roff[g0], roff[g1] - roff[g0]};
+            if (u.nv > cap) throw Invalid("a rope / contact group does not fit the cell");
+            units.push_back(u);
         }
-        if (cur.nropes > 0) cells.push_back(cur);
-        std::vector<int> order(cells.size());
-        for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+        std::vector<int> byNv(units.size());
+        for (size_t i = 0; i < byNv.size(); i++) byNv[i] = (int)i;
+        std::stable_sort(byNv.begin(), byNv.end(), [&](int a, int b) { return units[a].nv > units[b].nv; });
+        std::vector<std::vector<int>> members;
+        std::vector<int> used;
+        std::multimap<int, int> freeCap;  // free slots -> cell
+        for (int u : byNv) {
+            const int need = units[u].nv;
+            auto it = freeCap.lower_bound(need);
+            int c;
+            if (it == freeCap.end()) {
+                c = (int)members.size();
+                members.emplace_back();
+                used.push_back(0);
+            } else {
+                c = it->second;
+                freeCap.erase(it);
+            }
+            members[c].push_back(u);
+            used[c] += need;
+            if (cap - used[c] > 0) freeCap.emplace(cap - used[c], c);
+        }
+        const int nc = (int)members.size();
+        cells.assign(nc, Cell{0, 0, 0, 0});
+        std::vector<int> map((size_t)nc * cap, -1);
+        for (int c = 0; c < nc; c++) {
+            std::sort(members[c].begin(), members[c].end());
+            int slot = 0;
+            for (int u : members[c])
+                for (int k = 0; k < units[u].nv; k++) map[(size_t)c * cap + slot++] = units[u].v0 + k;
+            cells[c] = Cell{c * cap, slot, (int)members[c].size(), 0};
+        }
+        std::vector<int> order(nc);
+        for (int i = 0; i < nc; i++) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a].nv > cells[b].nv; });
         if (d_cells) hipFree(d_cells);
         if (d_order) hipFree(d_order);
-        d_cells = dalloc<Cell>(cells.size(), "cells");
-        d_order = dalloc<int>(order.size(), "cell order");
-        hcheck(hipMemcpy(d_cells, cells.data(), cells.size() * sizeof(Cell), hipMemcpyHostToDevice), "upload cells");
-        hcheck(hipMemcpy(d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice), "upload order");
+        if (D.cellMap) hipFree(const_cast<int*>(D.cellMap));
+        d_cells = dalloc<Cell>(nc, "cells");
+        d_order = dalloc<int>(nc, "cell order");
+        int* dm = dalloc<int>(map.size(), "cell map");
+        D.cellMap = dm;
+        hcheck(hipMemcpy(d_cells, cells.data(), nc * sizeof(Cell), hipMemcpyHostToDevice), "upload cells");
+        hcheck(hipMemcpy(d_order, order.data(), nc * sizeof(int), hipMemcpyHostToDevice), "upload order");
+        hcheck(hipMemcpy(dm, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice), "upload cell map");
+        // skin lists are indexed by cell slot
+        const size_t vl = map.size() * VERLET_CAP;
+        if (vl > vlistCap) {
+            if (D.vlist) hipFree(D.vlist);
+            D.vlist = dalloc<int>(vl, "skin lists");
+            vlistCap = vl;
+        }
         cellsFor = var;
     }
     int max_unit() const {
@@ -613,6 +671,7 @@ struct rs_batch {
             if (var < 0 || !variant_supported(var)) var = default_variant();
             if (var < 0) throw Invalid("no persistent-kernel variant fits these ropes; use dispatch mode 0");
             if (cellsFor != var) build_cells(var);
+            ensure_frame_buffers(S);
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing && mode == 1) {
@@ -1090,7 +1149,6 @@ int rs_set_rope_contact(rs_batch* b, int enabled, float radius, float barrier, f
             if (2.f * b->minSegLen < 2.f * radius + barrier)
                 throw Invalid("rope contact needs 2 * radius + barrier <= 2 * shortest segment");
             if (!b->D.coll) b->D.coll = dalloc<int>((size_t)b->nv * MAX_COLL, "collision lists");
-            if (!b->D.vlist) b->D.vlist = dalloc<int>((size_t)b->nv * VERLET_CAP, "skin lists");
             b->groupRopes = group_ropes;
             b->radius = radius;
             b->barrier = barrier;
@@ -1311,6 +1369,7 @@ int rs_autotune(rs_batch* b, float dt, int substeps, int program, float* best_ms
         dr.dt = dt;
         b->lastH = b->h;
         b->h = dt / float(substeps);
+        b->ensure_frame_buffers(substeps);
         const int oldMode = b->mode, oldVar = b->variant;
         b->mode = 1;
         hipEvent_t e0, e1;

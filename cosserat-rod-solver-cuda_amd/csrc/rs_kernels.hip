@@ -383,7 +383,39 @@ __global__ void k_check_broadphase(Params P, Dev D, const float4* xs, const floa
 //   q^k of the previous orientation sweep] | [orientation(k): read dx^(k+1), neighbours' q^k -> own q^(k+1)
 //   kept in a register] | ...
 // so nobody reads a slot while its owner overwrites it, and the results equal the per-substep kernels'.
-template <int BLOCK, int VPT>
+// Mesh poses of a rope during substep s of the persistent kernel: the precomputed per-substep poses while
+// meshes move, the committed poses otherwise.
+__device__ __forceinline__ const Pose* cell_poses(const Params& P, const Dev& D, int rope, int s) {
+    return P.sdfMoving ? D.posesS + ((size_t)s * P.nropes + rope) * P.numSdf : D.poses + (size_t)rope * MAX_SDF;
+}
+
+// Root program of every rope at every substep of the frame (csrc/batch.cu:122-159), ahead of the persistent
+// kernel so its per-vertex threads only load the result.
+__global__ void k_root_traj(Params P, Drive Dr, int S, float4* out) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= S * P.nropes) return;
+    const int s = idx / P.nropes, e = idx - s * P.nropes;
+    f3 x, v;
+    q4 q{0.f, 0.f, 0.f, 1.f};
+    bool setQ;
+    root_program(Dr, e, s, S, P.h, x, v, q, setQ);
+    out[3 * (size_t)idx] = make_float4(x.x, x.y, x.z, setQ ? 1.f : 0.f);
+    out[3 * (size_t)idx + 1] = st3(v);
+    out[3 * (size_t)idx + 2] = stq(q);
+}
+
+// Pose of every (rope, mesh) slot at every substep of a frame with moving meshes (YB/sim/iteration.cu:12-33).
+__global__ void k_pose_traj(Params P, Dev D, int S, Pose* out) {
+    const int M = P.numSdf;
+    const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (idx >= (long long)S * P.nropes * M) return;
+    const int m = (int)(idx % M);
+    const long long sr = idx / M;
+    const int r = (int)(sr % P.nropes), s = (int)(sr / P.nropes);
+    store_pose(out + idx, substep_pose(D.motion[(size_t)r * MAX_SDF + m], D.grids[m], s, S, P.h));
+}
+
+template <int BLOCK, int VPT, bool FORCES>
 __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const Cell* cells, const int* order, int S,
                                                 unsigned long long* stats) {
     constexpr int CAP = BLOCK * VPT;
@@ -424,7 +456,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
 #pragma unroll
     for (int k = 0; k < VPT; k++) {
         const int sl = t + k * BLOCK;
-        gi[k] = sl < C.nv ? C.v0 + sl : -1;
+        gi[k] = sl < C.nv ? D.cellMap[C.v0 + sl] : -1;
         nc[k] = P.numSdf > 0 ? -1 : 0;  // a frame starts by visiting every mesh (sdf.cu:226-227)
         cand[k] = 0;
         ncol[k] = 0;
@@ -438,11 +470,12 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             qrp[k] = (fl & HAS_PREV) ? ldq(D.qrest[i - 1]) : qr[k];
             rope[k] = D.ropeOf[i];
             isRoot[k] = D.roff[rope[k]] == i;
-            {
+            {  // a rope contact group occupies consecutive slots, so slot = vertex - (i - sl) inside it
                 const int G = P.groupRopes;
                 const int g = rope[k] / G;
-                grp[k] = (D.roff[min(g * G, P.nropes)] - C.v0) | ((D.roff[min((g + 1) * G, P.nropes)] - C.v0) << 16);
-                rng[k] = (D.roff[rope[k]] - C.v0) | ((D.roff[rope[k] + 1] - C.v0) << 16);
+                const int base = i - sl;
+                grp[k] = (D.roff[min(g * G, P.nropes)] - base) | ((D.roff[min((g + 1) * G, P.nropes)] - base) << 16);
+                rng[k] = (D.roff[rope[k]] - base) | ((D.roff[rope[k] + 1] - base) << 16);
             }
             f3 x = ld3(D.x[i]);
             q4 q = ldq(D.q[i]);
@@ -484,7 +517,12 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     f3 x, v;
                     q4 q;
                     bool setQ;
-                    root_program(Dr, rope[k], s, S, h, x, v, q, setQ);
+                    const float4* rt = D.rootTraj + 3 * ((size_t)s * P.nropes + rope[k]);
+                    const float4 r0 = rt[0];
+                    x = ld3(r0);
+                    v = ld3(rt[1]);
+                    setQ = r0.w != 0.f;
+                    if (setQ) q = ldq(rt[2]);
                     sX[sl] = x.x;
                     sX[CAP + sl] = x.y;
                     sX[2 * CAP + sl] = x.z;
@@ -494,13 +532,6 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         sQ[sl] = stq(q);
                     }
                 }
-            }
-        }
-        if (P.numSdf > 0 && P.sdfMoving) {
-            for (int idx = t; idx < C.nropes * P.numSdf; idx += BLOCK) {
-                const int r = C.rope0 + idx / P.numSdf, m = idx % P.numSdf;
-                const size_t slot = (size_t)r * MAX_SDF + m;
-                store_pose(D.poses + slot, substep_pose(D.motion[slot], D.grids[m], s, S, h));
             }
         }
         // init
@@ -692,7 +723,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     const float mg = 1e-4f + 2e-6f * fmaxf(fabsf(a0.x), fmaxf(fabsf(a0.y), fabsf(a0.z)));
                     const float rm = ri + mg;
                     const f3 lo = vmin(a0, x1) - mk3(rm, rm, rm), hi = vmax(a0, x1) + mk3(rm, rm, rm);
-                    const int* vl = D.vlist + (size_t)i * VERLET_CAP;
+                    const int* vl = D.vlist + (size_t)(C.v0 + sl) * VERLET_CAP;
                     const bool full = vcnt[k] < 0;
                     const int g0 = grp[k] & 0xFFFF, g1 = grp[k] >> 16;
                     const int rs0 = rng[k] & 0xFFFF, rs1 = rng[k] >> 16;
@@ -712,7 +743,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         float d2;
                         if (pair_hit(a1, c0 - a0, c1 - a0, ri + rj, d2)) {
                             if (cnt < MAX_COLL) {
-                                D.coll[(size_t)i * MAX_COLL + cnt] = C.v0 + j;
+                                D.coll[(size_t)i * MAX_COLL + cnt] = (i - sl) + j;
                                 minDist = fminf(minDist, 0.45f * sqrtf(d2));
                             }
                             cnt++;
@@ -763,13 +794,13 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         p1 = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
                         p1dx = mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]);
                     }
-                    vertex_terms(P, D.grids, D.poses + (size_t)rope[k] * MAX_SDF, nc[k], cand[k], prm[k].x, prm[k].y,
+                    vertex_terms(P, D.grids, cell_poses(P, D, rope[k], s), nc[k], cand[k], prm[k].x, prm[k].y,
                                  prm[k].z, fl, p0, d0, y[k], qo[k], p1, p1dx, fk[k], Hk[k], f2, H2);
                     if (P.ropeContact && (fl & HAS_NEXT)) {
                         const float dmp = P.damping / h;
                         const int* lst = D.coll + (size_t)gi[k] * MAX_COLL;
                         for (int c = 0; c < ncol[k]; c++) {
-                            const int j = lst[c] - C.v0;
+                            const int j = lst[c] - (gi[k] - sl);
                             collision_term(P, dmp, p0, p1, d0, p1dx, mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]),
                                            mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]),
                                            mk3(sD[j], sD[CAP + j], sD[2 * CAP + j]),
@@ -841,12 +872,12 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 const uint32_t fl = flags_of(prm[k]);
                 const f3 d = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
                 const f3 x = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-                const Pose* envPoses = D.poses + (size_t)rope[k] * MAX_SDF;
+                const Pose* envPoses = cell_poses(P, D, rope[k], s);
                 if (prm[k].x != 0.f) {
                     f3 tg = mk3(1.f, 0.f, 0.f);
                     if (P.dragQuad > 0.f) tg = qrot((fl & HAS_NEXT) ? qo[k] : ldq(sQ[sl - 1]), mk3(1.f, 0.f, 0.f));
                     vel[k] = end_velocity(P, d, tg);
-                    if (P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0)) {
+                    if (FORCES) {
                         const float kap = P.cKappa / h;
                         const f3 fv = surface_readout(P, D.grids, envPoses, D.wrench + (size_t)rope[k] * MAX_SDF * 6,
                                                       nc[k], cand[k], x, d, kap * kap / prm[k].x);
@@ -976,19 +1007,39 @@ size_t cell_lds_bytes(int block, int vpt) {
     return cap * (4 + 3 + 3 + 9 + 1 + 1 + 3 + 1) * sizeof(float) + (cap / 16) * (6 + 2 + 2) * sizeof(float);
 }
 
-template <int B, int V>
-static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
+template <int B, int V, bool F>
+static hipError_t launch_cell_f(const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
                                 int ncells, int S, unsigned long long* stats, hipStream_t st) {
     const size_t lds = cell_lds_bytes(B, V);
     static bool attrSet = false;
     if (!attrSet) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cell<B, V>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cell<B, V, F>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         attrSet = true;
     }
-    hipLaunchKernelGGL((k_cell<B, V>), dim3(ncells), dim3(B), lds, st, P, D, Dr, cells, order, S, stats);
+    hipLaunchKernelGGL((k_cell<B, V, F>), dim3(ncells), dim3(B), lds, st, P, D, Dr, cells, order, S, stats);
     return hipGetLastError();
+}
+template <int B, int V>
+static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
+                                int ncells, int S, unsigned long long* stats, hipStream_t st) {
+    if (Dr.mode == DRIVE_PROGRAM) {
+        hipLaunchKernelGGL(k_root_traj, dim3(nblk(S * P.nropes, 256)), dim3(256), 0, st, P, Dr, S,
+                           const_cast<float4*>(D.rootTraj));
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (P.numSdf > 0 && P.sdfMoving) {
+        const long long n = (long long)S * P.nropes * P.numSdf;
+        hipLaunchKernelGGL(k_pose_traj, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, P, D, S,
+                           const_cast<Pose*>(D.posesS));
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0))
+        return launch_cell_f<B, V, true>(P, D, Dr, cells, order, ncells, S, stats, st);
+    return launch_cell_f<B, V, false>(P, D, Dr, cells, order, ncells, S, stats, st);
 }
 
 const CellVariant kCellVariants[] = {{64, 1}, {128, 1}, {256, 1}, {512, 1}, {1024, 1}, {256, 2}, {256, 4}, {512, 2}};

@@ -92,6 +92,10 @@ struct Dev {
     float* outVel;   // [nv * 3]
     float* outQuat;  // [nv * 4]
     int* err;        // [2]
+    // per-frame inputs of the persistent kernel, precomputed per substep
+    const float4* rootTraj;  // [S][nropes][3]: root program x (w = 1 when it sets the frame), v, q
+    const Pose* posesS;      // [S][nropes][numSdf] mesh poses while meshes move
+    const int* cellMap;      // [cells * cell capacity] vertex of each cell slot (-1 past the cell's end)
 };
 
 // What drives the roots during one frame.
@@ -107,10 +111,11 @@ struct Drive {
     const float* q1;  // [nropes * 4]
 };
 
-// A cell: a contiguous run of whole ropes (whole contact groups) that one workgroup advances
-// through every substep of a frame.
+// A cell: whole ropes (whole contact groups) that one workgroup advances through every substep of a frame.
+// Its slots [0, nv) map to vertices through Dev::cellMap[v0 + slot]; each rope and each contact group
+// occupies consecutive slots.
 struct Cell {
-    int v0, nv, rope0, nropes;
+    int v0, nv, units, pad_;
 };
 
 }  // namespace rs
