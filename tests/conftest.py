@@ -38,6 +38,26 @@ def rs():
     return ropesim
 
 
+@pytest.fixture(params=["oracle", pytest.param("hip", marks=pytest.mark.gpu)])
+def solver(request):
+    """The analytic checks run on the CPU oracle and, on a GPU, on the HIP path (tests/hip_adapter.py gives it
+    the oracle's test surface)."""
+    if request.param == "oracle":
+        from oracle import pyoracle
+        pyoracle.lib()
+        return pyoracle
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("gpu test without a GPU")
+    import hip_adapter
+    return hip_adapter
+
+
+def fp32(solver, prec=64):
+    """True when the run computes in float32 (the device, or the oracle at prec 32)."""
+    return getattr(solver, "FP32", False) or prec == 32
+
+
 def to_dev(a):
     import torch
     return torch.as_tensor(np.ascontiguousarray(a, np.float32), device="cuda")

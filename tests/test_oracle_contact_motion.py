@@ -1,4 +1,4 @@
-"""More of the reference's analytic checks, restated on the CPU oracle (CPU only).
+"""More of the reference's analytic checks, on the CPU oracle and (on a GPU) on the HIP path.
 
 These pin the parts of the oracle the first physics file leaves open:
 - the bending stiffness mapping k_bend -> EI (the rest Darboux scale and the orientation sweep):
@@ -8,19 +8,20 @@ These pin the parts of the oracle the first physics file leaves open:
 - moving meshes: stage / per-substep pose / commit, friction relative to the moving surface,
   rotation: reference tests/test_mesh_motion.py:92-146;
 - the contact force readout and mesh wrench: reference tests/test_mesh_motion.py:244-275.
-Tolerances are the reference tests' own.
+Tolerances are the reference tests' own (float32 bounds for the exact linear-drag identity on the device).
 """
 import numpy as np
 import pytest
 
+from conftest import fp32
 from test_oracle_physics import advance, make, rope_line
 
 
 # ------------------------------------------------------------------------- bending
 
-def cantilever_tip_drop(oracle, EI):
+def cantilever_tip_drop(solver, EI):
     n, seg, rho, EA = 26, 0.02, 0.1, 2000.0
-    o = make(oracle, rope_line(n, seg, (1, 0, 0)), density=rho, EA=EA, EI=EI, fix_root_orientation=True)
+    o = make(solver, rope_line(n, seg, (1, 0, 0)), density=rho, EA=EA, EI=EI, fix_root_orientation=True)
     o.set_gravity([0.0, -9.8, 0.0])
     o.drag = 2.0
     advance(o, 1 / 60, 1 / 3000, 240)  # settle 4 s (tests/conftest.py settle of the reference)
@@ -28,27 +29,27 @@ def cantilever_tip_drop(oracle, EI):
 
 
 @pytest.mark.parametrize("EI", [0.4, 0.8])
-def test_cantilever_droop_vs_beam_theory(oracle, EI):
+def test_cantilever_droop_vs_beam_theory(solver, EI):
     """Clamped, uniformly loaded cantilever: tip drop w L^4 / (8 EI); the reference measures a ratio of
     0.92-0.93 (discretisation) and bands it to (0.80, 1.05) so a stray factor of 2 fails."""
-    delta, L, rho = cantilever_tip_drop(oracle, EI)
+    delta, L, rho = cantilever_tip_drop(solver, EI)
     ratio = delta / (rho * 9.8 * L ** 4 / (8.0 * EI))
     assert 0.80 < ratio < 1.05, ratio
 
 
-def test_cantilever_scales_inversely_with_EI(oracle):
-    d4 = cantilever_tip_drop(oracle, 0.4)[0]
-    d8 = cantilever_tip_drop(oracle, 0.8)[0]
+def test_cantilever_scales_inversely_with_EI(solver):
+    d4 = cantilever_tip_drop(solver, 0.4)[0]
+    d8 = cantilever_tip_drop(solver, 0.8)[0]
     assert d4 / d8 == pytest.approx(2.0, rel=0.10)
 
 
 # ------------------------------------------------------------------------- linear drag
 
-def test_linear_drag_discrete_decay(oracle):
+def test_linear_drag_discrete_decay(solver):
     """A free straight rope translating rigidly in zero gravity feels no internal force; each substep
     scales its velocity by (1 - drag h) exactly (YB/sim/iteration.cu:112)."""
     n, seg = 11, 0.03
-    o = make(oracle, rope_line(n, seg, (1, 0, 0)), pinned=np.zeros(n, bool))
+    o = make(solver, rope_line(n, seg, (1, 0, 0)), pinned=np.zeros(n, bool))
     o.set_gravity([0.0, 0.0, 0.0])
     o.drag = 0.7
     v0 = np.tile([0.3, -0.2, 0.1], (n, 1))
@@ -59,9 +60,11 @@ def test_linear_drag_discrete_decay(oracle):
         o.step_free(S * h, S)
     k = 1.0 - 0.7 * h
     steps = S * F
-    assert np.allclose(o.velocities(), v0 * k ** steps, rtol=1e-10, atol=0)
+    # float32 on the device: a rope translating along its own axis carries ~3e-5 of stretch rounding noise
+    tol = 2e-4 if fp32(solver) else 1e-9
+    assert np.allclose(o.velocities(), v0 * k ** steps, rtol=tol, atol=0)
     moved = v0 * h * sum(k ** j for j in range(steps))  # x += h v before each decay
-    assert np.allclose(o.positions() - x0, moved, rtol=1e-9, atol=1e-14)
+    assert np.allclose(o.positions() - x0, moved, rtol=tol, atol=1e-6 if fp32(solver) else 1e-14)
 
 
 # ------------------------------------------------------------------------- rope on rope
@@ -75,7 +78,7 @@ RR = dict(radius=0.9 * R, barrier=0.2 * R, k_collision=0.025, k_friction=1e7)
 
 
 @pytest.mark.parametrize("mu,expect_slide", [(0.5, False), (0.2, True)])
-def test_rope_on_rope_coulomb(oracle, mu, expect_slide):
+def test_rope_on_rope_coulomb(solver, mu, expect_slide):
     """Rope B lies on rope A on a 20 degree slope and A grips the floor. B sticks for mu > tan 20 and
     otherwise slides with a = g (sin - mu cos) within 2 % (the rope-rope friction law)."""
     th = np.radians(20.0)
@@ -84,7 +87,7 @@ def test_rope_on_rope_coulomb(oracle, mu, expect_slide):
     pos = np.zeros((2, NV, 3), np.float32)
     pos[:, :, 0] = np.arange(NV) * SEG
     pos[0, :, 2], pos[1, :, 2] = zA, zB
-    o = oracle.OracleBatch(pos, RHO, EA, EI, H, ITR, pinned=np.zeros((2, NV), bool), prec=64)
+    o = solver.OracleBatch(pos, RHO, EA, EI, H, ITR, pinned=np.zeros((2, NV), bool), prec=64)
     o.damping = 0.0
     o.drag = 0.0
     o.set_planes(np.array([[0.0, 0.0, 1.0, 0.0]]), radius=R, mu_s=1.0)
@@ -131,9 +134,9 @@ def rope_rows(n, z=0.02, y=0.0):
     return pos.astype(np.float32)
 
 
-def batch(oracle, pos, mu_s=0.5, mu_k=None, pinned=None):
+def batch(solver, pos, mu_s=0.5, mu_k=None, pinned=None):
     pin = np.zeros(pos.shape[:2], bool) if pinned is None else pinned
-    o = oracle.OracleBatch(pos, RHO, EA, EI, H, ITR, pinned=pin, prec=64)
+    o = solver.OracleBatch(pos, RHO, EA, EI, H, ITR, pinned=pin, prec=64)
     o.damping = 0.0
     o.drag = 0.0
     o.set_gravity([0.0, 0.0, -G])
@@ -141,13 +144,13 @@ def batch(oracle, pos, mu_s=0.5, mu_k=None, pinned=None):
     return o
 
 
-def test_conveyor_carries_rope_with_friction(oracle, slab):
+def test_conveyor_carries_rope_with_friction(solver, slab):
     """A slab translating at 0.2 m/s under a resting rope: with friction the rope rides along after a slip
     lag of v^2 / (2 mu g); without friction it stays."""
     v = 0.2
     out = {}
     for mu in (0.5, 0.0):
-        o = batch(oracle, rope_rows(1), mu_s=mu)
+        o = batch(solver, rope_rows(1), mu_s=mu)
         g = slab()
         o.add_mesh(*g.as_args(), pos=np.array([0, 0, -0.02]))
         for _ in range(30):
@@ -167,11 +170,11 @@ def test_conveyor_carries_rope_with_friction(oracle, slab):
     assert np.abs(d).max() < 1e-3
 
 
-def test_turntable_rotates_rope(oracle, slab):
+def test_turntable_rotates_rope(solver, slab):
     """A slab spun up about world z to 0.5 rad/s under a rope off the axis: the rope sticks and follows
     the rigid rotation (the per-substep pose quaternion and omega x r in the surface velocity)."""
     om = 0.5
-    o = batch(oracle, rope_rows(1, y=0.3))
+    o = batch(solver, rope_rows(1, y=0.3))
     g = slab(lo=(-0.3125, -0.3125, -0.1875), hi=(1.3125, 0.6, 0.1875))
     o.add_mesh(*g.as_args(), pos=np.array([0, 0, -0.02]))
     for _ in range(30):
@@ -192,10 +195,10 @@ def test_turntable_rotates_rope(oracle, slab):
     assert np.allclose(vel[:, 1], om * r[:, 0], atol=0.02 * om)
 
 
-def test_resting_rope_wrench_equals_weight(oracle, slab):
+def test_resting_rope_wrench_equals_weight(solver, slab):
     """Rope resting on a slab: the wrench on the mesh is its weight at its centre of mass, and the
     per-vertex contact forces sum to +M g."""
-    o = batch(oracle, rope_rows(1))
+    o = batch(solver, rope_rows(1))
     g = slab()
     o.add_mesh(*g.as_args(), pos=np.array([0, 0, -0.02]))
     o.enable_contact_forces(True)
