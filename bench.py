@@ -135,6 +135,78 @@ def differentiation_bench(n_ropes, frames):
             "adjoint_forward_ratio": bwd / fwd, "timing": "median of 3 rollouts after 2 warm-ups"}
 
 
+def scale_bench(frames, adj_frames):
+    """Config-5 workload: forward throughput of the graph-captured persistent frame kernel with everything on,
+    then a differentiable rollout over adj_frames frames of the same geometry without rope-rope contact (the
+    adjoint does not cover it) with a tip-tracking loss; device memory in use at the peak."""
+    import numpy as np
+    import torch
+
+    import ropesim
+    from ropesim import scenes
+    t0 = time.perf_counter()
+    s = scenes.config5(frames=max(frames, adj_frames) + 1)
+    free0 = torch.cuda.mem_get_info()[0]
+    low = [free0]
+    b = s.build(ropesim)
+    fis = s.frame_inputs(max(frames, adj_frames) + 1)
+    dev = [{k: torch.as_tensor(v, device="cuda") for k, v in fi.items() if v is not None} for fi in fis]
+    b.set_dispatch(1, -1)
+    b.autotune(s.dt, s.substeps, True)
+    b.set_graph(True)
+    setup = time.perf_counter() - t0
+    s.step(b, dev[0])  # warm-up: captures the graph
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for k in range(1, frames + 1):
+        s.step(b, dev[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    low.append(torch.cuda.mem_get_info()[0])
+    assert bool(torch.isfinite(torch.as_tensor(b.positions(), device="cuda")).all())
+    units = s.num_vertices * s.substeps
+    out = {"workload": f"config5: {s.num_ropes} ropes x 96 vertices (config-1 material, +-20 % per-vertex EA/EI/mass), "
+                       f"groups of 4 with rope-rope contact, {len(s.meshes)} static SDF meshes per environment "
+                       "(96^3 sphere/box/capsule/torus lattices), plane, root program, 100 substeps x 8 iterations",
+           "vertices": s.num_vertices, "dispatch": list(b.dispatch), "graph_captures": b.graph_captures,
+           "forward_ms_per_frame": 1e3 * wall / frames, "forward_vertex_substeps_per_s": units * frames / wall,
+           "setup_s": round(setup, 1)}
+    del b
+    # adjoint leg: same geometry and material, rope-rope contact off
+    s.rope_contact = None
+    b = s.build(ropesim)
+    b.set_dispatch(1, -1)
+    b.autotune(s.dt, s.substeps, True)
+    F = adj_frames
+    P0, V0, P1 = (torch.stack([dev[k][n] for k in range(F)]) for n in ("p0", "v0", "p1"))
+    nv = s.num_vertices
+    ea = torch.tensor(s.ea, device="cuda", requires_grad=True)
+    ei = torch.tensor(s.ei, device="cuda", requires_grad=True)
+    mass = torch.tensor(s.masses, device="cuda", requires_grad=True)
+    drag = torch.tensor(0.0, device="cuda", requires_grad=True)
+    p1 = P1.clone().requires_grad_(True)
+    tips = torch.as_tensor(s.offsets[1:] - 1, device="cuda", dtype=torch.long)
+    target = torch.as_tensor(s.positions[s.offsets[1:] - 1], device="cuda")[None] + 0.05
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    x = ropesim.diff.rollout(b, ea, ei, mass, drag, P0, V0, p1, s.dt, s.substeps)
+    loss = ((x[:, tips] - target) ** 2).sum()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    low.append(torch.cuda.mem_get_info()[0])
+    loss.backward()
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    low.append(torch.cuda.mem_get_info()[0])
+    assert all(bool(torch.isfinite(g).all()) for g in (ea.grad, ei.grad, mass.grad, p1.grad))
+    out.update({"adjoint_workload": f"{F} frames, rope-rope contact off, tip-tracking loss",
+                "rollout_forward_ms_per_frame": 1e3 * (t3 - t2) / F, "adjoint_ms_per_frame": 1e3 * (t4 - t3) / F,
+                "adjoint_forward_ratio": (t4 - t3) / (t3 - t2),
+                "device_memory_in_use_peak_gb": (free0 - min(low)) / 1e9,
+                "timing": "first rollout (includes one-time allocations)"})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +219,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--diff-ropes", type=int, default=256, help="config-4 ropes of the differentiation leg")
     ap.add_argument("--diff-frames", type=int, default=120, help="config-4 frames (0 skips the leg)")
+    ap.add_argument("--scale-frames", type=int, default=3, help="config-5 forward frames timed (0 skips the leg)")
+    ap.add_argument("--scale-adjoint-frames", type=int, default=30, help="config-5 frames of the adjoint leg")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -230,6 +304,11 @@ def main():
         d = differentiation_bench(args.diff_ropes, args.diff_frames)
         line["differentiation"] = d
         line["adjoint_forward_ratio"] = d["adjoint_forward_ratio"]
+    if args.scale_frames > 0:
+        try:
+            line["scale"] = scale_bench(args.scale_frames, args.scale_adjoint_frames)
+        except Exception as e:  # the headline line must still print
+            line["scale"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_ropes, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())))
     print(json.dumps(line), flush=True)

@@ -24,7 +24,8 @@ using namespace rs;
 namespace {
 
 thread_local std::string g_msg;
-// Device memory for one frame's adjoint history; larger batches are differentiated in chunks of ropes.
+// Device memory for one frame's adjoint history: at least 2 GiB, up to 40 % of the free memory; larger batches
+// are differentiated in chunks of ropes.
 constexpr size_t kAdjointHistoryBudget = size_t(2) << 30;
 
 struct Invalid : std::runtime_error {
@@ -1616,7 +1617,10 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
         int maxI = 1;
         for (const Params& P : T.P) maxI = std::max(maxI, P.numItr);
         const size_t perV = adjoint_history_bytes(maxI) * (size_t)T.maxS;
-        const size_t budget = std::max(kAdjointHistoryBudget, perV * (size_t)maxV);
+        size_t freeB = 0, totalB = 0;
+        if (hipMemGetInfo(&freeB, &totalB) != hipSuccess) freeB = 0;
+        const size_t budget =
+            std::max(std::max(kAdjointHistoryBudget, freeB / 5 * 2 + b->histBytes), perV * (size_t)maxV);
         struct Chunk {
             int rope0, nropes, v0, nv;
         };

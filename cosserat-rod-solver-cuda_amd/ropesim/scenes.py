@@ -12,6 +12,7 @@ Configurations (sizes scale down through the arguments for quick checks):
   config3  N ropes with V ~ U{16..256}, per-segment EA / EI, contact groups of 4 with rope-rope
            contact, 8 kinematic SDF meshes per environment on SE(3) random walks, ground plane
   config4  N ropes x 65 (config-1 material), plane + one static sphere SDF per rope
+  config5  16384 ropes x 96 (config-1 material, per-vertex jitter), 64 meshes on 96^3 grids, groups of 4
 """
 from __future__ import annotations
 
@@ -66,6 +67,7 @@ class Scene:
     fix_root_orientation: bool = False
     ea: np.ndarray | None = None   # float32 [nv] per segment at its first vertex
     ei: np.ndarray | None = None
+    masses: np.ndarray | None = None  # float32 [nv] node masses (pinned entries ignored)
     planes: np.ndarray | None = None
     contact: dict | None = None    # radius, mu_s, mu_k, v_slip, kappa
     meshes: list = field(default_factory=list)
@@ -94,6 +96,8 @@ class Scene:
             b.set_stretch_stiffnesses(self.ea)
         if self.ei is not None:
             b.set_bend_stiffnesses(self.ei)
+        if self.masses is not None:
+            b.set_masses(self.masses)
         if self.contact is not None:
             b.set_contact(**self.contact)
         if self.planes is not None:
@@ -302,3 +306,68 @@ def config4(n_ropes=256, V=65, frames=120, seed=4, grid_n=48):
                  1.0 / 3000.0, 8, 1.0 / 60.0, 50, frames, drag=0.2, planes=np.array([[0, 0, 1, 0]], np.float32),
                  contact=contact, meshes=[Mesh(g.values, g.origin, g.spacing, 0.5, 0.4)], mesh_pos0=mpos,
                  mesh_quat0=mq, seed=seed)
+
+
+def config5(n_ropes=16384, V=96, frames=3, seed=5, n_meshes=64, grid_n=96, group=4, n_shapes=8, rope_contact=True):
+    """Scale: n_ropes x V on 0.01 m segments with config-1 material (rho 0.1, EA 500, EI 1e-4) jittered by
+    U(-20 %, +20 %) per vertex, h 1/6000 (100 substeps), 8 iterations; contact groups of `group` ropes with
+    rope-rope contact; n_meshes SDF meshes per environment on grid_n^3 float32 grids (spheres, boxes, capsules
+    and tori from closed forms, n_shapes distinct lattices reused across the slots), placed once per
+    environment; plane z = 0; roots driven by the root program. Environments share one region of space."""
+    rng = np.random.default_rng(seed)
+    seg, R = 0.01, 0.004
+    roots = np.column_stack([rng.uniform(-0.5, 0.5, (n_ropes, 2)), rng.uniform(1.0, 1.3, n_ropes)])
+    d = _lower_hemisphere(rng, n_ropes)
+    d[:, 2] = np.minimum(d[:, 2], -0.5)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos = (roots[:, None, :] + seg * np.arange(V)[None, :, None] * d[:, None, :]).astype(np.float32)
+    nv = n_ropes * V
+    jitter = lambda: rng.uniform(0.8, 1.2, nv).astype(np.float32)
+    ea, ei = 500.0 * jitter(), 1e-4 * jitter()
+    lumped = np.full((n_ropes, V), 0.1 * seg, np.float32)  # rho l; the tip carries half a segment
+    lumped[:, -1] *= 0.5
+    masses = lumped.reshape(-1) * jitter()
+    shapes, radii = [], []
+    for k in range(n_shapes):
+        kind = k % 4
+        if kind == 0:
+            r = rng.uniform(0.05, 0.12)
+            phi, ext = _sdf.sdf_sphere(r), np.array([r, r, r])
+        elif kind == 1:
+            half = rng.uniform(0.03, 0.1, 3)
+            phi, ext = _sdf.sdf_box(half), half
+        elif kind == 2:
+            r, hl = rng.uniform(0.02, 0.05), rng.uniform(0.04, 0.1)
+            phi, ext = _sdf.sdf_capsule(r, hl), np.array([r, r, r + hl])
+        else:
+            a, b = rng.uniform(0.05, 0.09), rng.uniform(0.015, 0.03)
+            phi, ext = _sdf.sdf_torus(a, b), np.array([a + b, a + b, b])
+        margin = R + 0.02
+        lo, hi = -(ext + margin), ext + margin
+        spacing = float((hi - lo).max() / (grid_n - 1))
+        g = _sdf.from_function(phi, lo, lo + spacing * (grid_n - 1), spacing)
+        shapes.append(Mesh(g.values, g.origin, g.spacing, 0.5, 0.4))
+        radii.append(float(np.linalg.norm(ext)))
+    meshes = [shapes[m % n_shapes] for m in range(n_meshes)]
+    E = (n_ropes + group - 1) // group
+    # one layout of the mesh slots per environment, kept clear of the environment's ropes at the start
+    mpos = np.zeros((E, n_meshes, 3))
+    mq = np.zeros((E, n_meshes, 4))
+    for e in range(E):
+        verts = pos[e * group:min((e + 1) * group, n_ropes)].reshape(-1, 3).astype(np.float64)
+        for m in range(n_meshes):
+            for _ in range(100):
+                c = np.array([rng.uniform(-1.5, 1.5), rng.uniform(-1.5, 1.5), rng.uniform(0.15, 1.6)])
+                if np.linalg.norm(verts - c, axis=1).min() > radii[m % n_shapes] + 0.03:
+                    break
+            mpos[e, m] = c
+            mq[e, m] = _quat_exp(rng.normal(0, 1.0, 3))
+    env = np.arange(n_ropes) // group
+    contact = dict(radius=R, mu_s=0.5, mu_k=0.4, v_slip=1e-3, kappa=2.0)
+    rc = dict(radius=0.9 * R, barrier=0.2 * R, k_collision=0.025, mu=0.4, k_friction=1e7, detection_scaler=1.2,
+              group_ropes=group) if rope_contact else None
+    s = Scene("config5", pos.reshape(-1, 3), (np.arange(n_ropes + 1) * V).astype(np.int32), 0.1, 500.0, 1e-4,
+              1.0 / 6000.0, 8, 1.0 / 60.0, 100, frames, ea=ea, ei=ei, planes=np.array([[0, 0, 1, 0]], np.float32),
+              contact=contact, meshes=meshes, mesh_pos0=mpos[env].astype(np.float32),
+              mesh_quat0=mq[env].astype(np.float32), rope_contact=rc, masses=masses, seed=seed)
+    return s
