@@ -215,6 +215,9 @@ def main():
     ap.add_argument("--ropes", type=int, default=4096)
     ap.add_argument("--mode", type=int, default=1, help="0: per-substep kernels, 1: persistent frame kernel")
     ap.add_argument("--variant", type=int, default=-1, help="frame-kernel variant (-1: autotune)")
+    ap.add_argument("--ieee", action="store_true",
+                    help="correctly rounded, unfused arithmetic (the float32 oracle's) instead of the default fast "
+                         "arithmetic (fused multiply-add, hardware reciprocal / square root / log)")
     ap.add_argument("--cpu-sample-ropes", type=int, default=768, help="config-3 ropes timed on the CPU oracle (about 10 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--diff-ropes", type=int, default=256, help="config-4 ropes of the differentiation leg")
@@ -238,6 +241,7 @@ def main():
     K, W = args.steps, args.warmup
     s = scenes.config3(n_ropes=args.ropes, frames=K + W)
     b = s.build(ropesim)
+    b.fast_math = not args.ieee
     fis = s.frame_inputs(K + W)
     dev = [{k: torch.as_tensor(v, device="cuda") for k, v in fi.items() if v is not None} for fi in fis]
     torch.cuda.synchronize()
@@ -288,7 +292,8 @@ def main():
                                f"groups of 4 with rope-rope contact, 8 SDF meshes/env (64^3), plane, root program, "
                                f"{s.substeps} substeps x {s.num_itr} iterations per 1/60 s frame",
                    "ropes": args.ropes, "vertices": s.num_vertices, "substeps": s.substeps, "iterations": s.num_itr,
-                   "dispatch": list(b.dispatch), "autotune_ms": tuned_ms, "setup_s": round(setup_s, 1)},
+                   "dispatch": list(b.dispatch), "arithmetic": "ieee" if args.ieee else "fast",
+                   "autotune_ms": tuned_ms, "setup_s": round(setup_s, 1)},
     }
     if launches > 0:
         per_launch_s = kern_ms / launches * 1e-3

@@ -52,14 +52,17 @@ __device__ __forceinline__ q4 rot_ex_adj(q4 q, f3 c) {
 
 // Adjoint of surface_contact (YB/sim/contact.cuh:27-46) at fixed branch: given f̄ and H̄ = -u δ^T, the
 // adjoints of the relative displacement, the normal, the gap and the stiffness.
+template <bool F>
 __device__ __forceinline__ void surface_contact_adj(const Params& P, f3 n, float gap, f3 d, float k, float muS,
                                                     float muK, f3 fb, f3 u, f3 dl, f3& db, f3& nb, float& gapb,
                                                     float& kb) {
+    using A = Ar<F>;
+    using M = V<F>;
     const float eps = P.cVSlip, h = P.h;
     const float N = -k * gap;
-    const f3 ut = d - dot(n, d) * n;
-    const float ul = len(ut);
-    const float vt = ul / h;
+    const f3 ut = M::axpy(-M::dot(n, d), n, d);
+    const float ul = M::len(ut);
+    const float vt = A::div(ul, h);
     const float un = dot(u, n), dn = dot(dl, n);
     const float trG = -dot(u, dl), nGn = -un * dn;
     float Nb = dot(fb, n);
@@ -68,18 +71,19 @@ __device__ __forceinline__ void surface_contact_adj(const Params& P, f3 n, float
     float kt;
     f3 utb;
     if (vt <= eps) {
-        kt = muS * N / (eps * h);
-        Nb = Nb + ktb * (muS / (eps * h));
+        kt = A::div(muS * N, eps * h);
+        Nb = Nb + ktb * A::div(muS, eps * h);
         utb = (-kt) * fb;
     } else {
-        const float r = vt / eps - 1.f;
+        const float r = A::div(vt, eps) - 1.f;
         const float s = 1.f - fminf(1.f, r);
-        const float C = s * muS + (1.f - s) * muK;
-        kt = C * N / ul;
-        Nb = Nb + ktb * (C / ul);
-        const float dC = r < 1.f ? (muS - muK) * (-1.f / (eps * h)) : 0.f;
-        const float ulb = ktb * ((dC * N) / ul - (C * N) / (ul * ul));
-        utb = (-kt) * fb + (ulb / ul) * ut;
+        const float C = A::fma(s, muS, (1.f - s) * muK);
+        const float iul = A::rcp(ul);
+        kt = A::div(C * N, ul);
+        Nb = Nb + ktb * (C * iul);
+        const float dC = r < 1.f ? (muS - muK) * (-A::rcp(eps * h)) : 0.f;
+        const float ulb = ktb * ((dC * N) * iul - (C * N) * (iul * iul));
+        utb = (-kt) * fb + (ulb * iul) * ut;
     }
     const float nutb = dot(n, utb);
     db = utb - nutb * n;
@@ -90,16 +94,18 @@ __device__ __forceinline__ void surface_contact_adj(const Params& P, f3 n, float
 
 // Adjoint of the surface terms of one free vertex (surfaces_at, YB/sim/contact.cuh:141-163): adds to x̄, d̄
 // and returns the adjoint of the contact stiffness k.
+template <bool F>
 __device__ __forceinline__ float surfaces_adj(const Params& P, const Grid* grids, const Pose* envPoses, int ncand,
                                               uint64_t cand, f3 p, f3 d, float k, f3 fb, f3 u, f3 dl, f3& xb, f3& db) {
+    using M = V<F>;
     float kcb = 0.f;
     for (int pl = 0; pl < P.numPlanes; pl++) {
         const f3 n = mk3(P.planes[pl][0], P.planes[pl][1], P.planes[pl][2]);
-        const float gap = ((dot(n, p) - P.planes[pl][3]) + dot(n, d)) - P.cRadius;
+        const float gap = ((M::dot(n, p) - P.planes[pl][3]) + M::dot(n, d)) - P.cRadius;
         if (gap >= 0.f) continue;
         f3 dRb, nb;
         float gapb, kb;
-        surface_contact_adj(P, n, gap, d, k, P.cMuS, P.cMuK, fb, u, dl, dRb, nb, gapb, kb);
+        surface_contact_adj<F>(P, n, gap, d, k, P.cMuS, P.cMuK, fb, u, dl, dRb, nb, gapb, kb);
         db = db + (dRb + gapb * n);
         xb = xb + gapb * n;
         kcb = kcb + kb;
@@ -114,29 +120,29 @@ __device__ __forceinline__ float surfaces_adj(const Params& P, const Grid* grids
             const f3 pp = mk3(a.x, a.y, a.z);
             float phi;
             f3 gr, mx;
-            if (!sdf_sample_t<true>(grids + m, qrot(qconj(q), (p - pp) + d), phi, gr, mx)) continue;
+            if (!sdf_sample_t<true, F>(grids + m, M::qrot(qconj(q), (p - pp) + d), phi, gr, mx)) continue;
             const float gap = phi - P.cRadius;
             if (gap >= 0.f) continue;
-            const float gl = len(gr);
+            const float gl = M::len(gr);
             if (!(gl > 0.f)) continue;
-            const f3 gh = gr / gl;
-            const f3 n = qrot(q, gh);
+            const f3 gh = M::div(gr, gl);
+            const f3 n = M::qrot(q, gh);
             f3 dRel = d, om = mk3(0.f, 0.f, 0.f);
             if (P.sdfMoving) {
                 const float4 c2 = s[2], e = s[3];
                 om = mk3(c2.z, c2.w, e.x);
-                dRel = dRel - P.h * (mk3(b.w, c2.x, c2.y) + cross(om, p - pp));
+                dRel = dRel - P.h * (mk3(b.w, c2.x, c2.y) + M::cross(om, p - pp));
             }
             f3 dRb, nb;
             float gapb, kb;
-            surface_contact_adj(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, fb, u, dl, dRb, nb, gapb, kb);
+            surface_contact_adj<F>(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, fb, u, dl, dRb, nb, gapb, kb);
             db = db + dRb;
             if (P.sdfMoving) xb = xb + P.h * cross(om, dRb);
-            const f3 ghb = qrot(qconj(q), nb);
-            const f3 grb = (ghb - dot(gh, ghb) * gh) / gl;
+            const f3 ghb = M::qrot(qconj(q), nb);
+            const f3 grb = M::div(ghb - dot(gh, ghb) * gh, gl);
             const f3 pob = gapb * gr + mk3(mx.x * grb.y + mx.y * grb.z, mx.x * grb.x + mx.z * grb.z,
                                            mx.y * grb.x + mx.z * grb.y);
-            const f3 pwb = qrot(q, pob);
+            const f3 pwb = M::qrot(q, pob);
             xb = xb + pwb;
             db = db + pwb;
             kcb = kcb + kb;
@@ -178,7 +184,7 @@ __device__ __forceinline__ Vc load_vc(const Dev& D, int i, bool act) {
 // the frame kernel (so the values, and every branch, are those of the recorded forward). Every
 // substep's inputs and every iteration's d^k, q^k, total H and Newton direction δ = H^-1 F go to the
 // history. The frame-end state is compared bit for bit with the next checkpoint.
-template <int B>
+template <int B, bool F>
 __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A, Drive Dr, int S) {
     __shared__ float4 sQ[2][B];
     __shared__ float sX[3 * B], sD[2][3 * B], sF2[9 * B];
@@ -252,8 +258,8 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
                     p1 = ld3s(sX, B, t + 1);
                     p1dx = ld3s(dk, B, t + 1);
                 }
-                vertex_terms(P, D.grids, sPose, nc, cand, c.w, c.l, c.kS, c.fl, p0, d0, y, ldq(sQ[it & 1][t]), p1, p1dx,
-                             f, H, f2, H2);
+                vertex_terms<F>(P, D.grids, sPose, nc, cand, c.w, c.l, c.kS, c.fl, p0, d0, y, ldq(sQ[it & 1][t]), p1,
+                                p1dx, f, H, f2, H2);
                 st3s(sF2, B, t, f2);
 #pragma unroll
                 for (int m = 0; m < 6; m++) sF2[(3 + m) * B + t] = H2.m[m];
@@ -269,8 +275,8 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
                 }
                 f3 d = ld3s(dk, B, t), dl = mk3(0.f, 0.f, 0.f);
                 if (c.w != 0.f) {
-                    dl = ssolve(H, f);
-                    d = solve_update(P, d, H, f, false, INFINITY);
+                    dl = V<F>::ssolve(H, f);
+                    d = solve_update<F>(P, d, H, f, false, INFINITY);
                 }
                 st3s(dn, B, t, d);
                 A.hD[hk(A, I, s, it + 1, li)] = st3(d);
@@ -287,7 +293,7 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
                 if (!(c.fl & FIX_ORIENT) && (c.fl & HAS_NEXT)) {
                     const q4 qp = (c.fl & HAS_PREV) ? ldq(sQ[it & 1][t - 1]) : q0;
                     const q4 qn = (c.fl & HAS_NEXT_ORIENT) ? ldq(sQ[it & 1][t + 1]) : q0;
-                    qo = orient_update(c.fl, c.l, c.kS, ld3s(sX, B, t), ld3s(sX, B, t + 1), ld3s(dn, B, t),
+                    qo = orient_update<F>(c.fl, c.l, c.kS, ld3s(sX, B, t), ld3s(sX, B, t + 1), ld3s(dn, B, t),
                                        ld3s(dn, B, t + 1), qp, q0, qn, c.qrp, c.qr);
                 }
                 sQ[(it + 1) & 1][t] = stq(qo);
@@ -301,8 +307,8 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
             q = ldq(sQ[I & 1][t]);
             if (c.w != 0.f) {
                 f3 tg = mk3(1.f, 0.f, 0.f);
-                if (P.dragQuad > 0.f) tg = qrot((c.fl & HAS_NEXT) ? q : ldq(sQ[I & 1][t - 1]), mk3(1.f, 0.f, 0.f));
-                v = end_velocity(P, d, tg);
+                if (P.dragQuad > 0.f) tg = V<F>::qrot_ex((c.fl & HAS_NEXT) ? q : ldq(sQ[I & 1][t - 1]));
+                v = end_velocity<F>(P, d, tg);
             }
             x = x + d;
             if (P.numSdf > 0) {
@@ -325,7 +331,7 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
 // Pass 2: backwards through the frame of one rope from the history (no recompute of the iterations).
 // Per iteration three phases: orientation adjoint (pushes to the neighbours), Newton adjoint u = a H^-1 d̄,
 // segment-term adjoint (pushes to the next vertex).
-template <int B>
+template <int B, bool F>
 __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A, Drive Dr, int S) {
     __shared__ float ex[EX_N * B];
     __shared__ Pose sPose[MAX_SDF];
@@ -353,8 +359,10 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
     double gk = 0.0, gm = 0.0, gdrag = 0.0;
     double grho[4] = {0.0, 0.0, 0.0, 0.0};
     f3 gp0 = mk3(0.f, 0.f, 0.f), gv0 = gp0, gp1 = gp0;
-    const float dmp = P.damping / h;
-    const float kap = P.cKappa / h;
+    using Af = Ar<F>;
+    using M = V<F>;
+    const float dmp = Af::div(P.damping, h);
+    const float kap = Af::div(P.cKappa, h);
     const bool upd = !(c.fl & FIX_ORIENT) && (c.fl & HAS_NEXT);
     const bool hn = (c.fl & HAS_NEXT) != 0, hp = (c.fl & HAS_PREV) != 0;
     const q4 EX{1.f, 0.f, 0.f, 0.f};
@@ -383,7 +391,7 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
         q4 aQ = aq;
         if (act && c.w != 0.f) {
             const f3 d = ld3(A.hD[hk(A, I, s, I, li)]);
-            const float invH = 1.f / h;
+            const float invH = Af::rcp(h);
             aD = aD + (invH * (1.f - P.drag * h)) * av;
             gdrag -= (double)h * (double)dot(d * invH, av);
         }
@@ -421,33 +429,33 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                 f3 pD = mk3(0.f, 0.f, 0.f);
                 if (upd) {
                     keepQ = q4{0.f, 0.f, 0.f, 0.f};
-                    const f3 er = ((X1 - X0) + (dn1 - dn0)) / c.l;
+                    const f3 er = M::div((X1 - X0) + (dn1 - dn0), c.l);
                     const f3 e = er * (-2.f * c.kS);
                     q4 b{0.f, 0.f, 0.f, 0.f};
                     float s1 = 0.f, s2 = 0.f;
                     if (hp) {
-                        s1 = qdot(qmul(qconj(qp), q0), c.qrp) > 0.f ? 1.f : -1.f;
-                        b = qadd(b, qscale(s1, qmul(qp, c.qrp)));
+                        s1 = M::qdot(M::qmul(qconj(qp), q0), c.qrp) > 0.f ? 1.f : -1.f;
+                        b = qadd(b, qscale(s1, M::qmul(qp, c.qrp)));
                     }
                     if (c.fl & HAS_NEXT_ORIENT) {
-                        s2 = qdot(qmul(qconj(q0), qn), c.qr) > 0.f ? 1.f : -1.f;
-                        b = qadd(b, qscale(s2, qmul(qn, qconj(c.qr))));
+                        s2 = M::qdot(M::qmul(qconj(q0), qn), c.qr) > 0.f ? 1.f : -1.f;
+                        b = qadd(b, qscale(s2, M::qmul(qn, qconj(c.qr))));
                     }
-                    const float le = len(e), lb = qlen(b);
+                    const float le = M::len(e), lb = M::qlen(b);
                     const float lam = le + lb;
-                    const q4 Z = qadd(qmul(qmul(mkq(e, 0.f), b), EX), qscale(lam, b));
-                    const float zl = sqrtf(qdot(Z, Z));
-                    const q4 qo = qscale(1.f / zl, Z);
-                    const q4 Zb = qscale(1.f / zl, qsub(aQ, qscale(qdot(qo, aQ), qo)));
-                    const float lamb = qdot(Zb, b);
+                    const q4 Z = qadd(M::qmul_ex(M::qmul_pure(e, b)), qscale(lam, b));
+                    const float izl = Af::rsq(M::qdot(Z, Z));
+                    const q4 qo = qscale(izl, Z);
+                    const q4 Zb = qscale(izl, qsub(aQ, qscale(M::qdot(qo, aQ), qo)));
+                    const float lamb = M::qdot(Zb, b);
                     q4 bb = qscale(lam, Zb);
-                    const q4 Ab = qmul(Zb, qconj(EX));
-                    bb = qadd(bb, qmul(qconj(mkq(e, 0.f)), Ab));
-                    f3 eb = qvec(qmul(Ab, qconj(b)));
-                    if (le > 0.f) eb = eb + (lamb / le) * e;
-                    if (lb > 0.f) bb = qadd(bb, qscale(lamb / lb, b));
+                    const q4 Ab = M::qmul(Zb, qconj(EX));
+                    bb = qadd(bb, M::qmul(qconj(mkq(e, 0.f)), Ab));
+                    f3 eb = qvec(M::qmul(Ab, qconj(b)));
+                    if (le > 0.f) eb = eb + Af::div(lamb, le) * e;
+                    if (lb > 0.f) bb = qadd(bb, qscale(Af::div(lamb, lb), b));
                     gk -= 2.0 * (double)dot(eb, er);
-                    pD = ((-2.f * c.kS) / c.l) * eb;
+                    pD = Af::div(-2.f * c.kS, c.l) * eb;
                     aD = aD - pD;
                     aX = aX - pD;
                     if (hp) {
@@ -487,7 +495,7 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                     grho[2] += gr.z;
                     grho[3] += gr.w;
                 }
-                if (c.w != 0.f) u = P.accel * ssolve(H, aD);
+                if (c.w != 0.f) u = P.accel * M::ssolve(H, aD);
                 st3s(ex + EX_U * B, B, t, u);
                 st3s(ex + (EX_U + 3) * B, B, t, dl);
             }
@@ -497,20 +505,20 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                 f3 pD = mk3(0.f, 0.f, 0.f), pX = pD;
                 const float trG = -dot(u, dl);
                 if (c.w != 0.f) {
-                    const float coef = 1.f / (h * h * c.w);
+                    const float coef = Af::rcp(h * h * c.w);
                     aY = aY + coef * u;
                     aD = aD - coef * u;
                     gm += ((double)dot(y - d0, u) + (double)trG) / ((double)h * (double)h);
                     if (P.numPlanes > 0 || P.numSdf > 0) {
-                        const float kc = kap * kap / c.w;
-                        const float kcb = surfaces_adj(P, D.grids, sPose, nc, cand, X0, d0, kc, u, u, dl, aX, aD);
+                        const float kc = Af::div(kap * kap, c.w);
+                        const float kcb = surfaces_adj<F>(P, D.grids, sPose, nc, cand, X0, d0, kc, u, u, dl, aX, aD);
                         gm += (double)kcb * ((double)kap * (double)kap);
                     }
                 }
                 if (hn) {
                     const f3 u2 = ld3s(ex + EX_U * B, B, t + 1), dl2 = ld3s(ex + (EX_U + 3) * B, B, t + 1);
-                    const float invl = 1.f / c.l;
-                    const f3 cc = ((X1 - X0) + (d1 - d0)) * invl - qrot(q0, mk3(1.f, 0.f, 0.f));
+                    const float invl = Af::rcp(c.l);
+                    const f3 cc = ((X1 - X0) + (d1 - d0)) * invl - M::qrot_ex(q0);
                     const float kk = c.kS * invl;
                     const float dd = kk * invl;
                     const f3 fdiff = u - u2;
@@ -651,14 +659,20 @@ int adjoint_block_for(int maxRopeVerts) {
     return -1;
 }
 
+template <int B, bool F>
+static hipError_t launch_frame_f(const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S, int nropes,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL((k_adjoint_forward<B, F>), dim3(nropes), dim3(B), 0, s, P, D, A, Dr, S);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_adjoint_reverse<B, F>), dim3(nropes), dim3(B), 0, s, P, D, A, Dr, S);
+    return hipGetLastError();
+}
 template <int B>
 static hipError_t launch_frame_t(const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S, int nropes,
                                  hipStream_t s) {
-    hipLaunchKernelGGL(k_adjoint_forward<B>, dim3(nropes), dim3(B), 0, s, P, D, A, Dr, S);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adjoint_reverse<B>, dim3(nropes), dim3(B), 0, s, P, D, A, Dr, S);
-    return hipGetLastError();
+    if (P.fastMath) return launch_frame_f<B, true>(P, D, A, Dr, S, nropes, s);
+    return launch_frame_f<B, false>(P, D, A, Dr, S, nropes, s);
 }
 
 hipError_t launch_adjoint_frame(int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,

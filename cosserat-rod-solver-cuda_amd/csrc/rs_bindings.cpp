@@ -115,6 +115,12 @@ public:
     bool rootsPinned = true;
 
     rs_batch* b() const { return h->b; }
+    void set_fast_math(bool on) { check(rs_set_fast_math(b(), on ? 1 : 0)); }
+    bool fast_math() const {
+        int on = 0;
+        check(rs_get_fast_math(b(), &on));
+        return on != 0;
+    }
 
     void create(int nropes, const std::vector<int>& offsets, const float* pos, const uint8_t* pinned, float density,
                 float ea, float ei, float max_h, int num_itr, bool fix_root, bool straight_rest, float min_seg) {
@@ -711,6 +717,10 @@ PYBIND11_MODULE(_ropesim, m) {
     });
 
     py::class_<RopeSim>(m, "RopeSim")
+        .def_property("fast_math", &RopeSim::fast_math, &RopeSim::set_fast_math,
+             "Arithmetic of the per-iteration routines: True (default) fused multiply-add and the hardware "
+             "reciprocal / square root / log (about 1 ulp); False every operation correctly rounded and unfused, "
+             "as the float32 oracle")
         .def(py::init<int, float, const farray&, float, float, float, float, int, bool, bool>(), py::arg("num_verts"),
              py::arg("seg_len") = 0.02f, py::arg("dir") = farray(py::cast(std::vector<float>{1, 0, 0})),
              py::arg("density") = 0.1f, py::arg("k_stretch") = 500.0f, py::arg("k_bend") = 1e-6f,
@@ -772,6 +782,10 @@ PYBIND11_MODULE(_ropesim, m) {
         .def_property_readonly("num_verts", [](RopeSim& s) { return s.n; });
 
     py::class_<RopeBatch>(m, "RopeBatch")
+        .def_property("fast_math", &RopeBatch::fast_math, &RopeBatch::set_fast_math,
+             "Arithmetic of the per-iteration routines: True (default) fused multiply-add and the hardware "
+             "reciprocal / square root / log (about 1 ulp); False every operation correctly rounded and unfused, "
+             "as the float32 oracle")
         .def(py::init([](int num_envs, int V, const farray& root_pos, float seg_len, const farray& dir, float density,
                          float k_stretch, float k_bend, float max_h, int num_itr, bool fix_root, bool /*deterministic*/) {
                  return make_batch_uniform(num_envs, V, root_pos, seg_len, dir, density, k_stretch, k_bend, max_h, num_itr,

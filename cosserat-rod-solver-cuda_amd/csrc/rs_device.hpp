@@ -75,32 +75,36 @@ __device__ __forceinline__ void store_pose(Pose* p, const Pose& P) {
 }
 
 // Penalty + regularised Coulomb contact at one surface (YB/sim/contact.cuh:27-46).
+template <bool F>
 __device__ __forceinline__ void surface_contact(const Params& P, f3 n, float gap, f3 d, float k, float muS, float muK,
                                                 f3& f, s3& H) {
+    using A = Ar<F>;
+    using M = V<F>;
     const float eps = P.cVSlip;
     const float N = -k * gap;
-    f = f + N * n;
-    s3 nn = souter(n);
-    sadd(H, sscale(nn, k));
-    const f3 ut = d - dot(n, d) * n;
-    const float ul = len(ut);
-    const float vt = ul / P.h;
+    f = M::axpy(N, n, f);
+    s3 nn = M::souter(n);
+    M::sadd_scaled(H, nn, k);
+    const f3 ut = M::axpy(-M::dot(n, d), n, d);
+    const float ul = M::len(ut);
+    const float vt = A::div(ul, P.h);
     float kt;
     if (vt <= eps) {
-        kt = muS * N / (eps * P.h);
+        kt = A::div(muS * N, eps * P.h);
     } else {
-        const float s = 1.f - fminf(1.f, vt / eps - 1.f);
-        kt = (s * muS + (1.f - s) * muK) * N / ul;
+        const float s = 1.f - fminf(1.f, A::div(vt, eps) - 1.f);
+        kt = A::div(A::fma(s, muS, (1.f - s) * muK) * N, ul);
     }
-    f = f - kt * ut;
+    f = M::axpy(-kt, ut, f);
     sadd_diag(nn, -1.f);
-    ssub(H, sscale(nn, kt));
+    M::sadd_scaled(H, nn, -kt);
 }
 
 // Trilinear SDF sample and the exact gradient of the interpolant (YB/sim/contact.cuh:57-76). With MIXED the
 // interpolant's mixed second derivatives (d2/dxdy, d2/dxdz, d2/dydz; the pure ones vanish) are returned too.
-template <bool MIXED>
+template <bool MIXED, bool F>
 __device__ __forceinline__ bool sdf_sample_t(const Grid* gp, f3 p, float& phi, f3& grad, f3& mixed) {
+    using A = Ar<F>;
     const f3 o = mk3(gp->ox, gp->oy, gp->oz);
     const float invH = gp->invH;
     const int nx = gp->nx, ny = gp->ny, nz = gp->nz;
@@ -115,43 +119,47 @@ __device__ __forceinline__ bool sdf_sample_t(const Grid* gp, f3 p, float& phi, f
     const float c000 = __ldg(c), c001 = __ldg(c + 1), c010 = __ldg(c + sy), c011 = __ldg(c + sy + 1);
     const float c100 = __ldg(c + sx), c101 = __ldg(c + sx + 1), c110 = __ldg(c + sx + sy), c111 = __ldg(c + sx + sy + 1);
     const float dx00 = c100 - c000, dx01 = c101 - c001, dx10 = c110 - c010, dx11 = c111 - c011;
-    const float c00 = c000 + fx * dx00, c01 = c001 + fx * dx01, c10 = c010 + fx * dx10, c11 = c011 + fx * dx11;
+    const float c00 = A::fma(fx, dx00, c000), c01 = A::fma(fx, dx01, c001), c10 = A::fma(fx, dx10, c010),
+                c11 = A::fma(fx, dx11, c011);
     const float dy0 = c10 - c00, dy1 = c11 - c01;
-    const float c0 = c00 + fy * dy0, c1 = c01 + fy * dy1;
-    phi = c0 + fz * (c1 - c0);
-    const float gx0 = dx00 + fy * (dx10 - dx00), gx1 = dx01 + fy * (dx11 - dx01);
-    grad = mk3(gx0 + fz * (gx1 - gx0), dy0 + fz * (dy1 - dy0), c1 - c0) * invH;
+    const float c0 = A::fma(fy, dy0, c00), c1 = A::fma(fy, dy1, c01);
+    phi = A::fma(fz, c1 - c0, c0);
+    const float gx0 = A::fma(fy, dx10 - dx00, dx00), gx1 = A::fma(fy, dx11 - dx01, dx01);
+    grad = mk3(A::fma(fz, gx1 - gx0, gx0), A::fma(fz, dy1 - dy0, dy0), c1 - c0) * invH;
     if (MIXED) {
         const float i2 = invH * invH;
-        mixed = mk3(((dx10 - dx00) + fz * ((dx11 - dx01) - (dx10 - dx00))) * i2, (gx1 - gx0) * i2, (dy1 - dy0) * i2);
+        mixed = mk3(A::fma(fz, (dx11 - dx01) - (dx10 - dx00), dx10 - dx00) * i2, (gx1 - gx0) * i2, (dy1 - dy0) * i2);
     }
     return true;
 }
+template <bool F>
 __device__ __forceinline__ bool sdf_sample(const Grid* gp, f3 p, float& phi, f3& grad) {
     f3 unused;
-    return sdf_sample_t<false>(gp, p, phi, grad, unused);
+    return sdf_sample_t<false, F>(gp, p, phi, grad, unused);
 }
 
 // Contact of a vertex with one posed mesh at the probe (pos - P.pos) + dx (YB/sim/contact.cuh:94-110). Reads
 // only the pose fields it needs (the velocities only while meshes move).
+template <bool F>
 __device__ __forceinline__ bool sdf_contact(const Params& Pm, const Grid* gp, const Pose* P, f3 p, f3 d, f3& n,
                                             float& gap, f3& dRel) {
+    using M = V<F>;
     const float4* s = reinterpret_cast<const float4*>(P);
     const float4 a = s[0], b = s[1];
     const q4 q{a.w, b.x, b.y, b.z};
     const f3 pp = mk3(a.x, a.y, a.z);
     float phi;
     f3 gr;
-    if (!sdf_sample(gp, qrot(qconj(q), (p - pp) + d), phi, gr)) return false;
+    if (!sdf_sample<F>(gp, M::qrot(qconj(q), (p - pp) + d), phi, gr)) return false;
     gap = phi - Pm.cRadius;
     if (gap >= 0.f) return false;
-    const float gl = len(gr);
+    const float gl = M::len(gr);
     if (!(gl > 0.f)) return false;
-    n = qrot(q, gr / gl);
+    n = M::qrot(q, M::div(gr, gl));
     dRel = d;
     if (Pm.sdfMoving) {
         const float4 c = s[2], e = s[3];
-        dRel = dRel - Pm.h * (mk3(b.w, c.x, c.y) + cross(mk3(c.z, c.w, e.x), p - pp));
+        dRel = dRel - Pm.h * (mk3(b.w, c.x, c.y) + M::cross(mk3(c.z, c.w, e.x), p - pp));
     }
     return true;
 }
@@ -160,13 +168,14 @@ __device__ __forceinline__ bool sdf_contact(const Params& Pm, const Grid* gp, co
 __device__ __forceinline__ int cand_get(uint64_t packed, int c) { return (int)((packed >> (6 * c)) & 63u); }
 
 // Planes, then candidate meshes (YB/sim/contact.cuh:141-163). `ncand` < 0: every mesh.
+template <bool F>
 __device__ __forceinline__ void surfaces_at(const Params& P, const Grid* grids, const Pose* envPoses, int ncand,
                                             uint64_t cand, f3 p, f3 d, float k, f3& f, s3& H) {
     for (int pl = 0; pl < P.numPlanes; pl++) {
         const f3 n = mk3(P.planes[pl][0], P.planes[pl][1], P.planes[pl][2]);
-        const float gap = ((dot(n, p) - P.planes[pl][3]) + dot(n, d)) - P.cRadius;
+        const float gap = ((V<F>::dot(n, p) - P.planes[pl][3]) + V<F>::dot(n, d)) - P.cRadius;
         if (gap >= 0.f) continue;
-        surface_contact(P, n, gap, d, k, P.cMuS, P.cMuK, f, H);
+        surface_contact<F>(P, n, gap, d, k, P.cMuS, P.cMuK, f, H);
     }
     if (P.numSdf > 0) {
         const int cnt = ncand < 0 ? P.numSdf : ncand;
@@ -174,23 +183,24 @@ __device__ __forceinline__ void surfaces_at(const Params& P, const Grid* grids, 
             const int m = ncand < 0 ? c : cand_get(cand, c);
             f3 n, dRel;
             float gap;
-            if (!sdf_contact(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
-            surface_contact(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, f, H);
+            if (!sdf_contact<F>(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
+            surface_contact<F>(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, f, H);
         }
     }
 }
 
 // Force readout at the end of a substep (YB/sim/contact.cuh:172-204): force on the vertex, and the wrench
 // the vertex applies to each mesh about its origin accumulated with float atomics.
+template <bool F>
 __device__ __forceinline__ f3 surface_readout(const Params& P, const Grid* grids, const Pose* envPoses, float* envWrench,
                                               int ncand, uint64_t cand, f3 p, f3 d, float k) {
     s3 H = szero();
     f3 fv = mk3(0.f, 0.f, 0.f);
     for (int pl = 0; pl < P.numPlanes; pl++) {
         const f3 n = mk3(P.planes[pl][0], P.planes[pl][1], P.planes[pl][2]);
-        const float gap = ((dot(n, p) - P.planes[pl][3]) + dot(n, d)) - P.cRadius;
+        const float gap = ((V<F>::dot(n, p) - P.planes[pl][3]) + V<F>::dot(n, d)) - P.cRadius;
         if (gap >= 0.f) continue;
-        surface_contact(P, n, gap, d, k, P.cMuS, P.cMuK, fv, H);
+        surface_contact<F>(P, n, gap, d, k, P.cMuS, P.cMuK, fv, H);
     }
     if (P.numSdf > 0) {
         const int cnt = ncand < 0 ? P.numSdf : ncand;
@@ -198,9 +208,9 @@ __device__ __forceinline__ f3 surface_readout(const Params& P, const Grid* grids
             const int m = ncand < 0 ? c : cand_get(cand, c);
             f3 n, dRel;
             float gap;
-            if (!sdf_contact(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
+            if (!sdf_contact<F>(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
             f3 fm = mk3(0.f, 0.f, 0.f);
-            surface_contact(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, fm, H);
+            surface_contact<F>(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, fm, H);
             fv = fv + fm;
             const Pose* Pp = envPoses + m;
             const f3 r = ((p - mk3(Pp->px, Pp->py, Pp->pz)) + d) - P.cRadius * n;
@@ -258,73 +268,79 @@ __device__ __forceinline__ void init_vertex(const Params& P, float invMass, f3 v
 
 // One rope-rope barrier + friction term of segment (i, i+1) against segment (j, j+1)
 // (YB/sim/cosserat.cu:99-158).
+template <bool F>
 __device__ __forceinline__ void collision_term(const Params& P, float dmp, f3 p0, f3 p1, f3 d0, f3 p1dx, f3 b0, f3 b1,
                                                f3 db0, f3 db1, f3& f, s3& H, f3& f2, s3& H2) {
-    const float invb = 1.f / P.barrier;
+    using A = Ar<F>;
+    using M = V<F>;
+    const float invb = A::rcp(P.barrier);
     const float rad2 = 2.f * P.radius;
     const float kCol = P.kCollision * invb;
     float u, v;
-    seg_closest(mk3(0.f, 0.f, 0.f), (p1 - p0) + (p1dx - d0), (b0 - p0) + (db0 - d0), (b1 - p0) + (db1 - d0), u, v);
-    const f3 dpos = mix(p0, p1, u) - mix(b0, b1, v);
-    const f3 ddpos = mix(d0, p1dx, u) - mix(db0, db1, v);
+    M::seg_closest(mk3(0.f, 0.f, 0.f), (p1 - p0) + (p1dx - d0), (b0 - p0) + (db0 - d0), (b1 - p0) + (db1 - d0), u, v);
+    const f3 dpos = M::mix(p0, p1, u) - M::mix(b0, b1, v);
+    const f3 ddpos = M::mix(d0, p1dx, u) - M::mix(db0, db1, v);
     f3 normal = dpos + ddpos;
-    float dd = len(normal);
+    float dd = M::len(normal);
     if (!(dd > 0.f)) return;
-    normal = normal * (1.f / dd);
+    normal = normal * A::rcp(dd);
     const float wy = u;
     const float wx = 1.f - u;
     dd = dd - rad2;
     dd = dd * invb;
     if (dd > 1.f) return;
     dd = fmaxf(dd, 1e-3f);
-    const float invd = 1.f / dd;
-    const float logd = logf(dd);
-    float dH = ((-3.f + (2.f + invd) * invd) - 2.f * logd) * kCol * invb;
-    const float ff = -(1.f - dd) * ((dd - 1.f) + 2.f * dd * logd) * invd * kCol;
-    f = f + (ff * wx - dmp * dH * wx * wx * dot(normal, d0)) * normal;
-    f2 = f2 + (ff * wy - dmp * dH * wy * wy * dot(normal, p1dx)) * normal;
+    const float invd = A::rcp(dd);
+    const float logd = A::log(dd);
+    float dH = A::fma(-2.f, logd, A::fma(2.f + invd, invd, -3.f)) * kCol * invb;
+    const float ff = -(1.f - dd) * A::fma(2.f * dd, logd, dd - 1.f) * invd * kCol;
+    f = M::axpy(A::fma(ff, wx, -(dmp * dH * wx * wx * M::dot(normal, d0))), normal, f);
+    f2 = M::axpy(A::fma(ff, wy, -(dmp * dH * wy * wy * M::dot(normal, p1dx))), normal, f2);
     dH = dH * (1.f + dmp);
-    s3 op = souter(normal);
-    sadd(H, sscale(op, dH * wx * wx));
-    sadd(H2, sscale(op, dH * wy * wy));
-    const f3 uu = ddpos - dot(normal, ddpos) * normal;
-    const float ul = len(uu);
+    s3 op = M::souter(normal);
+    M::sadd_scaled(H, op, dH * wx * wx);
+    M::sadd_scaled(H2, op, dH * wy * wy);
+    const f3 uu = M::axpy(-M::dot(normal, ddpos), normal, ddpos);
+    const float ul = M::len(uu);
     if (ul > 0.f) {
-        const float f1 = fminf(P.kFriction, P.fricMu * ff / ul);
+        const float f1 = fminf(P.kFriction, A::div(P.fricMu * ff, ul));
         sadd_diag(op, -1.f);
-        f = f - f1 * wx * uu;
-        ssub(H, sscale(op, wx * wx * f1));
-        f2 = f2 - f1 * wy * uu;
-        ssub(H2, sscale(op, wy * wy * f1));
+        f = M::axpy(-(f1 * wx), uu, f);
+        M::sadd_scaled(H, op, -(wx * wx * f1));
+        f2 = M::axpy(-(f1 * wy), uu, f2);
+        M::sadd_scaled(H2, op, -(wy * wy * f1));
     }
 }
 
 // Inertia + surface terms of vertex i and the stretch term of segment (i, i+1)
 // (YB/sim/cosserat.cu:41-88). Rope-rope terms are added by the caller.
+template <bool F>
 __device__ __forceinline__ void vertex_terms(const Params& P, const Grid* grids, const Pose* envPoses, int ncand,
                                              uint64_t cand, float invMass, float lRest, float kStretch, uint32_t fl,
                                              f3 p0, f3 d0, f3 y, q4 qi, f3 p1, f3 p1dx, f3& f, s3& H, f3& f2, s3& H2) {
+    using A = Ar<F>;
     const float h = P.h;
-    const float dmp = P.damping / h;
+    const float dmp = A::div(P.damping, h);
     f = mk3(0.f, 0.f, 0.f);
     H = szero();
     f2 = f;
     H2 = H;
     if (invMass != 0.f) {
-        H = sdiag(1.f / (invMass * h * h));
-        f = (1.f / (h * h * invMass)) * (y - d0);
+        H = sdiag(A::rcp(invMass * h * h));
+        f = A::rcp(h * h * invMass) * (y - d0);
         if (P.numPlanes > 0 || P.numSdf > 0) {
-            const float kap = P.cKappa / h;
-            surfaces_at(P, grids, envPoses, ncand, cand, p0, d0, kap * kap / invMass, f, H);
+            const float kap = A::div(P.cKappa, h);
+            surfaces_at<F>(P, grids, envPoses, ncand, cand, p0, d0, A::div(kap * kap, invMass), f, H);
         }
     }
     if (fl & HAS_NEXT) {
-        const float invl = 1.f / lRest;
-        const f3 c = ((p1 - p0) + (p1dx - d0)) * invl - qrot(qi, mk3(1.f, 0.f, 0.f));
+        const float invl = A::rcp(lRest);
+        const f3 c = ((p1 - p0) + (p1dx - d0)) * invl - V<F>::qrot_ex(qi);
         const float k = kStretch * invl;
         float dd = k * invl;
-        f = f + (k * c - (dmp * dd) * d0);
-        f2 = f2 + (-k * c - (dmp * dd) * p1dx);
+        const float dd0 = dmp * dd;
+        f = f + mk3(A::fma(k, c.x, -(dd0 * d0.x)), A::fma(k, c.y, -(dd0 * d0.y)), A::fma(k, c.z, -(dd0 * d0.z)));
+        f2 = f2 + mk3(A::fma(-k, c.x, -(dd0 * p1dx.x)), A::fma(-k, c.y, -(dd0 * p1dx.y)), A::fma(-k, c.z, -(dd0 * p1dx.z)));
         dd = dd * (1.f + dmp);
         sadd_diag(H, dd);
         sadd_diag(H2, dd);
@@ -332,45 +348,52 @@ __device__ __forceinline__ void vertex_terms(const Params& P, const Grid* grids,
 }
 
 // Local Newton update of a free vertex with the optional step clamp (YB/sim/cosserat.cu:206-215).
+template <bool F>
 __device__ __forceinline__ f3 solve_update(const Params& P, f3 d, const s3& H, f3 f, bool limit, float stepLimit) {
-    d = d + P.accel * ssolve(H, f);
+    d = V<F>::axpy(P.accel, V<F>::ssolve(H, f), d);
     if (limit) {
-        const float l = len(d);
-        if (l > stepLimit && l > 0.f) d = d * (stepLimit / l);
+        const float l = V<F>::len(d);
+        if (l > stepLimit && l > 0.f) d = d * Ar<F>::div(stepLimit, l);
     }
     return d;
 }
 
 // Closed-form quasi-static frame update of segment i (YB/sim/cosserat.cu:240-272).
+template <bool F>
 __device__ __forceinline__ q4 orient_update(uint32_t fl, float lRest, float kStretch, f3 p0, f3 p1, f3 d, f3 p1dx,
                                             q4 qPrev, q4 q0, q4 qNext, q4 qrPrev, q4 qr) {
-    f3 e = ((p1 - p0) + (p1dx - d)) / lRest;
+    using A = Ar<F>;
+    using M = V<F>;
+    f3 e = M::div((p1 - p0) + (p1dx - d), lRest);
     e = e * (-2.f * kStretch);
     q4 b{0.f, 0.f, 0.f, 0.f};
     if (fl & HAS_PREV) {
-        const float s = qdot(qmul(qconj(qPrev), q0), qrPrev) > 0.f ? 1.f : -1.f;
-        b = qadd(b, qscale(s, qmul(qPrev, qrPrev)));
+        const float s = M::qdot(M::qmul(qconj(qPrev), q0), qrPrev) > 0.f ? 1.f : -1.f;
+        b = qadd(b, qscale(s, M::qmul(qPrev, qrPrev)));
     }
     if (fl & HAS_NEXT_ORIENT) {
-        const float s = qdot(qmul(qconj(q0), qNext), qr) > 0.f ? 1.f : -1.f;
-        b = qadd(b, qscale(s, qmul(qNext, qconj(qr))));
+        const float s = M::qdot(M::qmul(qconj(q0), qNext), qr) > 0.f ? 1.f : -1.f;
+        b = qadd(b, qscale(s, M::qmul(qNext, qconj(qr))));
     }
-    const float lambda = len(e) + qlen(b);
-    const q4 prod = qmul(qmul(mkq(e, 0.f), b), q4{1.f, 0.f, 0.f, 0.f});
-    return qnormalize(qadd(prod, qscale(lambda, b)));
+    const float lambda = M::len(e) + M::qlen(b);
+    const q4 prod = M::qmul_ex(M::qmul_pure(e, b));
+    return M::qnormalize(q4{A::fma(lambda, b.x, prod.x), A::fma(lambda, b.y, prod.y), A::fma(lambda, b.z, prod.z),
+                            A::fma(lambda, b.w, prod.w)});
 }
 
 // Velocity update with quadratic and linear drag (YB/sim/iteration.cu:85-112). `t` is the segment
 // tangent (the tip uses its one real segment).
+template <bool F>
 __device__ __forceinline__ f3 end_velocity(const Params& P, f3 d, f3 t) {
-    const float invH = 1.f / P.h;
+    using M = V<F>;
+    const float invH = Ar<F>::rcp(P.h);
     f3 v = d * invH;
     const float kn = P.dragQuad;
     if (kn > 0.f) {
-        f3 vt = t * dot(t, v);
+        f3 vt = t * M::dot(t, v);
         f3 vn = v - vt;
-        vn = vn / (1.f + kn * len(vn) * P.h);
-        vt = vt / (1.f + P.dragQuadT * len(vt) * P.h);
+        vn = M::div(vn, 1.f + kn * M::len(vn) * P.h);
+        vt = M::div(vt, 1.f + P.dragQuadT * M::len(vt) * P.h);
         v = vn + vt;
     }
     return v * (1.f - P.drag * P.h);

@@ -102,6 +102,7 @@ struct rs_batch {
     float gx = 0.f, gy = -9.8f, gz = 0.f;
     float drag = 0.2f, damping = 1e-6f, dragQuad = 0.f, dragQuadT = 0.f, accel = 1.f;
     int numItr = 8;
+    bool fastMath = true;  // arithmetic of the iteration routines (Params::fastMath)
     float maxH = 1e-3f, h = 1e-3f, lastH = 0.f, time = 0.f;
     int numPlanes = 0;
     float planes[8][4] = {};
@@ -440,6 +441,7 @@ struct rs_batch {
         P.kFriction = kFriction;
         P.detR = radius + 0.5f * barrier;  // uploadMeta (YB/YarnBall.cu:261-272)
         P.scaledDetR = P.detR * detScaler;
+        P.fastMath = fastMath ? 1 : 0;
         return P;
     }
 
@@ -1524,6 +1526,19 @@ int rs_set_graph(rs_batch* b, int on) {
 int rs_graph_captures(const rs_batch* b, long long* captures) {
     if (!b || !captures) return RS_INVALID;
     *captures = b->graphCaptures;
+    return RS_OK;
+}
+
+int rs_set_fast_math(rs_batch* b, int on) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        if (b->tape.on) throw Invalid("the arithmetic mode cannot change while a rollout is being recorded");
+        b->fastMath = on != 0;
+    });
+}
+int rs_get_fast_math(const rs_batch* b, int* on) {
+    if (!b || !on) return RS_INVALID;
+    *on = b->fastMath ? 1 : 0;
     return RS_OK;
 }
 

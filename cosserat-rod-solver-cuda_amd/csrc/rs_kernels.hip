@@ -125,7 +125,7 @@ __device__ __forceinline__ uint64_t load_cand(const Dev& D, int i, int n) {
 // halo vertex on its left whose segment terms it assembles for its first owned vertex.
 constexpr int PLAIN_BLOCK = 256;
 constexpr int PLAIN_OWN = PLAIN_BLOCK - 1;
-template <bool LIMIT>
+template <bool LIMIT, bool F>
 __global__ __launch_bounds__(PLAIN_BLOCK) void k_pos(Params P, Dev D, const float4* dxIn, const float4* qIn,
                                                      float4* dxOut) {
     __shared__ float sF[10][PLAIN_BLOCK];
@@ -153,14 +153,14 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void k_pos(Params P, Dev D, const floa
             nc = D.ncand[i];
             if (nc > 0) cand = load_cand(D, i, nc);
         }
-        vertex_terms(P, D.grids, envPoses, nc, cand, prm.x, prm.y, prm.z, fl, p0, d0, ld3(D.v[i]), ldq(qIn[i]), p1,
-                     p1dx, f, H, f2, H2);
+        vertex_terms<F>(P, D.grids, envPoses, nc, cand, prm.x, prm.y, prm.z, fl, p0, d0, ld3(D.v[i]), ldq(qIn[i]), p1,
+                        p1dx, f, H, f2, H2);
         if (LIMIT && (fl & HAS_NEXT)) {
-            const float dmp = P.damping / P.h;
+            const float dmp = Ar<F>::div(P.damping, P.h);
             const int n = D.ncol[i];
             for (int c = 0; c < n; c++) {
                 const int j = D.coll[(size_t)i * MAX_COLL + c];
-                collision_term(P, dmp, p0, p1, d0, p1dx, ld3(D.x[j]), ld3(D.x[j + 1]), ld3(dxIn[j]), ld3(dxIn[j + 1]),
+                collision_term<F>(P, dmp, p0, p1, d0, p1dx, ld3(D.x[j]), ld3(D.x[j + 1]), ld3(dxIn[j]), ld3(dxIn[j + 1]),
                                f, H, f2, H2);
             }
         }
@@ -184,12 +184,13 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void k_pos(Params P, Dev D, const floa
         sadd(H, Hp);
     }
     f3 d = d0;
-    if (prm.x != 0.f) d = solve_update(P, d0, H, f, LIMIT, stepLimit);
+    if (prm.x != 0.f) d = solve_update<F>(P, d0, H, f, LIMIT, stepLimit);
     dxOut[i] = st3(d);
 }
 
 // Orientation sweep (YB/sim/cosserat.cu:227-273, snapshot mode): reads the new displacements and the
 // previous frames, writes the next frames (frames of other segments are copied through).
+template <bool F>
 __global__ void k_orient(Params P, Dev D, const float4* dxNew, const float4* qIn, float4* qOut) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.nv) return;
@@ -200,7 +201,7 @@ __global__ void k_orient(Params P, Dev D, const float4* dxNew, const float4* qIn
         const q4 qp = (fl & HAS_PREV) ? ldq(qIn[i - 1]) : q0;
         const q4 qn = (fl & HAS_NEXT_ORIENT) ? ldq(qIn[i + 1]) : q0;
         const q4 qrp = (fl & HAS_PREV) ? ldq(D.qrest[i - 1]) : q0;
-        qOut[i] = stq(orient_update(fl, prm.y, prm.z, ld3(D.x[i]), ld3(D.x[i + 1]), ld3(dxNew[i]), ld3(dxNew[i + 1]),
+        qOut[i] = stq(orient_update<F>(fl, prm.y, prm.z, ld3(D.x[i]), ld3(D.x[i + 1]), ld3(dxNew[i]), ld3(dxNew[i + 1]),
                                     qp, q0, qn, qrp, ldq(D.qrest[i])));
     } else {
         qOut[i] = stq(q0);
@@ -209,6 +210,7 @@ __global__ void k_orient(Params P, Dev D, const float4* dxNew, const float4* qIn
 
 // endItr (YB/sim/iteration.cu:77-129): velocity with drag, optional force readout, position commit,
 // broad phase for the next substep.
+template <bool F>
 __global__ void k_end(Params P, Dev D, const float4* dxIn, const float4* qCur) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.nv) return;
@@ -226,12 +228,12 @@ __global__ void k_end(Params P, Dev D, const float4* dxIn, const float4* qCur) {
     }
     if (prm.x != 0.f) {
         f3 t = mk3(1.f, 0.f, 0.f);
-        if (P.dragQuad > 0.f) t = qrot(ldq((fl & HAS_NEXT) ? qCur[i] : qCur[i - 1]), mk3(1.f, 0.f, 0.f));
-        D.v[i] = st3(end_velocity(P, d, t));
+        if (P.dragQuad > 0.f) t = V<F>::qrot_ex(ldq((fl & HAS_NEXT) ? qCur[i] : qCur[i - 1]));
+        D.v[i] = st3(end_velocity<F>(P, d, t));
         if (P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0)) {
-            const float kap = P.cKappa / P.h;
-            const f3 fv = surface_readout(P, D.grids, envPoses, D.wrench + (size_t)rope * MAX_SDF * 6, nc, cand, x, d,
-                                          kap * kap / prm.x);
+            const float kap = Ar<F>::div(P.cKappa, P.h);
+            const f3 fv = surface_readout<F>(P, D.grids, envPoses, D.wrench + (size_t)rope * MAX_SDF * 6, nc, cand, x, d,
+                                             Ar<F>::div(kap * kap, prm.x));
             float* fo = D.force + 3 * (size_t)i;
             fo[0] = fo[0] + fv.x;
             fo[1] = fo[1] + fv.y;
@@ -415,7 +417,7 @@ __global__ void k_pose_traj(Params P, Dev D, int S, Pose* out) {
     store_pose(out + idx, substep_pose(D.motion[(size_t)r * MAX_SDF + m], D.grids[m], s, S, P.h));
 }
 
-template <int BLOCK, int VPT, bool FORCES>
+template <int BLOCK, int VPT, bool FORCES, bool F>
 __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const Cell* cells, const int* order, int S,
                                                 unsigned long long* stats) {
     constexpr int CAP = BLOCK * VPT;
@@ -794,14 +796,14 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         p1 = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
                         p1dx = mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]);
                     }
-                    vertex_terms(P, D.grids, cell_poses(P, D, rope[k], s), nc[k], cand[k], prm[k].x, prm[k].y,
-                                 prm[k].z, fl, p0, d0, y[k], qo[k], p1, p1dx, fk[k], Hk[k], f2, H2);
+                    vertex_terms<F>(P, D.grids, cell_poses(P, D, rope[k], s), nc[k], cand[k], prm[k].x, prm[k].y,
+                                    prm[k].z, fl, p0, d0, y[k], qo[k], p1, p1dx, fk[k], Hk[k], f2, H2);
                     if (P.ropeContact && (fl & HAS_NEXT)) {
-                        const float dmp = P.damping / h;
+                        const float dmp = Ar<F>::div(P.damping, h);
                         const int* lst = D.coll + (size_t)gi[k] * MAX_COLL;
                         for (int c = 0; c < ncol[k]; c++) {
                             const int j = lst[c] - (gi[k] - sl);
-                            collision_term(P, dmp, p0, p1, d0, p1dx, mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]),
+                            collision_term<F>(P, dmp, p0, p1, d0, p1dx, mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]),
                                            mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]),
                                            mk3(sD[j], sD[CAP + j], sD[2 * CAP + j]),
                                            mk3(sD[j + 1], sD[CAP + j + 1], sD[2 * CAP + j + 1]), fk[k], Hk[k], f2, H2);
@@ -831,7 +833,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     }
                     if (prm[k].x != 0.f) {
                         f3 d = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
-                        d = solve_update(P, d, H, f, P.ropeContact != 0, limEff[k]);
+                        d = solve_update<F>(P, d, H, f, P.ropeContact != 0, limEff[k]);
                         sD[sl] = d.x;
                         sD[CAP + sl] = d.y;
                         sD[2 * CAP + sl] = d.z;
@@ -849,7 +851,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         const q4 q0 = ldq(sQ[sl]);
                         const q4 qp = (fl & HAS_PREV) ? ldq(sQ[sl - 1]) : q0;
                         const q4 qn = (fl & HAS_NEXT_ORIENT) ? ldq(sQ[sl + 1]) : q0;
-                        qo[k] = orient_update(fl, prm[k].y, prm[k].z, mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
+                        qo[k] = orient_update<F>(fl, prm[k].y, prm[k].z, mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
                                               mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]),
                                               mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]),
                                               mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]), qp, q0, qn, qrp[k],
@@ -875,12 +877,12 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 const Pose* envPoses = cell_poses(P, D, rope[k], s);
                 if (prm[k].x != 0.f) {
                     f3 tg = mk3(1.f, 0.f, 0.f);
-                    if (P.dragQuad > 0.f) tg = qrot((fl & HAS_NEXT) ? qo[k] : ldq(sQ[sl - 1]), mk3(1.f, 0.f, 0.f));
-                    vel[k] = end_velocity(P, d, tg);
+                    if (P.dragQuad > 0.f) tg = V<F>::qrot_ex((fl & HAS_NEXT) ? qo[k] : ldq(sQ[sl - 1]));
+                    vel[k] = end_velocity<F>(P, d, tg);
                     if (FORCES) {
-                        const float kap = P.cKappa / h;
-                        const f3 fv = surface_readout(P, D.grids, envPoses, D.wrench + (size_t)rope[k] * MAX_SDF * 6,
-                                                      nc[k], cand[k], x, d, kap * kap / prm[k].x);
+                        const float kap = Ar<F>::div(P.cKappa, h);
+                        const f3 fv = surface_readout<F>(P, D.grids, envPoses, D.wrench + (size_t)rope[k] * MAX_SDF * 6,
+                                                         nc[k], cand[k], x, d, Ar<F>::div(kap * kap, prm[k].x));
                         float* fo = D.force + 3 * (size_t)gi[k];
                         fo[0] = fo[0] + fv.x;
                         fo[1] = fo[1] + fv.y;
@@ -982,11 +984,19 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
     float4* qOut = D.q2;
     const int posBlocks = nblk(P.nv + 1, PLAIN_OWN);
     for (int it = 0; it < P.numItr; it++) {
-        if (P.ropeContact)
-            hipLaunchKernelGGL(k_pos<true>, dim3(posBlocks), dim3(PLAIN_BLOCK), 0, st, P, D, dIn, qIn, dOut);
-        else
-            hipLaunchKernelGGL(k_pos<false>, dim3(posBlocks), dim3(PLAIN_BLOCK), 0, st, P, D, dIn, qIn, dOut);
-        hipLaunchKernelGGL(k_orient, dim3(nvb), dim3(256), 0, st, P, D, dOut, qIn, qOut);
+        if (P.fastMath) {
+            if (P.ropeContact)
+                hipLaunchKernelGGL((k_pos<true, true>), dim3(posBlocks), dim3(PLAIN_BLOCK), 0, st, P, D, dIn, qIn, dOut);
+            else
+                hipLaunchKernelGGL((k_pos<false, true>), dim3(posBlocks), dim3(PLAIN_BLOCK), 0, st, P, D, dIn, qIn, dOut);
+            hipLaunchKernelGGL(k_orient<true>, dim3(nvb), dim3(256), 0, st, P, D, dOut, qIn, qOut);
+        } else {
+            if (P.ropeContact)
+                hipLaunchKernelGGL((k_pos<true, false>), dim3(posBlocks), dim3(PLAIN_BLOCK), 0, st, P, D, dIn, qIn, dOut);
+            else
+                hipLaunchKernelGGL((k_pos<false, false>), dim3(posBlocks), dim3(PLAIN_BLOCK), 0, st, P, D, dIn, qIn, dOut);
+            hipLaunchKernelGGL(k_orient<false>, dim3(nvb), dim3(256), 0, st, P, D, dOut, qIn, qOut);
+        }
         float4* t = dIn;
         dIn = dOut;
         dOut = t;
@@ -994,7 +1004,10 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
         qIn = qOut;
         qOut = t;
     }
-    hipLaunchKernelGGL(k_end, dim3(nvb), dim3(256), 0, st, P, D, dIn, qIn);
+    if (P.fastMath)
+        hipLaunchKernelGGL(k_end<true>, dim3(nvb), dim3(256), 0, st, P, D, dIn, qIn);
+    else
+        hipLaunchKernelGGL(k_end<false>, dim3(nvb), dim3(256), 0, st, P, D, dIn, qIn);
     if (qIn != D.q) {
         D.q2 = D.q;
         D.q = qIn;
@@ -1007,18 +1020,18 @@ size_t cell_lds_bytes(int block, int vpt) {
     return cap * (4 + 3 + 3 + 9 + 1 + 1 + 3 + 1) * sizeof(float) + (cap / 16) * (6 + 2 + 2) * sizeof(float);
 }
 
-template <int B, int V, bool F>
+template <int B, int V, bool FORCES, bool F>
 static hipError_t launch_cell_f(const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
                                 int ncells, int S, unsigned long long* stats, hipStream_t st) {
     const size_t lds = cell_lds_bytes(B, V);
     static bool attrSet = false;
     if (!attrSet) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cell<B, V, F>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cell<B, V, FORCES, F>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         attrSet = true;
     }
-    hipLaunchKernelGGL((k_cell<B, V, F>), dim3(ncells), dim3(B), lds, st, P, D, Dr, cells, order, S, stats);
+    hipLaunchKernelGGL((k_cell<B, V, FORCES, F>), dim3(ncells), dim3(B), lds, st, P, D, Dr, cells, order, S, stats);
     return hipGetLastError();
 }
 template <int B, int V>
@@ -1037,9 +1050,13 @@ static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, 
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0))
-        return launch_cell_f<B, V, true>(P, D, Dr, cells, order, ncells, S, stats, st);
-    return launch_cell_f<B, V, false>(P, D, Dr, cells, order, ncells, S, stats, st);
+    const bool forces = P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0);
+    if (P.fastMath) {
+        if (forces) return launch_cell_f<B, V, true, true>(P, D, Dr, cells, order, ncells, S, stats, st);
+        return launch_cell_f<B, V, false, true>(P, D, Dr, cells, order, ncells, S, stats, st);
+    }
+    if (forces) return launch_cell_f<B, V, true, false>(P, D, Dr, cells, order, ncells, S, stats, st);
+    return launch_cell_f<B, V, false, false>(P, D, Dr, cells, order, ncells, S, stats, st);
 }
 
 const CellVariant kCellVariants[] = {{64, 1}, {128, 1}, {256, 1}, {512, 1}, {1024, 1}, {256, 2}, {256, 4}, {512, 2}};

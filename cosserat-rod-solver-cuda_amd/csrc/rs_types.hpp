@@ -56,6 +56,7 @@ struct Params {
     int numSdf, sdfMoving, forcesOn;
     int ropeContact, groupRopes;
     float radius, barrier, kCollision, fricMu, kFriction, detR, scaledDetR;
+    int fastMath;  // arithmetic of the iteration routines: 0 IEEE, 1 fused multiply-add + hardware rcp/sqrt/rsq/log
 };
 
 // Device buffers.

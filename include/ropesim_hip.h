@@ -160,6 +160,15 @@ int rs_autotune(rs_batch* b, float dt, int substeps, int program, float* best_ms
  * captures. */
 int rs_set_graph(rs_batch* b, int on);
 int rs_graph_captures(const rs_batch* b, long long* captures);
+/* Arithmetic of the per-iteration routines, no counterpart in the reference (which compiles its kernels
+ * with nvcc's defaults: fused multiply-add, correctly rounded division and square root).
+ *   on = 1 (default): fused multiply-add and the hardware reciprocal, square root, reciprocal square root
+ *          and log (about 1 ulp each);
+ *   on = 0: every operation correctly rounded and unfused, the operations of the float32 oracle.
+ * Detection, broad phase, the root program and the mesh poses are IEEE in both modes. Both dispatch forms
+ * and the adjoint produce bit-identical results within a mode. Refused while a rollout is recorded. */
+int rs_set_fast_math(rs_batch* b, int on);
+int rs_get_fast_math(const rs_batch* b, int* on);
 
 /* Function-level checks, no counterpart in the reference: run the device rope-rope detection on given
  * substep-start positions and paddings (host [nv*3], [nv]) -> counts [nv], lists [nv*128] (-1 padded),

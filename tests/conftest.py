@@ -38,10 +38,12 @@ def rs():
     return ropesim
 
 
-@pytest.fixture(params=["oracle", pytest.param("hip", marks=pytest.mark.gpu)])
+@pytest.fixture(params=["oracle", pytest.param("hip", marks=pytest.mark.gpu),
+                        pytest.param("hip-ieee", marks=pytest.mark.gpu)])
 def solver(request):
-    """The analytic checks run on the CPU oracle and, on a GPU, on the HIP path (tests/hip_adapter.py gives it
-    the oracle's test surface)."""
+    """The analytic checks run on the CPU oracle and, on a GPU, on the HIP path in both arithmetic modes
+    (tests/hip_adapter.py gives it the oracle's test surface): "hip" the default fast arithmetic, "hip-ieee"
+    the correctly rounded one."""
     if request.param == "oracle":
         from oracle import pyoracle
         pyoracle.lib()
@@ -50,7 +52,7 @@ def solver(request):
     if not torch.cuda.is_available():
         pytest.fail("gpu test without a GPU")
     import hip_adapter
-    return hip_adapter
+    return hip_adapter if request.param == "hip" else hip_adapter.IEEE
 
 
 def fp32(solver, prec=64):

@@ -2,14 +2,17 @@
 checks run unchanged on the device: numpy in and out, root programs handed over as device tensors.
 
 Only what the analytic tests call is provided. Arithmetic is float32 on the device; `FP32` tells tests to
-use their float32 tolerances.
+use their float32 tolerances, `FAST` which arithmetic mode the batches run (`IEEE` is the same surface with
+every operation correctly rounded).
 """
 import ctypes as C
 import os
+import types
 
 import numpy as np
 
 FP32 = True
+FAST = True
 
 
 def _flat(positions, offsets):
@@ -25,6 +28,8 @@ def _flat(positions, offsets):
 
 
 class OracleBatch:
+    _fast = True
+
     def __init__(self, positions, density, k_stretch, k_bend, max_h, num_itr, fix_root_orientation=False,
                  pinned=None, straight_rest=False, prec=32, offsets=None):
         import ropesim
@@ -37,6 +42,7 @@ class OracleBatch:
         b = ropesim.RopeBatch.from_positions(flat, density, k_stretch, k_bend, max_h, num_itr,
                                              fix_root_orientation=fix_root_orientation, pinned=pin,
                                              straight_rest=straight_rest, offsets=off)
+        b.fast_math = self._fast
         object.__setattr__(self, "b", b)
         object.__setattr__(self, "prec", 32)
 
@@ -125,3 +131,10 @@ class OracleBatch:
 
     def step_free(self, dt, substeps):
         self.b.step_free(dt, substeps)
+
+
+class _OracleBatchIEEE(OracleBatch):
+    _fast = False
+
+
+IEEE = types.SimpleNamespace(OracleBatch=_OracleBatchIEEE, FP32=True, FAST=False)

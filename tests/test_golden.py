@@ -55,14 +55,16 @@ def _lib(rs):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("fast", [True, False], ids=["fast", "ieee"])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_device_config1_trajectory_matches_golden(rs, mode):
+def test_device_config1_trajectory_matches_golden(rs, mode, fast):
     import torch
     from ropesim import scenes
     g = load("config1_trajectory")
     s = scenes.config1(frames=60)
     b = s.build(rs)
     b.set_dispatch(mode)
+    b.fast_math = fast
     L = 0.03 * 32
     worst = 0.0
     for f, fi in enumerate(s.frame_inputs(60)):

@@ -58,7 +58,11 @@ def test_hanging_elongation_matches_EA_theory(solver, prec):
     dL = np.linalg.norm(np.diff(X, axis=0), axis=1).sum() - L0
     theory = 0.1 * L0 * G * L0 / (2 * 500.0)
     assert dL == pytest.approx(theory, rel=2e-3)
-    assert abs(X[-1, 0]) < 1e-6 and abs(X[-1, 1]) < 1e-6
+    # The tip stays on the axis. Correctly rounded arithmetic keeps the symmetric cancellations exact
+    # (1e-6 m); fused multiply-add rounds one product of a cancelling pair only (fma(a, b, -c d) is not 0
+    # when a b == c d), so the device's fast arithmetic drifts by float32 rounding: 1e-5 of rope length.
+    lat = 1e-5 * L0 if getattr(solver, "FAST", False) else 1e-6
+    assert abs(X[-1, 0]) < lat and abs(X[-1, 1]) < lat
 
 
 def test_discrete_hang_elongation_per_segment(solver):
