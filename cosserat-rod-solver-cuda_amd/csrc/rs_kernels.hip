@@ -425,7 +425,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
     static_assert(NCH <= 64 && BLOCK % 64 == 0, "chunk masks are 64 bits; waves are 64 lanes");
     extern __shared__ float4 smem4[];
     float4* sQ = smem4;                                  // [CAP] frames
-    float* sX = reinterpret_cast<float*>(sQ + CAP);     // [3][CAP] positions at the substep start
+    float4* sQr = smem4 + CAP;                           // [CAP] rest Darboux quaternions (read-only after setup)
+    float* sX = reinterpret_cast<float*>(sQr + CAP);    // [3][CAP] positions at the substep start
     float* sD = sX + 3 * CAP;                            // [3][CAP] displacement iterate
     float* sF = sD + 3 * CAP;                            // [9][CAP] segment terms for the next vertex
     float* sPad = sF + 9 * CAP;                          // [CAP]
@@ -441,16 +442,25 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
     int* sChG = reinterpret_cast<int*>(sOv + 2 * NCH);              // [2][NCH] group range of each chunk
     float* sX0 = reinterpret_cast<float*>(sChG + 2 * NCH);         // [3][CAP] positions at the last skin build
     float* sR = sX0 + 3 * CAP;                                      // [CAP] detection radius bound of the build
+    // per-vertex state read once per substep or once per iteration lives here rather than in registers
+    float* sVel = sR + CAP;      // [3][CAP] velocity
+    float* sVl = sVel + 3 * CAP;  // [3][CAP] velocity at the previous substep start
+    float* sY = sVl + 3 * CAP;    // [3][CAP] inertial target of the substep
+    auto ld3L = [&](const float* a, int sl) { return mk3(a[sl], a[CAP + sl], a[2 * CAP + sl]); };
+    auto st3L = [&](float* a, int sl, f3 v) {
+        a[sl] = v.x;
+        a[CAP + sl] = v.y;
+        a[2 * CAP + sl] = v.z;
+    };
 
     const Cell C = cells[order[blockIdx.x]];
     const int t = threadIdx.x;
 
     int gi[VPT], rope[VPT], nc[VPT], ncol[VPT];
-    int grp[VPT], rng[VPT];  // packed local [begin, end) of the contact group and of the rope (16 bits each)
-    int vcnt[VPT];           // skin-list length of the vertex's segment (-1: overflow, scan the whole group)
+    int grp[VPT];   // packed local [begin, end) of the contact group (16 bits each)
+    int vcnt[VPT];  // skin-list length of the vertex's segment (-1: overflow, scan the whole group)
     float4 prm[VPT];
-    q4 qr[VPT], qrp[VPT], qo[VPT];
-    f3 vel[VPT], vl[VPT], y[VPT];
+    q4 qo[VPT];
     float limEff[VPT];
     uint64_t cand[VPT];
     bool isRoot[VPT];
@@ -467,9 +477,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
         if (gi[k] >= 0) {
             const int i = gi[k];
             prm[k] = D.prm[i];
-            qr[k] = ldq(D.qrest[i]);
+            sQr[sl] = D.qrest[i];
             const uint32_t fl = flags_of(prm[k]);
-            qrp[k] = (fl & HAS_PREV) ? ldq(D.qrest[i - 1]) : qr[k];
             rope[k] = D.ropeOf[i];
             isRoot[k] = D.roff[rope[k]] == i;
             {  // a rope contact group occupies consecutive slots, so slot = vertex - (i - sl) inside it
@@ -477,31 +486,35 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 const int g = rope[k] / G;
                 const int base = i - sl;
                 grp[k] = (D.roff[min(g * G, P.nropes)] - base) | ((D.roff[min((g + 1) * G, P.nropes)] - base) << 16);
-                rng[k] = (D.roff[rope[k]] - base) | ((D.roff[rope[k] + 1] - base) << 16);
             }
             f3 x = ld3(D.x[i]);
             q4 q = ldq(D.q[i]);
-            vel[k] = ld3(D.v[i]);
-            vl[k] = ld3(D.vlast[i]);
+            f3 v = ld3(D.v[i]);
+            st3L(sVl, sl, ld3(D.vlast[i]));
             if (Dr.mode == DRIVE_CONST && isRoot[k]) {
                 const int e = rope[k];
                 x = mk3(Dr.p0[3 * e], Dr.p0[3 * e + 1], Dr.p0[3 * e + 2]);
-                vel[k] = mk3(Dr.v0[3 * e], Dr.v0[3 * e + 1], Dr.v0[3 * e + 2]);
+                v = mk3(Dr.v0[3 * e], Dr.v0[3 * e + 1], Dr.v0[3 * e + 2]);
                 if (Dr.hasQ0) q = qunit_or_identity(q4{Dr.q0[4 * e], Dr.q0[4 * e + 1], Dr.q0[4 * e + 2], Dr.q0[4 * e + 3]});
             }
             sX[sl] = x.x;
             sX[CAP + sl] = x.y;
             sX[2 * CAP + sl] = x.z;
+            st3L(sVel, sl, v);
             qo[k] = q;
             sQ[sl] = stq(q);
             sFl[sl] = fl | ((uint32_t)(grp[k] & 0xFFFF) << 4) | ((uint32_t)(grp[k] >> 16) << 15);
         } else {
             prm[k] = make_float4(0.f, 0.f, 0.f, 0.f);
             rope[k] = 0;
-            grp[k] = rng[k] = 0;
+            grp[k] = 0;
             isRoot[k] = false;
-            vel[k] = vl[k] = y[k] = mk3(0.f, 0.f, 0.f);
-            qr[k] = qrp[k] = qo[k] = q4{0.f, 0.f, 0.f, 1.f};
+            const f3 z = mk3(0.f, 0.f, 0.f);
+            st3L(sVel, sl, z);
+            st3L(sVl, sl, z);
+            st3L(sY, sl, z);
+            qo[k] = q4{0.f, 0.f, 0.f, 1.f};
+            sQr[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
             sX[sl] = sX[CAP + sl] = sX[2 * CAP + sl] = 0.f;
             sQ[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
             sFl[sl] = 0u;
@@ -528,7 +541,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     sX[sl] = x.x;
                     sX[CAP + sl] = x.y;
                     sX[2 * CAP + sl] = x.z;
-                    vel[k] = v;
+                    st3L(sVel, sl, v);
                     if (setQ) {
                         qo[k] = q;
                         sQ[sl] = stq(q);
@@ -541,10 +554,12 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
         for (int k = 0; k < VPT; k++) {
             if (gi[k] >= 0) {
                 const int sl = t + k * BLOCK;
-                f3 d;
+                f3 d, y;
                 float padding;
-                init_vertex(P, prm[k].x, vel[k], vl[k], d, y[k], padding);
-                vl[k] = vel[k];
+                const f3 v = ld3L(sVel, sl);
+                init_vertex(P, prm[k].x, v, ld3L(sVl, sl), d, y, padding);
+                st3L(sY, sl, y);
+                st3L(sVl, sl, v);
                 sD[sl] = d.x;
                 sD[CAP + sl] = d.y;
                 sD[2 * CAP + sl] = d.z;
@@ -728,7 +743,11 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     const int* vl = D.vlist + (size_t)(C.v0 + sl) * VERLET_CAP;
                     const bool full = vcnt[k] < 0;
                     const int g0 = grp[k] & 0xFFFF, g1 = grp[k] >> 16;
-                    const int rs0 = rng[k] & 0xFFFF, rs1 = rng[k] >> 16;
+                    int rs0 = 0, rs1 = 0;
+                    if (full) {  // local range of the vertex's own rope
+                        rs0 = D.roff[rope[k]] - (i - sl);
+                        rs1 = D.roff[rope[k] + 1] - (i - sl);
+                    }
                     const int n = full ? g1 - g0 : vcnt[k];
                     int cnt = 0, tests = 0;
                     for (int c = 0; c < n; c++) {
@@ -797,7 +816,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         p1dx = mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]);
                     }
                     vertex_terms<F>(P, D.grids, cell_poses(P, D, rope[k], s), nc[k], cand[k], prm[k].x, prm[k].y,
-                                    prm[k].z, fl, p0, d0, y[k], qo[k], p1, p1dx, fk[k], Hk[k], f2, H2);
+                                    prm[k].z, fl, p0, d0, ld3L(sY, sl), qo[k], p1, p1dx, fk[k], Hk[k], f2, H2);
                     if (P.ropeContact && (fl & HAS_NEXT)) {
                         const float dmp = Ar<F>::div(P.damping, h);
                         const int* lst = D.coll + (size_t)gi[k] * MAX_COLL;
@@ -854,8 +873,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         qo[k] = orient_update<F>(fl, prm[k].y, prm[k].z, mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
                                               mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]),
                                               mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]),
-                                              mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]), qp, q0, qn, qrp[k],
-                                              qr[k]);
+                                              mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]), qp, q0, qn,
+                                              ldq(sQr[(fl & HAS_PREV) ? sl - 1 : sl]), ldq(sQr[sl]));
                     }
                 }
             }
@@ -878,7 +897,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 if (prm[k].x != 0.f) {
                     f3 tg = mk3(1.f, 0.f, 0.f);
                     if (P.dragQuad > 0.f) tg = V<F>::qrot_ex((fl & HAS_NEXT) ? qo[k] : ldq(sQ[sl - 1]));
-                    vel[k] = end_velocity<F>(P, d, tg);
+                    st3L(sVel, sl, end_velocity<F>(P, d, tg));
                     if (FORCES) {
                         const float kap = Ar<F>::div(P.cKappa, h);
                         const f3 fv = surface_readout<F>(P, D.grids, envPoses, D.wrench + (size_t)rope[k] * MAX_SDF * 6,
@@ -913,15 +932,16 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             const f3 x = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
             const q4 q = qo[k];
             D.x[i] = st3(x);
-            D.v[i] = st3(vel[k]);
-            D.vlast[i] = st3(vl[k]);
+            const f3 vel = ld3L(sVel, sl);
+            D.v[i] = st3(vel);
+            D.vlast[i] = st3(ld3L(sVl, sl));
             D.q[i] = stq(q);
             D.outPos[3 * i] = x.x;
             D.outPos[3 * i + 1] = x.y;
             D.outPos[3 * i + 2] = x.z;
-            D.outVel[3 * i] = vel[k].x;
-            D.outVel[3 * i + 1] = vel[k].y;
-            D.outVel[3 * i + 2] = vel[k].z;
+            D.outVel[3 * i] = vel.x;
+            D.outVel[3 * i + 1] = vel.y;
+            D.outVel[3 * i + 2] = vel.z;
             D.outQuat[4 * i] = q.x;
             D.outQuat[4 * i + 1] = q.y;
             D.outQuat[4 * i + 2] = q.z;
@@ -1017,7 +1037,7 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
 
 size_t cell_lds_bytes(int block, int vpt) {
     const size_t cap = (size_t)block * vpt;
-    return cap * (4 + 3 + 3 + 9 + 1 + 1 + 3 + 1) * sizeof(float) + (cap / 16) * (6 + 2 + 2) * sizeof(float);
+    return cap * (4 + 4 + 3 + 3 + 9 + 1 + 1 + 3 + 1 + 3 + 3 + 3) * sizeof(float) + (cap / 16) * (6 + 2 + 2) * sizeof(float);
 }
 
 template <int B, int V, bool FORCES, bool F>

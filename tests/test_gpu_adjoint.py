@@ -182,3 +182,30 @@ def test_recording_refuses_what_the_adjoint_does_not_cover(rs):
     b.set_rope_contact(True, 0.001, 0.001, 0.01, 0.1, 1.0)
     with pytest.raises(ValueError, match="rope-rope"):
         b.tape_begin(2)
+
+
+@pytest.mark.parametrize("fast", [True, False], ids=["fast", "ieee"])
+def test_rollout_function_gradcheck(rs, fast):
+    """torch.autograd.gradcheck of the autograd.Function (ropesim.diff.rollout) on a tiny smooth case: two
+    9-vertex ropes, 2 frames of 5 substeps, no contact. The inputs are relative perturbations s of EA, EI,
+    mass and drag (x (1 + s)) and absolute ones of p1 (+ 1 cm * s), so one step eps = 1e-2 suits every
+    family; the device computes in float32, which sets the tolerances (float32 rounding of the positions
+    over 2 eps is ~1e-5 of a Jacobian entry of order 1e-2..1)."""
+    import torch
+    s = contact_scene(n_ropes=2, V=9, frames=2, substeps=5, with_contact=False)
+    b = s.build(rs)
+    b.fast_math = fast
+    ea0, ei0, m0 = (torch.tensor(a, dtype=torch.float64, device="cuda") for a in nominal_material(s, b))
+    p0, v0, p1 = (torch.tensor(a, dtype=torch.float64, device="cuda") for a in program(s))
+    drag0 = float(s.drag)
+
+    def fn(sea, sei, sm, sdrag, sp1):
+        x = rs.diff.rollout(b, ea0 * (1 + sea), ei0 * (1 + sei), m0 * (1 + sm), drag0 * (1 + sdrag), p0, v0,
+                            p1 + 0.01 * sp1, s.dt, s.substeps)
+        return x.double()
+
+    nv = s.num_vertices
+    z = lambda *shape: torch.zeros(*shape, dtype=torch.float64, device="cuda", requires_grad=True)
+    inputs = (z(nv), z(nv), z(nv), z(()), z(*p1.shape))
+    assert torch.autograd.gradcheck(fn, inputs, eps=1e-2, atol=2e-4, rtol=2e-2, check_undefined_grad=False,
+                                    check_batched_grad=False, nondet_tol=0.0)
