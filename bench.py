@@ -17,6 +17,7 @@ baseline (the float32 oracle on the host cores, a bounded sample of the same wor
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -26,6 +27,24 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "cosserat-rod-solver-cuda_amd")]
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 B_ALG = 136.0  # bytes per vertex-substep: the float32 state round trip (x, v, v_last, q, w, l, k, rest q, flags)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_cell_4096_uneven_ropes_hbm_traffic.md")
+
+
+def profiled_traffic():
+    """HBM bytes of one frame-kernel launch on the default workload, as measured with two rocprofv3 --pmc runs
+    (FETCH_SIZE and WRITE_SIZE, scaled by a plain device copy of known size) and recorded in the traffic
+    profile; None when the profile is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                m = re.match(r"traffic_bytes_per_launch = (\d+)\s*$", line)
+                if m:
+                    return int(m.group(1))
+    except OSError:
+        pass
+    return None
 
 
 def flops_per_vertex_substep(iters, pair_tests_per_vertex=0.0):
@@ -298,8 +317,11 @@ def main():
     if launches > 0:
         per_launch_s = kern_ms / launches * 1e-3
         achieved = B_ALG * units / per_launch_s / 1e9
+        default = args.ropes == 4096 and args.mode == 1 and args.variant < 0 and not args.ieee
+        traffic = profiled_traffic() if default else None
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                             "kernel": "k_cell (persistent frame kernel)", "kernel_ms": kern_ms / launches}
         fl = flops_per_vertex_substep(s.num_itr)
         tf = fl * units / per_launch_s / 1e12

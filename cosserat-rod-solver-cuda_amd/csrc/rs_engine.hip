@@ -195,7 +195,7 @@ struct rs_batch {
             hipEventDestroy(e.second);
         }
         void* ptrs[] = {D.x, D.v, D.vlast, D.q, D.q2, D.dx, D.dx2, D.prm, D.qrest, D.ropeOf, d_roff, D.pad, D.maxStep,
-                        D.ncol, D.coll, D.vlist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.outPos,
+                        D.ncol, D.coll, D.vlist, D.clist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.outPos,
                         D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_hits,
                         d_grids, d_cur, d_tgt, d_tele, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
                         tape.poses, tape.motion, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
@@ -592,12 +592,15 @@ struct rs_batch {
         hcheck(hipMemcpy(d_cells, cells.data(), nc * sizeof(Cell), hipMemcpyHostToDevice), "upload cells");
         hcheck(hipMemcpy(d_order, order.data(), nc * sizeof(int), hipMemcpyHostToDevice), "upload order");
         hcheck(hipMemcpy(dm, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice), "upload cell map");
-        // skin lists are indexed by cell slot
-        const size_t vl = map.size() * VERLET_CAP;
-        if (vl > vlistCap) {
+        // the persistent kernel's skin and candidate lists are indexed by cell slot
+        if (ropeContact && map.size() > vlistCap) {
             if (D.vlist) hipFree(D.vlist);
-            D.vlist = dalloc<int>(vl, "skin lists");
-            vlistCap = vl;
+            if (D.clist) hipFree(D.clist);
+            D.vlist = nullptr;
+            D.clist = nullptr;
+            D.vlist = dalloc<uint16_t>(map.size() * VERLET_CAP, "skin lists");
+            D.clist = dalloc<uint16_t>(map.size() * MAX_COLL, "cell candidate lists");
+            vlistCap = map.size();
         }
         cellsFor = var;
     }

@@ -431,11 +431,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
     float* sF = sD + 3 * CAP;                            // [9][CAP] segment terms for the next vertex
     float* sPad = sF + 9 * CAP;                          // [CAP]
     uint32_t* sFl = reinterpret_cast<uint32_t*>(sPad + CAP);  // [CAP]
-    // during the rope-rope broad phase sF holds each segment's padded box, its radius and step limit
+    // during a skin-list build sF holds each segment's padded box and, after it, the list lengths
     float* sBlo = sF;            // [3][CAP]
     float* sBhi = sF + 3 * CAP;  // [3][CAP]
-    float* sRj = sF + 6 * CAP;   // [CAP]
-    float* sLim = sF + 7 * CAP;  // [CAP]
     int* sNcol = reinterpret_cast<int*>(sF + 8 * CAP);  // [CAP]
     float* sBox = reinterpret_cast<float*>(sFl + CAP);   // [6][NCH]
     uint32_t* sOv = reinterpret_cast<uint32_t*>(sBox + 6 * NCH);  // [2][NCH] chunk-overlap bit masks (lo, hi words)
@@ -446,6 +444,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
     float* sVel = sR + CAP;      // [3][CAP] velocity
     float* sVl = sVel + 3 * CAP;  // [3][CAP] velocity at the previous substep start
     float* sY = sVl + 3 * CAP;    // [3][CAP] inertial target of the substep
+    float* sLim = sY + 3 * CAP;   // [CAP] step limit of each segment this substep
     auto ld3L = [&](const float* a, int sl) { return mk3(a[sl], a[CAP + sl], a[2 * CAP + sl]); };
     auto st3L = [&](float* a, int sl, f3 v) {
         a[sl] = v.x;
@@ -686,7 +685,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         }
                         int cnt = 0;
                         uint64_t mask = ((uint64_t)sOv[NCH + A] << 32) | sOv[A];
-                        int* vl = D.vlist + (size_t)(C.v0 + i) * VERLET_CAP;
+                        uint16_t* vl = D.vlist + (size_t)C.v0 * VERLET_CAP + i;  // entry e at vl[e * CAP]
                         while (mask) {
                             const int B = __builtin_ctzll(mask);
                             mask &= mask - 1;
@@ -707,7 +706,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
 #pragma unroll
                             for (int u = 0; u < 4; u++) {
                                 if (hm & (1u << u)) {
-                                    if (idx < VERLET_CAP) vl[idx] = B * 16 + 4 * g + u;
+                                    if (idx < VERLET_CAP) vl[(size_t)idx * CAP] = (uint16_t)(B * 16 + 4 * g + u);
                                     idx++;
                                 }
                             }
@@ -740,7 +739,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     const float mg = 1e-4f + 2e-6f * fmaxf(fabsf(a0.x), fmaxf(fabsf(a0.y), fabsf(a0.z)));
                     const float rm = ri + mg;
                     const f3 lo = vmin(a0, x1) - mk3(rm, rm, rm), hi = vmax(a0, x1) + mk3(rm, rm, rm);
-                    const int* vl = D.vlist + (size_t)(C.v0 + sl) * VERLET_CAP;
+                    const uint16_t* vl = D.vlist + (size_t)C.v0 * VERLET_CAP + sl;
+                    uint16_t* cl = D.clist + (size_t)C.v0 * MAX_COLL + sl;
                     const bool full = vcnt[k] < 0;
                     const int g0 = grp[k] & 0xFFFF, g1 = grp[k] >> 16;
                     int rs0 = 0, rs1 = 0;
@@ -750,8 +750,15 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     }
                     const int n = full ? g1 - g0 : vcnt[k];
                     int cnt = 0, tests = 0;
+                    int jn = (!full && n > 0) ? (int)vl[0] : 0;  // the next skin entry is loaded one step ahead
                     for (int c = 0; c < n; c++) {
-                        const int j = full ? g0 + c : vl[c];
+                        int j;
+                        if (full) {
+                            j = g0 + c;
+                        } else {
+                            j = jn;
+                            if (c + 1 < n) jn = (int)vl[(size_t)(c + 1) * CAP];
+                        }
                         if (full && (!(sFl[j] & HAS_NEXT) || (j >= rs0 && j < rs1 && abs(j - sl) <= 2))) continue;
                         const f3 c0 = mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]);
                         const f3 c1 = mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]);
@@ -764,7 +771,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         float d2;
                         if (pair_hit(a1, c0 - a0, c1 - a0, ri + rj, d2)) {
                             if (cnt < MAX_COLL) {
-                                D.coll[(size_t)i * MAX_COLL + cnt] = (i - sl) + j;
+                                cl[(size_t)cnt * CAP] = (uint16_t)j;
                                 minDist = fminf(minDist, 0.45f * sqrtf(d2));
                             }
                             cnt++;
@@ -779,8 +786,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 }
                 minD[k] = minDist;
             }
-            // sF held the build boxes: wait until every segment has finished with them
-            __syncthreads();
+            // sLim has a region of its own (last read before the previous substep's iterations), so one
+            // barrier publishes the limits to the neighbours
 #pragma unroll
             for (int k = 0; k < VPT; k++) sLim[t + k * BLOCK] = minD[k];
             __syncthreads();
@@ -792,7 +799,6 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     if (flags_of(prm[k]) & HAS_PREV) limEff[k] = fminf(limEff[k], sLim[sl - 1]);
                 }
             }
-            __syncthreads();  // the step limits are read before sF takes the segment terms
         }
 
         // numItr x (position sweep, orientation sweep)
@@ -819,9 +825,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                                     prm[k].z, fl, p0, d0, ld3L(sY, sl), qo[k], p1, p1dx, fk[k], Hk[k], f2, H2);
                     if (P.ropeContact && (fl & HAS_NEXT)) {
                         const float dmp = Ar<F>::div(P.damping, h);
-                        const int* lst = D.coll + (size_t)gi[k] * MAX_COLL;
+                        const uint16_t* lst = D.clist + (size_t)C.v0 * MAX_COLL + sl;
                         for (int c = 0; c < ncol[k]; c++) {
-                            const int j = lst[c] - (gi[k] - sl);
+                            const int j = lst[(size_t)c * CAP];
                             collision_term<F>(P, dmp, p0, p1, d0, p1dx, mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]),
                                            mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]),
                                            mk3(sD[j], sD[CAP + j], sD[2 * CAP + j]),
@@ -1037,7 +1043,7 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
 
 size_t cell_lds_bytes(int block, int vpt) {
     const size_t cap = (size_t)block * vpt;
-    return cap * (4 + 4 + 3 + 3 + 9 + 1 + 1 + 3 + 1 + 3 + 3 + 3) * sizeof(float) + (cap / 16) * (6 + 2 + 2) * sizeof(float);
+    return cap * (4 + 4 + 3 + 3 + 9 + 1 + 1 + 3 + 1 + 3 + 3 + 3 + 1) * sizeof(float) + (cap / 16) * (6 + 2 + 2) * sizeof(float);
 }
 
 template <int B, int V, bool FORCES, bool F>

@@ -79,7 +79,10 @@ struct Dev {
     float* maxStep;   // step limit of each segment this substep
     int* ncol;        // [nv]
     int* coll;        // [nv * MAX_COLL] row-major, ascending partner index
-    int* vlist;       // [nv * VERLET_CAP] skin lists of the persistent kernel (cell-local partner slots)
+    // Lists of the persistent kernel, per cell of `cap` slots, entry-major ([entry][slot], so the lanes of a
+    // wave reading entry e of their own lists touch consecutive addresses), cell-local slots as 16 bits:
+    uint16_t* vlist;  // skin lists, [cells][VERLET_CAP][cap]
+    uint16_t* clist;  // rope-rope candidate lists of the substep, [cells][MAX_COLL][cap]
     // meshes
     const Grid* grids;     // [MAX_SDF]
     Pose* poses;           // [nropes * MAX_SDF]
