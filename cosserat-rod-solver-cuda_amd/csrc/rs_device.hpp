@@ -267,12 +267,27 @@ __device__ __forceinline__ void init_vertex(const Params& P, float invMass, f3 v
 }
 
 // One rope-rope barrier + friction term of segment (i, i+1) against segment (j, j+1)
-// (YB/sim/cosserat.cu:99-158).
+// (YB/sim/cosserat.cu:99-158), split in two: collision_eval computes everything the term adds (it does not
+// depend on the accumulators), collision_apply adds it to f, H (vertex i) and f2, H2 (vertex i+1). The pair
+// of calls performs exactly the operations of the undivided term, so a kernel may evaluate terms on other
+// lanes and apply them in list order on the owner lane without changing a bit.
+struct PairTerm {
+    f3 n;                // contact normal
+    float s1, s2;        // normal force factors on i and i+1
+    float h1, h2;        // normal Hessian factors on i and i+1
+    f3 u;                // tangential relative displacement
+    float a1, a2;        // friction force factors on i and i+1
+    float b1, b2;        // friction Hessian factors on i and i+1
+    int on;              // bit 0: the barrier is active; bit 1: friction acts
+};
+
 template <bool F>
-__device__ __forceinline__ void collision_term(const Params& P, float dmp, f3 p0, f3 p1, f3 d0, f3 p1dx, f3 b0, f3 b1,
-                                               f3 db0, f3 db1, f3& f, s3& H, f3& f2, s3& H2) {
+__device__ __forceinline__ PairTerm collision_eval(const Params& P, float dmp, f3 p0, f3 p1, f3 d0, f3 p1dx, f3 b0,
+                                                   f3 b1, f3 db0, f3 db1) {
     using A = Ar<F>;
     using M = V<F>;
+    PairTerm T;
+    T.on = 0;
     const float invb = A::rcp(P.barrier);
     const float rad2 = 2.f * P.radius;
     const float kCol = P.kCollision * invb;
@@ -282,34 +297,60 @@ __device__ __forceinline__ void collision_term(const Params& P, float dmp, f3 p0
     const f3 ddpos = M::mix(d0, p1dx, u) - M::mix(db0, db1, v);
     f3 normal = dpos + ddpos;
     float dd = M::len(normal);
-    if (!(dd > 0.f)) return;
+    if (!(dd > 0.f)) return T;
     normal = normal * A::rcp(dd);
     const float wy = u;
     const float wx = 1.f - u;
     dd = dd - rad2;
     dd = dd * invb;
-    if (dd > 1.f) return;
+    if (dd > 1.f) return T;
     dd = fmaxf(dd, 1e-3f);
     const float invd = A::rcp(dd);
     const float logd = A::log(dd);
     float dH = A::fma(-2.f, logd, A::fma(2.f + invd, invd, -3.f)) * kCol * invb;
     const float ff = -(1.f - dd) * A::fma(2.f * dd, logd, dd - 1.f) * invd * kCol;
-    f = M::axpy(A::fma(ff, wx, -(dmp * dH * wx * wx * M::dot(normal, d0))), normal, f);
-    f2 = M::axpy(A::fma(ff, wy, -(dmp * dH * wy * wy * M::dot(normal, p1dx))), normal, f2);
+    T.n = normal;
+    T.s1 = A::fma(ff, wx, -(dmp * dH * wx * wx * M::dot(normal, d0)));
+    T.s2 = A::fma(ff, wy, -(dmp * dH * wy * wy * M::dot(normal, p1dx)));
     dH = dH * (1.f + dmp);
-    s3 op = M::souter(normal);
-    M::sadd_scaled(H, op, dH * wx * wx);
-    M::sadd_scaled(H2, op, dH * wy * wy);
-    const f3 uu = M::axpy(-M::dot(normal, ddpos), normal, ddpos);
-    const float ul = M::len(uu);
+    T.h1 = dH * wx * wx;
+    T.h2 = dH * wy * wy;
+    T.u = M::axpy(-M::dot(normal, ddpos), normal, ddpos);
+    T.on = 1;
+    const float ul = M::len(T.u);
     if (ul > 0.f) {
         const float f1 = fminf(P.kFriction, A::div(P.fricMu * ff, ul));
-        sadd_diag(op, -1.f);
-        f = M::axpy(-(f1 * wx), uu, f);
-        M::sadd_scaled(H, op, -(wx * wx * f1));
-        f2 = M::axpy(-(f1 * wy), uu, f2);
-        M::sadd_scaled(H2, op, -(wy * wy * f1));
+        T.a1 = -(f1 * wx);
+        T.b1 = -(wx * wx * f1);
+        T.a2 = -(f1 * wy);
+        T.b2 = -(wy * wy * f1);
+        T.on = 3;
     }
+    return T;
+}
+
+template <bool F>
+__device__ __forceinline__ void collision_apply(const PairTerm& T, f3& f, s3& H, f3& f2, s3& H2) {
+    using M = V<F>;
+    if (!(T.on & 1)) return;
+    f = M::axpy(T.s1, T.n, f);
+    f2 = M::axpy(T.s2, T.n, f2);
+    s3 op = M::souter(T.n);
+    M::sadd_scaled(H, op, T.h1);
+    M::sadd_scaled(H2, op, T.h2);
+    if (T.on & 2) {
+        sadd_diag(op, -1.f);
+        f = M::axpy(T.a1, T.u, f);
+        M::sadd_scaled(H, op, T.b1);
+        f2 = M::axpy(T.a2, T.u, f2);
+        M::sadd_scaled(H2, op, T.b2);
+    }
+}
+
+template <bool F>
+__device__ __forceinline__ void collision_term(const Params& P, float dmp, f3 p0, f3 p1, f3 d0, f3 p1dx, f3 b0, f3 b1,
+                                               f3 db0, f3 db1, f3& f, s3& H, f3& f2, s3& H2) {
+    collision_apply<F>(collision_eval<F>(P, dmp, p0, p1, d0, p1dx, b0, b1, db0, db1), f, H, f2, H2);
 }
 
 // Inertia + surface terms of vertex i and the stretch term of segment (i, i+1)

@@ -801,7 +801,17 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             }
         }
 
-        // numItr x (position sweep, orientation sweep)
+        // numItr x (position sweep, orientation sweep). A wave with rope-rope pairs this substep issues ahead
+        // of the others on its SIMD: the barriers wait for it.
+        if (P.ropeContact) {
+            bool pairs = false;
+#pragma unroll
+            for (int k = 0; k < VPT; k++) pairs = pairs || ncol[k] > 0;
+            if (__any(pairs))
+                __builtin_amdgcn_s_setprio(2);
+            else
+                __builtin_amdgcn_s_setprio(0);
+        }
         for (int it = 0; it < P.numItr; it++) {
             f3 fk[VPT];
             s3 Hk[VPT];
@@ -829,9 +839,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         for (int c = 0; c < ncol[k]; c++) {
                             const int j = lst[(size_t)c * CAP];
                             collision_term<F>(P, dmp, p0, p1, d0, p1dx, mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]),
-                                           mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]),
-                                           mk3(sD[j], sD[CAP + j], sD[2 * CAP + j]),
-                                           mk3(sD[j + 1], sD[CAP + j + 1], sD[2 * CAP + j + 1]), fk[k], Hk[k], f2, H2);
+                                              mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]),
+                                              mk3(sD[j], sD[CAP + j], sD[2 * CAP + j]),
+                                              mk3(sD[j + 1], sD[CAP + j + 1], sD[2 * CAP + j + 1]), fk[k], Hk[k], f2, H2);
                         }
                     }
                 }
@@ -1085,7 +1095,8 @@ static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, 
     return launch_cell_f<B, V, false, false>(P, D, Dr, cells, order, ncells, S, stats, st);
 }
 
-const CellVariant kCellVariants[] = {{64, 1}, {128, 1}, {256, 1}, {512, 1}, {1024, 1}, {256, 2}, {256, 4}, {512, 2}};
+const CellVariant kCellVariants[] = {{64, 1},  {128, 1}, {256, 1}, {384, 1}, {512, 1},
+                                     {768, 1}, {1024, 1}, {256, 2}, {256, 4}, {512, 2}};
 const int kNumCellVariants = sizeof(kCellVariants) / sizeof(kCellVariants[0]);
 
 hipError_t launch_cell(int variant, const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
@@ -1095,11 +1106,13 @@ hipError_t launch_cell(int variant, const Params& P, const Dev& D, const Drive& 
         case 0: return launch_cell_t<64, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
         case 1: return launch_cell_t<128, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
         case 2: return launch_cell_t<256, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
-        case 3: return launch_cell_t<512, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
-        case 4: return launch_cell_t<1024, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
-        case 5: return launch_cell_t<256, 2>(P, D, Dr, cells, order, ncells, S, stats, st);
-        case 6: return launch_cell_t<256, 4>(P, D, Dr, cells, order, ncells, S, stats, st);
-        case 7: return launch_cell_t<512, 2>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 3: return launch_cell_t<384, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 4: return launch_cell_t<512, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 5: return launch_cell_t<768, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 6: return launch_cell_t<1024, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 7: return launch_cell_t<256, 2>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 8: return launch_cell_t<256, 4>(P, D, Dr, cells, order, ncells, S, stats, st);
+        case 9: return launch_cell_t<512, 2>(P, D, Dr, cells, order, ncells, S, stats, st);
         default: return hipErrorInvalidValue;
     }
 }
