@@ -56,8 +56,10 @@ __device__ __forceinline__ Pose substep_pose(const Motion& M, const Grid& g, int
     }
     f3 lo, hi;
     world_box(g, pos, q, lo, hi);
+    // the padding build_candidates adds for a moving mesh, once per pose instead of once per vertex
+    const float bpad = h * (len(vel) + len(om) * 0.5f * len(hi - lo));
     return Pose{pos.x, pos.y, pos.z, q.x, q.y, q.z, q.w, vel.x, vel.y, vel.z, om.x, om.y, om.z,
-                lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, 0.f};
+                lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, bpad};
 }
 
 __device__ __forceinline__ Pose load_pose(const Pose* p) {
@@ -71,7 +73,7 @@ __device__ __forceinline__ void store_pose(Pose* p, const Pose& P) {
     s[1] = make_float4(P.qy, P.qz, P.qw, P.vx);
     s[2] = make_float4(P.vy, P.vz, P.wx, P.wy);
     s[3] = make_float4(P.wz, P.lox, P.loy, P.loz);
-    s[4] = make_float4(P.hix, P.hiy, P.hiz, 0.f);
+    s[4] = make_float4(P.hix, P.hiy, P.hiz, P.bpad);
 }
 
 // Penalty + regularised Coulomb contact at one surface (YB/sim/contact.cuh:27-46).
@@ -236,11 +238,8 @@ __device__ __forceinline__ int build_candidates(const Params& P, const Pose* env
         const float4* s = reinterpret_cast<const float4*>(envPoses + m);
         const float4 e = s[3], f = s[4];
         const f3 lo = mk3(e.y, e.z, e.w), hi = mk3(f.x, f.y, f.z);
-        float pm = padv;
-        if (P.sdfMoving) {
-            const float4 b = s[1], c = s[2];
-            pm = padv + P.h * (len(mk3(b.w, c.x, c.y)) + len(mk3(c.z, c.w, e.x)) * 0.5f * len(hi - lo));
-        }
+        // moving meshes: + h (|v| + |w| |hi - lo| / 2), evaluated with the pose (substep_pose)
+        const float pm = P.sdfMoving ? padv + f.w : padv;
         if (p.x >= lo.x - pm && p.x <= hi.x + pm && p.y >= lo.y - pm && p.y <= hi.y + pm && p.z >= lo.z - pm &&
             p.z <= hi.z + pm) {
             if (n < MAX_CAND) packed |= (uint64_t)m << (6 * n);

@@ -31,7 +31,8 @@ struct Grid {
 // Pose of (rope, mesh) at the end of the current substep plus its velocity over it and the world
 // AABB of its lattice (YarnBall.h:147). 20 floats so a slot is five float4 loads.
 struct Pose {
-    float px, py, pz, qx, qy, qz, qw, vx, vy, vz, wx, wy, wz, lox, loy, loz, hix, hiy, hiz, pad_;
+    float px, py, pz, qx, qy, qz, qw, vx, vy, vz, wx, wy, wz, lox, loy, loz, hix, hiy, hiz;
+    float bpad;  // the broad phase's moving-mesh padding h (|v| + |w| |hi - lo| / 2) of this pose
 };
 // One frame of kinematic motion (YarnBall.h:152).
 struct Motion {
