@@ -232,7 +232,7 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
         if (act) {
             f3 d;
             float padding;
-            init_vertex(P, c.w, v, vl, d, y, padding);
+            init_vertex<F>(P, c.w, v, vl, d, y, padding);
             A.hX[hs(A, s, li)] = make_float4(x.x, x.y, x.z, __int_as_float(nc));
             A.hV[hs(A, s, li)] = st3(v);
             A.hVL[hs(A, s, li)] = st3(vl);
@@ -434,12 +434,14 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                     q4 b{0.f, 0.f, 0.f, 0.f};
                     float s1 = 0.f, s2 = 0.f;
                     if (hp) {
-                        s1 = M::qdot(M::qmul(qconj(qp), q0), c.qrp) > 0.f ? 1.f : -1.f;
-                        b = qadd(b, qscale(s1, M::qmul(qp, c.qrp)));
+                        const q4 tp = M::qmul(qp, c.qrp);
+                        s1 = twist_sign_prev<F>(qp, q0, c.qrp, tp);
+                        b = qadd(b, qscale(s1, tp));
                     }
                     if (c.fl & HAS_NEXT_ORIENT) {
-                        s2 = M::qdot(M::qmul(qconj(q0), qn), c.qr) > 0.f ? 1.f : -1.f;
-                        b = qadd(b, qscale(s2, M::qmul(qn, qconj(c.qr))));
+                        const q4 tn = M::qmul(qn, qconj(c.qr));
+                        s2 = twist_sign_next<F>(q0, qn, c.qr, tn);
+                        b = qadd(b, qscale(s2, tn));
                     }
                     const float le = M::len(e), lb = M::qlen(b);
                     const float lam = le + lb;
@@ -551,10 +553,9 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             if (c.w != 0.f) {
                 vb = vb + h * aY;
                 const f3 g = mk3(P.gx, P.gy, P.gz);
-                const float g2 = len2(g);
+                const float g2 = M::len2(g);
                 if (g2 > 0.f) {
-                    const f3 a = (vs - vls) / P.lastH;
-                    const float sr = dot(a, g) / g2;
+                    const float sr = gravity_fraction<F>(P, vs, vls, g, g2);  // the forward's clamp decision
                     if (sr > 0.f && sr < 1.f) {
                         const f3 ab = ((h * h) * dot(g, aD) / g2) * g;
                         vb = vb + ab / P.lastH;

@@ -249,17 +249,24 @@ __device__ __forceinline__ int build_candidates(const Params& P, const Pose* env
     return n <= MAX_CAND ? n : -1;
 }
 
+// The gravity-acceleration clamp of initItr: the fraction of gravity the last substep's acceleration
+// carried, (a . g) / |g|^2 with a = (v - v_last) / lastH (YB/sim/iteration.cu:47-56); unclamped.
+template <bool F>
+__device__ __forceinline__ float gravity_fraction(const Params& P, f3 v, f3 vl, f3 g, float g2) {
+    return Ar<F>::div(V<F>::dot(V<F>::div(v - vl, P.lastH), g), g2);
+}
+
 // initItr for one vertex (YB/sim/iteration.cu:34-69): displacement guess, inertial target y, padding.
+template <bool F>
 __device__ __forceinline__ void init_vertex(const Params& P, float invMass, f3 v, f3 vl, f3& d, f3& y, float& padding) {
     const f3 g = mk3(P.gx, P.gy, P.gz);
     d = P.h * v;
     y = d + (P.h * P.h) * g;
-    padding = P.scaledDetR + len(y);
+    padding = P.scaledDetR + V<F>::len(y);
     if (invMass != 0.f) {
-        const float g2 = len2(g);
+        const float g2 = V<F>::len2(g);
         if (g2 > 0.f) {
-            const f3 a = (v - vl) / P.lastH;
-            const float s = clampf(dot(a, g) / g2, 0.f, 1.f);
+            const float s = clampf(gravity_fraction<F>(P, v, vl, g, g2), 0.f, 1.f);
             d = d + (P.h * P.h * s) * g;
         }
     }
@@ -398,6 +405,22 @@ __device__ __forceinline__ f3 solve_update(const Params& P, f3 d, const s3& H, f
     return d;
 }
 
+// Signs of the rest-twist terms of the frame update (YB/sim/cosserat.cu:253-262): sign <conj(q_{i-1}) q_i,
+// rho_{i-1}> and sign <conj(q_i) q_{i+1}, rho_i>. Left and right quaternion multiplications are transposed
+// by the conjugate (<conj(a) b, c> = <b, a c>, <a c, d> = <a, d conj(c)>), so the fast mode takes the same
+// numbers as <q_i, t> with t = q_{i-1} rho_{i-1} and t = q_{i+1} conj(rho_i), the products the update needs
+// anyway; IEEE mode evaluates the reference's form.
+template <bool F>
+__device__ __forceinline__ float twist_sign_prev(q4 qPrev, q4 q0, q4 qrPrev, q4 t) {
+    const float c = F ? V<F>::qdot(q0, t) : V<F>::qdot(V<F>::qmul(qconj(qPrev), q0), qrPrev);
+    return c > 0.f ? 1.f : -1.f;
+}
+template <bool F>
+__device__ __forceinline__ float twist_sign_next(q4 q0, q4 qNext, q4 qr, q4 t) {
+    const float c = F ? V<F>::qdot(q0, t) : V<F>::qdot(V<F>::qmul(qconj(q0), qNext), qr);
+    return c > 0.f ? 1.f : -1.f;
+}
+
 // Closed-form quasi-static frame update of segment i (YB/sim/cosserat.cu:240-272).
 template <bool F>
 __device__ __forceinline__ q4 orient_update(uint32_t fl, float lRest, float kStretch, f3 p0, f3 p1, f3 d, f3 p1dx,
@@ -408,12 +431,12 @@ __device__ __forceinline__ q4 orient_update(uint32_t fl, float lRest, float kStr
     e = e * (-2.f * kStretch);
     q4 b{0.f, 0.f, 0.f, 0.f};
     if (fl & HAS_PREV) {
-        const float s = M::qdot(M::qmul(qconj(qPrev), q0), qrPrev) > 0.f ? 1.f : -1.f;
-        b = qadd(b, qscale(s, M::qmul(qPrev, qrPrev)));
+        const q4 t = M::qmul(qPrev, qrPrev);
+        b = qadd(b, qscale(twist_sign_prev<F>(qPrev, q0, qrPrev, t), t));
     }
     if (fl & HAS_NEXT_ORIENT) {
-        const float s = M::qdot(M::qmul(qconj(q0), qNext), qr) > 0.f ? 1.f : -1.f;
-        b = qadd(b, qscale(s, M::qmul(qNext, qconj(qr))));
+        const q4 t = M::qmul(qNext, qconj(qr));
+        b = qadd(b, qscale(twist_sign_next<F>(q0, qNext, qr, t), t));
     }
     const float lambda = M::len(e) + M::qlen(b);
     const q4 prod = M::qmul_ex(M::qmul_pure(e, b));

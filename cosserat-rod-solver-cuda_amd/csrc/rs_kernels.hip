@@ -57,6 +57,7 @@ __global__ void k_root_program(Params P, Dev D, Drive Dr, float4* q, int s, int 
 
 // initItr (YB/sim/iteration.cu:10-69): kinematic mesh poses of this substep, then per vertex the
 // displacement guess, the inertial target (stored in the velocity slot) and the detection padding.
+template <bool F>
 __global__ void k_init(Params P, Dev D, int sub, int nSub, float4* dxOut) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (P.sdfMoving && tid < P.nropes * MAX_SDF) {
@@ -70,7 +71,7 @@ __global__ void k_init(Params P, Dev D, int sub, int nSub, float4* dxOut) {
     D.vlast[tid] = st3(v);
     f3 d, y;
     float padding;
-    init_vertex(P, prm.x, v, vl, d, y, padding);
+    init_vertex<F>(P, prm.x, v, vl, d, y, padding);
     if (prm.x != 0.f) D.v[tid] = st3(y);
     D.pad[tid] = padding;
     dxOut[tid] = st3(d);
@@ -556,7 +557,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 f3 d, y;
                 float padding;
                 const f3 v = ld3L(sVel, sl);
-                init_vertex(P, prm[k].x, v, ld3L(sVl, sl), d, y, padding);
+                init_vertex<F>(P, prm[k].x, v, ld3L(sVl, sl), d, y, padding);
                 st3L(sY, sl, y);
                 st3L(sVl, sl, v);
                 sD[sl] = d.x;
@@ -1012,7 +1013,10 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
     if (Dr.mode == DRIVE_PROGRAM)
         hipLaunchKernelGGL(k_root_program, dim3(nblk(P.nropes, 128)), dim3(128), 0, st, P, D, Dr, D.q, s, S);
     const int initThreads = P.sdfMoving ? max(P.nv, P.nropes * MAX_SDF) : P.nv;
-    hipLaunchKernelGGL(k_init, dim3(nblk(initThreads, 256)), dim3(256), 0, st, P, D, s, S, D.dx);
+    if (P.fastMath)
+        hipLaunchKernelGGL(k_init<true>, dim3(nblk(initThreads, 256)), dim3(256), 0, st, P, D, s, S, D.dx);
+    else
+        hipLaunchKernelGGL(k_init<false>, dim3(nblk(initThreads, 256)), dim3(256), 0, st, P, D, s, S, D.dx);
     if (P.ropeContact) hipLaunchKernelGGL(k_detect, dim3(nblk(P.nv, 128)), dim3(128), 0, st, P, D);
     float4* dIn = D.dx;
     float4* dOut = D.dx2;
