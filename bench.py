@@ -13,6 +13,8 @@ baseline (the float32 oracle on the host cores, a bounded sample of the same wor
 "differentiation" object is the second half of the metric, on the config-4 workload:
 - forward time of the per-substep kernels and of the persistent frame kernel;
 - a differentiable rollout's forward (with per-frame checkpoints) and adjoint time, and their ratio.
+The "readout" object continues the headline batch with the contact-force readout on (float wrench sums, then
+the reproducible fixed-point sums) and reports the time per frame of each against the readout off.
 """
 import argparse
 import json
@@ -245,6 +247,7 @@ def main():
     ap.add_argument("--diff-frames", type=int, default=120, help="config-4 frames (0 skips the leg)")
     ap.add_argument("--scale-frames", type=int, default=3, help="config-5 forward frames timed (0 skips the leg)")
     ap.add_argument("--scale-adjoint-frames", type=int, default=30, help="config-5 frames of the adjoint leg")
+    ap.add_argument("--no-readout", action="store_true", help="skip the contact-force readout leg")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -260,10 +263,11 @@ def main():
 
     t_setup = time.perf_counter()
     K, W = args.steps, args.warmup
-    s = scenes.config3(n_ropes=args.ropes, frames=K + W)
+    RK = 0 if args.no_readout or K < 2 else K  # frames per readout mode (one warm-up), after the headline frames
+    s = scenes.config3(n_ropes=args.ropes, frames=K + W + 2 * RK)
     b = s.build(ropesim)
     b.fast_math = not args.ieee
-    fis = s.frame_inputs(K + W)
+    fis = s.frame_inputs(K + W + 2 * RK)
     dev = [{k: torch.as_tensor(v, device="cuda") for k, v in fi.items() if v is not None} for fi in fis]
     torch.cuda.synchronize()
     tuned_ms = None
@@ -294,6 +298,26 @@ def main():
     b.set_kernel_timing(False)
     x = torch.as_tensor(b.positions(), device="cuda")
     assert bool(torch.isfinite(x).all()), "non-finite state"
+
+    readout = None
+    if RK > 0:  # the same batch continues with the contact-force readout on: float sums, then fixed-point sums
+        readout = {"workload": "the headline batch continued with enable_contact_forces(True): per-vertex forces "
+                               "and per-(rope, mesh) wrenches averaged over each frame", "frames": RK,
+                   "off_ms_per_frame": 1e3 * wall / K}
+        k = W + K
+        for name, det in (("float", False), ("fixed_point", True)):
+            b.enable_contact_forces(True, deterministic=det)
+            s.step(b, dev[k], lambda a: a)  # recapture with the readout compiled in
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for j in range(k + 1, k + RK):
+                s.step(b, dev[j], lambda a: a)
+            torch.cuda.synchronize()
+            readout[name + "_ms_per_frame"] = 1e3 * (time.perf_counter() - t1) / (RK - 1)
+            k += RK
+        b.enable_contact_forces(False)
+        for name in ("float", "fixed_point"):
+            readout[name + "_overhead"] = readout[name + "_ms_per_frame"] / readout["off_ms_per_frame"] - 1.0
 
     units = s.num_vertices * s.substeps          # vertex-substeps per frame
     value = units * K / wall
@@ -332,6 +356,8 @@ def main():
         tf = fl * units / per_launch_s / 1e12
         line["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP32_PEAK_TFLOPS,
                         "flop_per_vertex_substep": fl, "note": "solver flops only; rope-rope pair tests excluded"}
+    if readout is not None:
+        line["readout"] = readout
     if args.diff_frames > 0:
         d = differentiation_bench(args.diff_ropes, args.diff_frames)
         line["differentiation"] = d

@@ -263,7 +263,17 @@ public:
         check(rs_set_mesh_targets(b(), index, hp.data(), hq.data(), rows, tele, teleAll, 0));
     }
     void clear_meshes() { check(rs_clear_meshes(b())); }
-    void enable_contact_forces(bool on) { check(rs_enable_contact_forces(b(), on ? 1 : 0)); }
+    // deterministic: wrench sums in 64-bit fixed point, reproducible bit for bit (an extension; the reference's
+    // float atomics are not). None leaves the current mode.
+    void enable_contact_forces(bool on, py::object deterministic) {
+        if (!deterministic.is_none()) check(rs_set_contact_force_mode(b(), deterministic.cast<bool>() ? 1 : 0));
+        check(rs_enable_contact_forces(b(), on ? 1 : 0));
+    }
+    bool deterministic_forces() const {
+        int on = 0;
+        check(rs_get_contact_force_mode(b(), &on));
+        return on != 0;
+    }
     int num_meshes() const { return rs_num_meshes(b()); }
     py::array_t<float> contact_forces() {
         py::array_t<float> out(vshape(3));
@@ -754,7 +764,9 @@ PYBIND11_MODULE(_ropesim, m) {
              },
              py::arg("index"), py::arg("pos"), py::arg("quat"), py::arg("teleport") = false)
         .def("clear_meshes", &RopeSim::clear_meshes)
-        .def("enable_contact_forces", &RopeSim::enable_contact_forces, py::arg("on") = true)
+        .def("enable_contact_forces", &RopeSim::enable_contact_forces, py::arg("on") = true,
+             py::arg("deterministic") = py::none())
+        .def_property_readonly("deterministic_forces", &RopeSim::deterministic_forces)
         .def("contact_forces", &RopeSim::contact_forces_1)
         .def("mesh_wrenches", &RopeSim::mesh_wrenches_1)
         .def("set_velocities", &RopeSim::set_velocities, py::arg("velocities"))
@@ -851,7 +863,9 @@ PYBIND11_MODULE(_ropesim, m) {
              },
              py::arg("pos"), py::arg("quat"), py::arg("teleport") = false)
         .def("clear_meshes", &RopeBatch::clear_meshes)
-        .def("enable_contact_forces", &RopeBatch::enable_contact_forces, py::arg("on") = true)
+        .def("enable_contact_forces", &RopeBatch::enable_contact_forces, py::arg("on") = true,
+             py::arg("deterministic") = py::none())
+        .def_property_readonly("deterministic_forces", &RopeBatch::deterministic_forces)
         .def("set_gravity", &RopeBatch::set_gravity, py::arg("gravity"))
         .def("inv_masses", &RopeBatch::inv_masses)
         .def("set_masses", [](RopeBatch& b, const farray& a) { b.set_per_vertex(RS_V_MASSES, a, "set_masses", "[V] or [num_envs, V] node masses (kg)"); },

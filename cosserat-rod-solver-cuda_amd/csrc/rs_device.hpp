@@ -191,39 +191,151 @@ __device__ __forceinline__ void surfaces_at(const Params& P, const Grid* grids, 
     }
 }
 
-// Force readout at the end of a substep (YB/sim/contact.cuh:172-204): force on the vertex, and the wrench
-// the vertex applies to each mesh about its origin accumulated with float atomics.
+// Wrench accumulation of the force readout. The wrench array is [ropes][MAX_SDF][6]; `key` = rope * MAX_SDF
+// + mesh names one wrench. Lanes of a wave holding a contribution to the same wrench are summed across the
+// wave first, and one lane issues the six global updates: a wave of vertices resting on one mesh costs six
+// atomics instead of six per lane. Fixed-point accumulation (P.forcesFixed): every contribution is rounded
+// to a 64-bit integer in units of 2^-WRENCH_FIX_BITS before any sum, and integer sums are exact, so the
+// result depends neither on the order of the updates nor on which lanes share a wave: it is the same bit
+// for bit in every dispatch form, cell variant and run.
+constexpr int WRENCH_FIX_BITS = 32;
+constexpr float WRENCH_FIX_MAX = 1.0e9f;  // |contribution| clamp, keeps 2^32 * value inside int64
+
+__device__ __forceinline__ long long wrench_to_fixed(float w) {
+    const float c = fminf(fmaxf(w, -WRENCH_FIX_MAX), WRENCH_FIX_MAX);
+    return __double2ll_rn((double)c * 4294967296.0);
+}
+
+// Sums over each row of 16 lanes, left in every lane of the row (DPP: quad swaps, then the half-row and row
+// mirrors). Every lane of the wave must be active.
+__device__ __forceinline__ float row16_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));  // row_mirror
+    return v;
+}
+template <int CTRL>
+__device__ __forceinline__ long long mov_dpp64(long long v) {
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(unsigned long long)v, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)((unsigned long long)v >> 32), CTRL, 0xF, 0xF, false);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ long long row16_sum(long long v) {
+    v += mov_dpp64<0xB1>(v);
+    v += mov_dpp64<0x4E>(v);
+    v += mov_dpp64<0x141>(v);
+    v += mov_dpp64<0x140>(v);
+    return v;
+}
+// Wave total from the row sums, as a wave-uniform value: ((row0 + row1) + row2) + row3.
+__device__ __forceinline__ float wave_total(float v) {
+    v = row16_sum(v);
+    const int b = __float_as_int(v);
+    return ((__int_as_float(__builtin_amdgcn_readlane(b, 0)) + __int_as_float(__builtin_amdgcn_readlane(b, 16))) +
+            __int_as_float(__builtin_amdgcn_readlane(b, 32))) +
+           __int_as_float(__builtin_amdgcn_readlane(b, 48));
+}
+__device__ __forceinline__ long long readlane64(long long v, int l) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(unsigned long long)v, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)v >> 32), l);
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ long long wave_total(long long v) {
+    v = row16_sum(v);
+    return readlane64(v, 0) + readlane64(v, 16) + readlane64(v, 32) + readlane64(v, 48);
+}
+
+// Lanes sharing a wrench sum across the wave when more than this many hold a contribution to it; fewer
+// update it directly (their atomics hardly contend).
+constexpr int WAVE_SUM_MIN_LANES = 3;
+
+// Every lane of the wave calls this, in uniform control flow; `has` marks the lanes with a contribution.
+__device__ __forceinline__ void wave_wrench_flush(const Params& P, const Dev& D, bool has, int key, const float (&w)[6]) {
+    unsigned long long pend = __ballot(has);
+    while (pend) {
+        const int lead = __ffsll((long long)pend) - 1;
+        const int lk = __builtin_amdgcn_readlane(key, lead);
+        const bool mine = has && key == lk;
+        const unsigned long long grp = __ballot(mine);
+        if (__popcll(grp) < WAVE_SUM_MIN_LANES) {
+            if (mine) {
+                if (P.forcesFixed) {
+                    unsigned long long* dst = D.wrenchFx + (size_t)lk * 6;
+#pragma unroll
+                    for (int j = 0; j < 6; j++) atomicAdd(dst + j, (unsigned long long)wrench_to_fixed(w[j]));
+                } else {
+                    float* dst = D.wrench + (size_t)lk * 6;
+#pragma unroll
+                    for (int j = 0; j < 6; j++) atomicAdd(dst + j, w[j]);
+                }
+            }
+        } else if (P.forcesFixed) {
+            unsigned long long* dst = D.wrenchFx + (size_t)lk * 6;
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const long long t = wave_total(mine ? wrench_to_fixed(w[j]) : 0ll);
+                if (__lane_id() == lead) atomicAdd(dst + j, (unsigned long long)t);
+            }
+        } else {
+            float* dst = D.wrench + (size_t)lk * 6;
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const float t = wave_total(mine ? w[j] : 0.f);
+                if (__lane_id() == lead) atomicAdd(dst + j, t);
+            }
+        }
+        has = has && !mine;
+        pend &= ~grp;
+    }
+}
+
+// Force readout at the end of a substep (YB/sim/contact.cuh:172-204): the surface law re-evaluated at the
+// final displacement gives the force on the vertex (returned) and the wrench (-f, r x -f) the vertex applies
+// to each mesh about the mesh origin, r = (x - P.pos) + dx - R n (contact.cuh:196). Wave-cooperative: every
+// lane of the wave calls it in uniform control flow, `on` false for lanes without a free vertex; the mesh loop
+// runs to the wave's largest candidate count so the wrench sums see every lane.
 template <bool F>
-__device__ __forceinline__ f3 surface_readout(const Params& P, const Grid* grids, const Pose* envPoses, float* envWrench,
+__device__ __forceinline__ f3 surface_readout(const Params& P, const Dev& D, const Pose* envPoses, int rope, bool on,
                                               int ncand, uint64_t cand, f3 p, f3 d, float k) {
     s3 H = szero();
     f3 fv = mk3(0.f, 0.f, 0.f);
-    for (int pl = 0; pl < P.numPlanes; pl++) {
-        const f3 n = mk3(P.planes[pl][0], P.planes[pl][1], P.planes[pl][2]);
-        const float gap = ((V<F>::dot(n, p) - P.planes[pl][3]) + V<F>::dot(n, d)) - P.cRadius;
-        if (gap >= 0.f) continue;
-        surface_contact<F>(P, n, gap, d, k, P.cMuS, P.cMuK, fv, H);
+    if (on) {
+        for (int pl = 0; pl < P.numPlanes; pl++) {
+            const f3 n = mk3(P.planes[pl][0], P.planes[pl][1], P.planes[pl][2]);
+            const float gap = ((V<F>::dot(n, p) - P.planes[pl][3]) + V<F>::dot(n, d)) - P.cRadius;
+            if (gap >= 0.f) continue;
+            surface_contact<F>(P, n, gap, d, k, P.cMuS, P.cMuK, fv, H);
+        }
     }
     if (P.numSdf > 0) {
-        const int cnt = ncand < 0 ? P.numSdf : ncand;
-        for (int c = 0; c < cnt; c++) {
-            const int m = ncand < 0 ? c : cand_get(cand, c);
-            f3 n, dRel;
-            float gap;
-            if (!sdf_contact<F>(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
-            f3 fm = mk3(0.f, 0.f, 0.f);
-            surface_contact<F>(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, fm, H);
-            fv = fv + fm;
-            const Pose* Pp = envPoses + m;
-            const f3 r = ((p - mk3(Pp->px, Pp->py, Pp->pz)) + d) - P.cRadius * n;
-            const f3 tq = cross(r, -fm);
-            float* w = envWrench + m * 6;
-            atomicAdd(w + 0, -fm.x);
-            atomicAdd(w + 1, -fm.y);
-            atomicAdd(w + 2, -fm.z);
-            atomicAdd(w + 3, tq.x);
-            atomicAdd(w + 4, tq.y);
-            atomicAdd(w + 5, tq.z);
+        const int cnt = on ? (ncand < 0 ? P.numSdf : ncand) : 0;
+        for (int c = 0; __any(c < cnt); c++) {
+            bool has = false;
+            int key = 0;
+            float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (c < cnt) {
+                const int m = ncand < 0 ? c : cand_get(cand, c);
+                f3 n, dRel;
+                float gap;
+                if (sdf_contact<F>(P, D.grids + m, envPoses + m, p, d, n, gap, dRel)) {
+                    f3 fm = mk3(0.f, 0.f, 0.f);
+                    surface_contact<F>(P, n, gap, dRel, k, D.grids[m].muS, D.grids[m].muK, fm, H);
+                    fv = fv + fm;
+                    const Pose* Pp = envPoses + m;
+                    const f3 r = ((p - mk3(Pp->px, Pp->py, Pp->pz)) + d) - P.cRadius * n;
+                    const f3 tq = cross(r, -fm);
+                    w[0] = -fm.x;
+                    w[1] = -fm.y;
+                    w[2] = -fm.z;
+                    w[3] = tq.x;
+                    w[4] = tq.y;
+                    w[5] = tq.z;
+                    has = true;
+                    key = rope * MAX_SDF + m;
+                }
+            }
+            wave_wrench_flush(P, D, has, key, w);
         }
     }
     return fv;

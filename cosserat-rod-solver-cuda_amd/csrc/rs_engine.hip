@@ -141,6 +141,7 @@ struct rs_batch {
     float* d_stage = nullptr;
     uint8_t* d_stageTele = nullptr;
     bool pendingSet = false, moving = false, sdfMoving = false, forcesOn = false, meshesTouched = false;
+    bool forcesFixed = false;  // readout wrenches summed as 64-bit fixed point (reproducible bit for bit)
     int lastSubsteps = 1;
     // differentiation: unit rest Darboux directions, the recorded rollout (per-frame checkpoints) and the
     // adjoint's buffers
@@ -195,7 +196,7 @@ struct rs_batch {
             hipEventDestroy(e.second);
         }
         void* ptrs[] = {D.x, D.v, D.vlast, D.q, D.q2, D.dx, D.dx2, D.prm, D.qrest, D.ropeOf, d_roff, D.pad, D.maxStep,
-                        D.ncol, D.coll, D.vlist, D.clist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.outPos,
+                        D.ncol, D.coll, D.vlist, D.clist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.wrenchFx, D.outPos,
                         D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_hits,
                         d_grids, d_cur, d_tgt, d_tele, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
                         tape.poses, tape.motion, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
@@ -432,6 +433,7 @@ struct rs_batch {
         P.numSdf = (int)grids.size();
         P.sdfMoving = sdfMoving ? 1 : 0;
         P.forcesOn = forcesOn ? 1 : 0;
+        P.forcesFixed = forcesFixed ? 1 : 0;
         P.ropeContact = ropeContact ? 1 : 0;
         P.groupRopes = groupRopes;
         P.radius = radius;
@@ -512,7 +514,11 @@ struct rs_batch {
         if (!grids.empty()) hcheck(launch_fill_i32(D.ncand, -1, nv, stream), "reset candidates");
         if (forcesOn) {
             hcheck(launch_fill_f32(D.force, 0.f, 3L * nv, stream), "zero forces");
-            hcheck(launch_fill_f32(D.wrench, 0.f, 6L * N * MAX_SDF, stream), "zero wrenches");
+            if (forcesFixed)
+                hcheck(launch_fill_i32(reinterpret_cast<int*>(D.wrenchFx), 0, 12L * N * MAX_SDF, stream),
+                       "zero wrench accumulators");
+            else
+                hcheck(launch_fill_f32(D.wrench, 0.f, 6L * N * MAX_SDF, stream), "zero wrenches");
         }
     }
     void end_step(int substeps) {  // SdfStore::end_step (csrc/sdf.cu:239-253)
@@ -526,7 +532,10 @@ struct rs_batch {
         if (forcesOn) {
             const float k = 1.f / (float)lastSubsteps;
             hcheck(launch_scale_f32(D.force, k, 3L * nv, stream), "average forces");
-            hcheck(launch_scale_f32(D.wrench, k, 6L * N * MAX_SDF, stream), "average wrenches");
+            if (forcesFixed)
+                hcheck(launch_wrench_from_fixed(D.wrenchFx, D.wrench, k, 6L * N * MAX_SDF, stream), "average wrenches");
+            else
+                hcheck(launch_scale_f32(D.wrench, k, 6L * N * MAX_SDF, stream), "average wrenches");
         }
     }
 
@@ -1265,6 +1274,25 @@ int rs_enable_contact_forces(rs_batch* b, int on) {
         b->ensure_mesh_state();
         b->forcesOn = on != 0;
     });
+}
+
+int rs_set_contact_force_mode(rs_batch* b, int fixed_point) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(b->mtx);
+        b->ensure_mesh_state();
+        if (fixed_point && !b->D.wrenchFx) {
+            b->D.wrenchFx = dalloc<unsigned long long>((size_t)b->N * MAX_SDF * 6, "wrench accumulators");
+            hcheck(hipMemset(b->D.wrenchFx, 0, (size_t)b->N * MAX_SDF * 6 * sizeof(unsigned long long)),
+                   "memset wrench accumulators");
+        }
+        b->forcesFixed = fixed_point != 0;
+    });
+}
+
+int rs_get_contact_force_mode(const rs_batch* b, int* fixed_point) {
+    if (!b || !fixed_point) return RS_INVALID;
+    *fixed_point = b->forcesFixed ? 1 : 0;
+    return RS_OK;
 }
 
 int rs_contact_forces_host(rs_batch* b, float* out) {

@@ -29,6 +29,11 @@ __global__ void k_scale_f32(float* p, float s, long n) {  // sdf.cu:89-92
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = p[i] * s;
 }
+// Fixed-point wrench sums (units of 2^-WRENCH_FIX_BITS) to float, averaged over the frame's substeps.
+__global__ void k_wrench_from_fixed(const unsigned long long* fx, float* out, float s, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)((double)(long long)fx[i] * (1.0 / 4294967296.0)) * s;
+}
 
 // Constant-velocity root drive once per frame (csrc/batch.cu:90-104).
 __global__ void k_root_const(Params P, Dev D, Drive Dr) {
@@ -213,8 +218,9 @@ __global__ void k_orient(Params P, Dev D, const float4* dxNew, const float4* qIn
 // broad phase for the next substep.
 template <bool F>
 __global__ void k_end(Params P, Dev D, const float4* dxIn, const float4* qCur) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.nv) return;
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i0 < P.nv;  // every lane stays to the readout's wave-wide wrench sums
+    const int i = valid ? i0 : P.nv - 1;
     const float4 prm = D.prm[i];
     const uint32_t fl = flags_of(prm);
     const f3 d = ld3(dxIn[i]);
@@ -227,20 +233,24 @@ __global__ void k_end(Params P, Dev D, const float4* dxIn, const float4* qCur) {
         nc = D.ncand[i];
         if (nc > 0) cand = load_cand(D, i, nc);
     }
-    if (prm.x != 0.f) {
+    const bool free = valid && prm.x != 0.f;
+    if (free) {
         f3 t = mk3(1.f, 0.f, 0.f);
         if (P.dragQuad > 0.f) t = V<F>::qrot_ex(ldq((fl & HAS_NEXT) ? qCur[i] : qCur[i - 1]));
         D.v[i] = st3(end_velocity<F>(P, d, t));
-        if (P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0)) {
-            const float kap = Ar<F>::div(P.cKappa, P.h);
-            const f3 fv = surface_readout<F>(P, D.grids, envPoses, D.wrench + (size_t)rope * MAX_SDF * 6, nc, cand, x, d,
-                                             Ar<F>::div(kap * kap, prm.x));
+    }
+    if (P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0)) {
+        const float kap = Ar<F>::div(P.cKappa, P.h);
+        const f3 fv = surface_readout<F>(P, D, envPoses, rope, free, nc, cand, x, d,
+                                         free ? Ar<F>::div(kap * kap, prm.x) : 0.f);
+        if (free) {
             float* fo = D.force + 3 * (size_t)i;
             fo[0] = fo[0] + fv.x;
             fo[1] = fo[1] + fv.y;
             fo[2] = fo[2] + fv.z;
         }
     }
+    if (!valid) return;
     const f3 xn = x + d;
     D.x[i] = st3(xn);
     if (P.numSdf > 0) {
@@ -902,7 +912,25 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             if (gi[k] >= 0) sQ[t + k * BLOCK] = stq(qo[k]);  // frames of the last orientation sweep
         __syncthreads();
 
-        // end of substep
+        // end of substep: force readout (every lane, for the wave-wide wrench sums), then velocity and commit
+        if (FORCES) {
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int sl = t + k * BLOCK;
+                const bool on = gi[k] >= 0 && prm[k].x != 0.f;
+                const float kap = Ar<F>::div(P.cKappa, h);
+                const f3 fv = surface_readout<F>(P, D, cell_poses(P, D, rope[k], s), rope[k], on, nc[k], cand[k],
+                                                 mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
+                                                 mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]),
+                                                 on ? Ar<F>::div(kap * kap, prm[k].x) : 0.f);
+                if (on) {
+                    float* fo = D.force + 3 * (size_t)gi[k];
+                    fo[0] = fo[0] + fv.x;
+                    fo[1] = fo[1] + fv.y;
+                    fo[2] = fo[2] + fv.z;
+                }
+            }
+        }
 #pragma unroll
         for (int k = 0; k < VPT; k++) {
             if (gi[k] >= 0) {
@@ -915,15 +943,6 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     f3 tg = mk3(1.f, 0.f, 0.f);
                     if (P.dragQuad > 0.f) tg = V<F>::qrot_ex((fl & HAS_NEXT) ? qo[k] : ldq(sQ[sl - 1]));
                     st3L(sVel, sl, end_velocity<F>(P, d, tg));
-                    if (FORCES) {
-                        const float kap = Ar<F>::div(P.cKappa, h);
-                        const f3 fv = surface_readout<F>(P, D.grids, envPoses, D.wrench + (size_t)rope[k] * MAX_SDF * 6,
-                                                         nc[k], cand[k], x, d, Ar<F>::div(kap * kap, prm[k].x));
-                        float* fo = D.force + 3 * (size_t)gi[k];
-                        fo[0] = fo[0] + fv.x;
-                        fo[1] = fo[1] + fv.y;
-                        fo[2] = fo[2] + fv.z;
-                    }
                 }
                 const f3 xn = x + d;
                 sX[sl] = xn.x;
@@ -984,6 +1003,11 @@ hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s) {
 hipError_t launch_fill_f32(float* p, float v, long n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_f32, dim3(nblk(n, 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+hipError_t launch_wrench_from_fixed(const unsigned long long* fx, float* out, float scale, long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_wrench_from_fixed, dim3(nblk(n, 256)), dim3(256), 0, s, fx, out, scale, n);
     return hipGetLastError();
 }
 hipError_t launch_scale_f32(float* p, float v, long n, hipStream_t s) {

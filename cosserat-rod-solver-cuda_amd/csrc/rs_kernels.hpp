@@ -17,6 +17,7 @@ size_t cell_lds_bytes(int block, int vpt);
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, long n, hipStream_t s);
 hipError_t launch_scale_f32(float* p, float v, long n, hipStream_t s);
+hipError_t launch_wrench_from_fixed(const unsigned long long* fx, float* out, float scale, long n, hipStream_t s);
 hipError_t launch_root_const(const Params& P, const Dev& D, const Drive& Dr, hipStream_t s);
 hipError_t launch_gather(const Params& P, const Dev& D, hipStream_t s);
 hipError_t launch_reset(const Params& P, const Dev& D, const float4* initX, const float4* initQ, const int* ids,

@@ -55,6 +55,7 @@ struct Params {
     float planes[8][4];
     float cRadius, cKappa, cMuS, cMuK, cVSlip;
     int numSdf, sdfMoving, forcesOn;
+    int forcesFixed;  // readout wrenches accumulated as 64-bit fixed point (order-independent sums)
     int ropeContact, groupRopes;
     float radius, barrier, kCollision, fricMu, kFriction, detR, scaledDetR;
     int fastMath;  // arithmetic of the iteration routines: 0 IEEE, 1 fused multiply-add + hardware rcp/sqrt/rsq/log
@@ -92,6 +93,7 @@ struct Dev {
     int* cand;             // [nv * MAX_CAND]
     float* force;          // [nv * 3] contact-force accumulator
     float* wrench;         // [nropes * MAX_SDF * 6]
+    unsigned long long* wrenchFx;  // [nropes * MAX_SDF * 6] fixed-point accumulator (forcesFixed)
     // outputs (gathered views) and the error word
     float* outPos;   // [nv * 3]
     float* outVel;   // [nv * 3]

@@ -137,6 +137,13 @@ int rs_num_meshes(const rs_batch* b);
 int rs_enable_contact_forces(rs_batch* b, int on);
 int rs_contact_forces_host(rs_batch* b, float* out); /* [num_vertices * 3] averaged over the last step */
 int rs_mesh_wrenches_host(rs_batch* b, float* out);  /* [num_ropes * num_meshes * 6] */
+/* Wrench summation of the readout. The reference sums with float atomics (contact.cuh:188-203, sdf.cu:89-92),
+ * whose order, and so whose last bits, change from run to run. fixed_point = 1 rounds every contribution to
+ * a 64-bit integer in units of 2^-32 N (N m) and sums exactly: the wrenches are then the same bit for bit in
+ * every run, dispatch form and cell variant. Contributions are clamped to |1e9|, and a frame's sum per
+ * component must stay below 2^31 N (N m). 0 (default) keeps float sums. No reference counterpart. */
+int rs_set_contact_force_mode(rs_batch* b, int fixed_point);
+int rs_get_contact_force_mode(const rs_batch* b, int* fixed_point);
 
 /* RopeSim.reset bookkeeping (csrc/bindings.cpp:303-316): time 0, previous substep = max_h, codes cleared. */
 int rs_reset_clock(rs_batch* b);
