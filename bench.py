@@ -14,7 +14,9 @@ baseline (the float32 oracle on the host cores, a bounded sample of the same wor
 - forward time of the per-substep kernels and of the persistent frame kernel;
 - a differentiable rollout's forward (with per-frame checkpoints) and adjoint time, and their ratio.
 The "readout" object continues the headline batch with the contact-force readout on (float wrench sums, then
-the reproducible fixed-point sums) and reports the time per frame of each against the readout off.
+the reproducible fixed-point sums) and reports the time per frame of each against the readout off, and does the
+same on ropes resting on a slab mesh. The "bake_mesh" object times ropesim.sdf.bake_mesh on a 32768-triangle
+torus.
 """
 import argparse
 import json
@@ -72,6 +74,77 @@ def cpu_baseline(sample_ropes, threads):
             "kind": "port",
             "sample": f"float32 oracle, {sample_ropes} ropes ({s.num_vertices} vertices, {sample_ropes // 4} "
                       f"environments) of the same generator, 1 frame of {s.substeps} substeps, {dt:.1f} s"}
+
+
+def resting_readout_bench(n_ropes, frames=5):
+    """Readout on ropes resting on one slab mesh (every vertex in contact in every substep): time per frame
+    with the readout off, with float wrench sums and with fixed-point sums, after two warm-up frames each."""
+    import numpy as np
+    import torch
+
+    import ropesim
+    from ropesim import scenes
+    s = scenes.resting(n_ropes=n_ropes)
+    b = s.build(ropesim)
+    b.set_dispatch(1, -1)
+    b.autotune(s.dt, s.substeps, False)
+    b.set_graph(True)
+    out = {"workload": f"{n_ropes} free ropes x 65 vertices resting on one slab SDF mesh, 100 substeps x 8 "
+                       "iterations per 1/60 s frame"}
+    for name, on, det in (("off", False, False), ("float", True, False), ("fixed_point", True, True)):
+        b.enable_contact_forces(on, deterministic=det)
+        for _ in range(2):
+            b.step_free(s.dt, s.substeps)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(frames):
+            b.step_free(s.dt, s.substeps)
+        torch.cuda.synchronize()
+        out[name + "_ms_per_frame"] = 1e3 * (time.perf_counter() - t) / frames
+    w = np.asarray(b.mesh_wrenches(), np.float64).reshape(n_ropes, 6)
+    m = 1.0 / np.asarray(b.inv_masses(), np.float64)
+    out["wrench_z_over_weight"] = float(-w[:, 2].sum() / (m.sum() * 9.81))
+    return out
+
+
+def torus_mesh(major, minor, nu, nv):
+    """Closed triangulated torus around the z axis: nu x nv quads, two triangles each."""
+    import numpy as np
+    u = 2 * np.pi * np.arange(nu) / nu
+    v = 2 * np.pi * np.arange(nv) / nv
+    U, V = np.meshgrid(u, v, indexing="ij")
+    pts = np.stack([(major + minor * np.cos(V)) * np.cos(U), (major + minor * np.cos(V)) * np.sin(U),
+                    minor * np.sin(V)], -1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = i * nv + j
+    b = ((i + 1) % nu) * nv + j
+    c = ((i + 1) % nu) * nv + (j + 1) % nv
+    d = i * nv + (j + 1) % nv
+    tri = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3), np.stack([a, c, d], -1).reshape(-1, 3)])
+    return pts, tri
+
+
+def bake_bench():
+    """Mesh distance grid on the GPU: a 32768-triangle torus on a ~96^3 lattice, signed by the winding number;
+    node-triangle pairs per second (each pair: closest point on the triangle and its solid angle)."""
+    import numpy as np
+
+    from ropesim import sdf
+    v, t = torus_mesh(0.2, 0.06, 256, 64)
+    spacing, margin = 0.005, 0.03
+    sdf.bake_mesh(v, t, spacing, margin)  # warm-up
+    t0 = time.perf_counter()
+    g = sdf.bake_mesh(v, t, spacing, margin)
+    dt = time.perf_counter() - t0
+    nodes = int(np.prod(g.values.shape))
+    ring = np.hypot(*np.meshgrid(g.origin[0] + spacing * np.arange(g.values.shape[0]),
+                                 g.origin[1] + spacing * np.arange(g.values.shape[1]), indexing="ij")) - 0.2
+    exact = np.sqrt(ring[:, :, None] ** 2 + (g.origin[2] + spacing * np.arange(g.values.shape[2]))[None, None] ** 2) - 0.06
+    return {"workload": f"closed torus mesh (R 0.2 m, r 0.06 m, {len(t)} triangles) on a "
+                        f"{'x'.join(map(str, g.values.shape))} lattice, signed by the winding number",
+            "ms": 1e3 * dt, "node_triangle_pairs_per_s": nodes * len(t) / dt,
+            "max_abs_dev_from_torus_sdf_m": float(np.abs(g.values - exact).max()),
+            "timing": "host to host: upload, kernel, download"}
 
 
 def differentiation_bench(n_ropes, frames):
@@ -248,6 +321,7 @@ def main():
     ap.add_argument("--scale-frames", type=int, default=3, help="config-5 forward frames timed (0 skips the leg)")
     ap.add_argument("--scale-adjoint-frames", type=int, default=30, help="config-5 frames of the adjoint leg")
     ap.add_argument("--no-readout", action="store_true", help="skip the contact-force readout leg")
+    ap.add_argument("--no-bake", action="store_true", help="skip the mesh-distance-grid leg")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -318,6 +392,7 @@ def main():
         b.enable_contact_forces(False)
         for name in ("float", "fixed_point"):
             readout[name + "_overhead"] = readout[name + "_ms_per_frame"] / readout["off_ms_per_frame"] - 1.0
+        readout["resting"] = resting_readout_bench(args.ropes)
 
     units = s.num_vertices * s.substeps          # vertex-substeps per frame
     value = units * K / wall
@@ -358,6 +433,8 @@ def main():
                         "flop_per_vertex_substep": fl, "note": "solver flops only; rope-rope pair tests excluded"}
     if readout is not None:
         line["readout"] = readout
+    if not args.no_bake:
+        line["bake_mesh"] = bake_bench()
     if args.diff_frames > 0:
         d = differentiation_bench(args.diff_ropes, args.diff_frames)
         line["differentiation"] = d

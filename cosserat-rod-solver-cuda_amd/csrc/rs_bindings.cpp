@@ -720,6 +720,30 @@ PYBIND11_MODULE(_ropesim, m) {
         });
 
     m.def("num_cell_variants", &rs_num_cell_variants);
+    // ropesim.sdf.bake_mesh's device pass (validation and the lattice are on the Python side, as in the reference)
+    m.def(
+        "bake_mesh_sdf",
+        [](const farray& vertices, const py::array_t<int, py::array::c_style | py::array::forcecast>& triangles,
+           const farray& origin, float spacing, std::vector<int> dims, float thickness, float round, int device) {
+            if (vertices.ndim() != 2 || vertices.shape(1) != 3) throw py::value_error("vertices must be [V, 3]");
+            if (triangles.ndim() != 2 || triangles.shape(1) != 3) throw py::value_error("triangles must be [T, 3]");
+            if (origin.size() != 3 || dims.size() != 3) throw py::value_error("origin and dims need 3 entries");
+            py::array_t<float> out({(py::ssize_t)dims[0], (py::ssize_t)dims[1], (py::ssize_t)dims[2]});
+            float* o = out.mutable_data();
+            const float* v = vertices.data();
+            const int* t = triangles.data();
+            const float* org = origin.data();
+            const int nv = (int)vertices.shape(0), nt = (int)triangles.shape(0);
+            int st;
+            {
+                py::gil_scoped_release nogil;
+                st = rs_bake_mesh_sdf(v, nv, t, nt, org, spacing, dims[0], dims[1], dims[2], thickness, round, device, o);
+            }
+            check(st);
+            return out;
+        },
+        py::arg("vertices"), py::arg("triangles"), py::arg("origin"), py::arg("spacing"), py::arg("dims"),
+        py::arg("thickness") = 0.f, py::arg("round") = 0.f, py::arg("device") = -1);
     m.def("cell_variant_info", [](int v) {
         int b = 0, t = 0, s = 0;
         check(rs_cell_variant_info(v, &b, &t, &s));

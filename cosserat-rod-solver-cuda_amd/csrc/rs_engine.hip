@@ -1772,3 +1772,56 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ mesh distance grids
+
+int rs_bake_mesh_sdf(const float* vertices, int num_vertices, const int* triangles, int num_triangles,
+                     const float* origin3, float spacing, int nx, int ny, int nz, float thickness, float round,
+                     int device, float* out) {
+    return guarded([&] {
+        if (!vertices || !triangles || !origin3 || !out) throw Invalid("bake_mesh: null pointer");
+        if (num_vertices < 3 || num_triangles < 1) throw Invalid("bake_mesh needs a triangle mesh");
+        if (!(spacing > 0.f) || nx < 2 || ny < 2 || nz < 2) throw Invalid("bake_mesh: bad lattice");
+        if (!(thickness >= 0.f) || !(round >= 0.f)) throw Invalid("bake_mesh: thickness and round must be >= 0");
+        std::vector<float4> tri(3 * (size_t)num_triangles);
+        for (int t = 0; t < num_triangles; t++)
+            for (int c = 0; c < 3; c++) {
+                const int v = triangles[3 * t + c];
+                if (v < 0 || v >= num_vertices) throw Invalid("triangle indices out of range");
+                tri[3 * (size_t)t + c] = make_float4(vertices[3 * v], vertices[3 * v + 1], vertices[3 * v + 2], 0.f);
+            }
+        int prev = -1;
+        if (device >= 0) {
+            hcheck(hipGetDevice(&prev), "get device");
+            hcheck(hipSetDevice(device), "set device");
+        }
+        const size_t nodes = (size_t)nx * ny * nz;
+        float4* dTri = nullptr;
+        float* dOut = nullptr;
+        hipStream_t st = nullptr;
+        try {
+            hcheck(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+            dTri = dalloc<float4>(tri.size(), "bake triangles");
+            dOut = dalloc<float>(nodes, "bake grid");
+            hcheck(hipMemcpyAsync(dTri, tri.data(), tri.size() * sizeof(float4), hipMemcpyHostToDevice, st),
+                   "upload triangles");
+            const double o[3] = {origin3[0], origin3[1], origin3[2]};
+            hcheck(launch_bake_mesh(dTri, num_triangles, o, (double)spacing, nx, ny, nz, thickness > 0.f ? 1 : 0,
+                                    (float)(0.5 * (double)thickness), round, dOut, st),
+                   "bake kernel");
+            hcheck(hipMemcpyAsync(out, dOut, nodes * sizeof(float), hipMemcpyDeviceToHost, st), "download grid");
+            hcheck(hipStreamSynchronize(st), "bake");
+        } catch (...) {
+            if (st) hipStreamSynchronize(st);
+            if (dTri) hipFree(dTri);
+            if (dOut) hipFree(dOut);
+            if (st) hipStreamDestroy(st);
+            if (prev >= 0) hipSetDevice(prev);
+            throw;
+        }
+        hipFree(dTri);
+        hipFree(dOut);
+        hipStreamDestroy(st);
+        if (prev >= 0) hipSetDevice(prev);
+    });
+}

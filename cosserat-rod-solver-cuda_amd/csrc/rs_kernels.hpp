@@ -31,6 +31,8 @@ hipError_t launch_stage_motion(Pose7* cur, const Pose7* tgt, uint8_t* tele, Moti
 hipError_t launch_commit(Pose7* cur, const Pose7* tgt, uint8_t* tele, Pose* poses, const Grid* grids, int numSdf,
                          int index, int N, hipStream_t s);
 hipError_t launch_check_detect(const Params& P, const Dev& D, hipStream_t s);
+hipError_t launch_bake_mesh(const float4* tri, int ntri, const double* origin, double spacing, int nx, int ny, int nz,
+                            int shell, float halfThick, float rnd, float* out, hipStream_t s);
 hipError_t launch_check_broadphase(const Params& P, const Dev& D, const float4* xs, const float4* dxs, int* counts,
                                    int* cand, hipStream_t s);
 

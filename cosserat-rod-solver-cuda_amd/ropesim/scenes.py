@@ -13,6 +13,7 @@ Configurations (sizes scale down through the arguments for quick checks):
            contact, 8 kinematic SDF meshes per environment on SE(3) random walks, ground plane
   config4  N ropes x 65 (config-1 material), plane + one static sphere SDF per rope
   config5  16384 ropes x 96 (config-1 material, per-vertex jitter), 64 meshes on 96^3 grids, groups of 4
+  resting  N free ropes x 65 lying on one slab mesh (every vertex in contact): the force-readout workload
 """
 from __future__ import annotations
 
@@ -371,3 +372,25 @@ def config5(n_ropes=16384, V=96, frames=3, seed=5, n_meshes=64, grid_n=96, group
               contact=contact, meshes=meshes, mesh_pos0=mpos[env].astype(np.float32),
               mesh_quat0=mq[env].astype(np.float32), rope_contact=rc, masses=masses, seed=seed)
     return s
+
+
+def resting(n_ropes=4096, V=65, frames=8, grid_spacing=1 / 64):
+    """Force-readout workload: n_ropes free ropes x V vertices (1 cm segments, config-1 material) lying side by
+    side on one slab mesh (a box SDF whose top face is z = 0, on a lattice of `grid_spacing`), released 0.5 mm
+    above it; h 1/6000 (100 substeps per 1/60 s frame), 8 iterations, no drag. Once settled every vertex
+    presses on the slab in every substep, so each wave's wrench contributions all go to one (rope, mesh)
+    sum or two."""
+    R = 0.004
+    pos = np.zeros((n_ropes, V, 3), np.float32)
+    pos[:, :, 0] = np.arange(V) * 0.01 - 0.005 * (V - 1)
+    pos[:, :, 1] = np.linspace(-0.5, 0.5, n_ropes)[:, None]
+    pos[:, :, 2] = R + 0.0005
+    half = 0.005 * (V - 1) + 0.1
+    g = _sdf.from_function(_sdf.sdf_box((half, 0.6, 0.05), (0, 0, -0.05)), (-half - 0.05, -0.65, -0.12),
+                           (half + 0.05, 0.65, 0.03), grid_spacing)
+    contact = dict(radius=R, mu_s=0.5, mu_k=0.4, v_slip=1e-3, kappa=2.0)
+    return Scene("resting", pos.reshape(-1, 3), (np.arange(n_ropes + 1) * V).astype(np.int32), 0.1, 500.0, 1e-4,
+                 1.0 / 6000.0, 8, 1.0 / 60.0, 100, frames, pinned=np.zeros(n_ropes * V, bool), contact=contact,
+                 meshes=[Mesh(g.values, g.origin, g.spacing, 0.5, 0.4)],
+                 mesh_pos0=np.zeros((n_ropes, 1, 3), np.float32),
+                 mesh_quat0=np.tile(np.array([0, 0, 0, 1], np.float32), (n_ropes, 1, 1)), drive="free")

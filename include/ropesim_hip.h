@@ -145,6 +145,16 @@ int rs_mesh_wrenches_host(rs_batch* b, float* out);  /* [num_ropes * num_meshes 
 int rs_set_contact_force_mode(rs_batch* b, int fixed_point);
 int rs_get_contact_force_mode(const rs_batch* b, int* fixed_point);
 
+/* Distance grid of a triangle mesh on a node lattice, computed on the GPU (reference ropesim/sdf.py:96-157,
+ * which calls Warp's mesh_query_point_sign_winding_number per node). Host arrays: vertices [num_vertices*3],
+ * triangles [num_triangles*3]; node (i, j, k) at origin + (i, j, k) * spacing (in double, then float); out
+ * [nx*ny*nz] C order. thickness > 0: |distance| - thickness / 2 (two-sided shell); else the distance signed
+ * negative where the generalized winding number exceeds 1/2 in magnitude. Then minus round. device < 0: the
+ * current device. Blocks until the grid is on the host. */
+int rs_bake_mesh_sdf(const float* vertices, int num_vertices, const int* triangles, int num_triangles,
+                     const float* origin3, float spacing, int nx, int ny, int nz, float thickness, float round,
+                     int device, float* out);
+
 /* RopeSim.reset bookkeeping (csrc/bindings.cpp:303-316): time 0, previous substep = max_h, codes cleared. */
 int rs_reset_clock(rs_batch* b);
 
