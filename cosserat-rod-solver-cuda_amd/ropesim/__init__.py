@@ -4,7 +4,7 @@
 - RopeBatch: many ropes (uneven lengths allowed) in one solver; roots driven from GPU tensors,
              positions / velocities / frames exposed zero-copy through __cuda_array_interface__.
 - CudaView:  the device-array view type.
-- ropesim.sdf: signed-distance grids (closed forms, trilinear host sampler).
+- ropesim.sdf: signed-distance grids (closed forms, triangle meshes baked on the GPU, trilinear host sampler).
 - ropesim.diff: differentiable rollouts (torch.autograd.Function over the hand-written adjoint).
 
 The solver is librope_hip.so (hand-written HIP for gfx950) behind a C ABI; _ropesim is its pybind11
