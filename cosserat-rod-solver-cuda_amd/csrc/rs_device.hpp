@@ -166,6 +166,18 @@ __device__ __forceinline__ bool sdf_contact(const Params& Pm, const Grid* gp, co
     return true;
 }
 
+// Is the world-space probe x + dx near the world box of a posed mesh's lattice? Outside the box (grown by a
+// margin far above the rounding of the box corners and of the probe) the probe is outside the lattice and
+// sdf_contact finds no contact, so skipping the mesh changes no result. Used where every mesh is visited (a
+// frame's first substep) to save the probe's rotation into the lattice frame.
+__device__ __forceinline__ bool probe_near_box(const Pose* P, f3 q) {
+    const float4* s = reinterpret_cast<const float4*>(P);
+    const float4 e = s[3], f = s[4];
+    const float mg = 1e-4f + 2e-6f * fmaxf(fabsf(q.x), fmaxf(fabsf(q.y), fabsf(q.z)));
+    return q.x >= e.y - mg && q.x <= f.x + mg && q.y >= e.z - mg && q.y <= f.y + mg && q.z >= e.w - mg &&
+           q.z <= f.z + mg;
+}
+
 // Broad-phase candidates are kept packed: 6 bits per mesh id, up to 8 ids in 64 bits.
 __device__ __forceinline__ int cand_get(uint64_t packed, int c) { return (int)((packed >> (6 * c)) & 63u); }
 
@@ -183,6 +195,7 @@ __device__ __forceinline__ void surfaces_at(const Params& P, const Grid* grids, 
         const int cnt = ncand < 0 ? P.numSdf : ncand;
         for (int c = 0; c < cnt; c++) {
             const int m = ncand < 0 ? c : cand_get(cand, c);
+            if (ncand < 0 && !probe_near_box(envPoses + m, p + d)) continue;
             f3 n, dRel;
             float gap;
             if (!sdf_contact<F>(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
@@ -318,7 +331,8 @@ __device__ __forceinline__ f3 surface_readout(const Params& P, const Dev& D, con
                 const int m = ncand < 0 ? c : cand_get(cand, c);
                 f3 n, dRel;
                 float gap;
-                if (sdf_contact<F>(P, D.grids + m, envPoses + m, p, d, n, gap, dRel)) {
+                if ((ncand >= 0 || probe_near_box(envPoses + m, p + d)) &&
+                    sdf_contact<F>(P, D.grids + m, envPoses + m, p, d, n, gap, dRel)) {
                     f3 fm = mk3(0.f, 0.f, 0.f);
                     surface_contact<F>(P, n, gap, dRel, k, D.grids[m].muS, D.grids[m].muK, fm, H);
                     fv = fv + fm;
@@ -359,6 +373,66 @@ __device__ __forceinline__ int build_candidates(const Params& P, const Pose* env
         }
     }
     return n <= MAX_CAND ? n : -1;
+}
+
+// build_candidates over the meshes of `mask` only (ascending): the same list whenever `mask` holds every mesh
+// the full scan would accept.
+__device__ __forceinline__ int build_candidates_masked(const Params& P, const Pose* envPoses, f3 p, float padv,
+                                                       uint64_t mask, uint64_t& packed) {
+    int n = 0;
+    packed = 0;
+    while (mask) {
+        const int m = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const float4* s = reinterpret_cast<const float4*>(envPoses + m);
+        const float4 e = s[3], f = s[4];
+        const f3 lo = mk3(e.y, e.z, e.w), hi = mk3(f.x, f.y, f.z);
+        const float pm = P.sdfMoving ? padv + f.w : padv;
+        if (p.x >= lo.x - pm && p.x <= hi.x + pm && p.y >= lo.y - pm && p.y <= hi.y + pm && p.z >= lo.z - pm &&
+            p.z <= hi.z + pm) {
+            if (n < MAX_CAND) packed |= (uint64_t)m << (6 * n);
+            n++;
+        }
+    }
+    return n <= MAX_CAND ? n : -1;
+}
+
+// Meshes that may accept some vertex of a chunk of 16 consecutive lanes whose vertices share one rope (and so
+// one pose array): the 16 lanes split the meshes and test each against the chunk's box of the vertices'
+// padded positions, grown by the moving-mesh padding and a margin far above the rounding of the per-vertex
+// test; the 16 partial masks are OR-ed. Lanes with act false add nothing. Every lane of the wave calls it in
+// uniform control flow.
+__device__ __forceinline__ uint64_t chunk_mesh_mask(const Params& P, const Pose* envPoses, bool act, f3 p, float padv) {
+    f3 lo = act ? mk3(p.x - padv, p.y - padv, p.z - padv) : mk3(INFINITY, INFINITY, INFINITY);
+    f3 hi = act ? mk3(p.x + padv, p.y + padv, p.z + padv) : mk3(-INFINITY, -INFINITY, -INFINITY);
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        lo = vmin(lo, mk3(__shfl_xor(lo.x, o), __shfl_xor(lo.y, o), __shfl_xor(lo.z, o)));
+        hi = vmax(hi, mk3(__shfl_xor(hi.x, o), __shfl_xor(hi.y, o), __shfl_xor(hi.z, o)));
+    }
+    uint32_t mlo = 0, mhi = 0;
+    if (lo.x <= hi.x) {  // some lane of the chunk is active
+        const float mg = 1e-4f + 2e-6f * fmaxf(fmaxf(fabsf(lo.x), fabsf(hi.x)),
+                                               fmaxf(fmaxf(fabsf(lo.y), fabsf(hi.y)), fmaxf(fabsf(lo.z), fabsf(hi.z))));
+        for (int m = __lane_id() & 15; m < P.numSdf; m += 16) {
+            const float4* s = reinterpret_cast<const float4*>(envPoses + m);
+            const float4 e = s[3], f = s[4];
+            const float bp = (P.sdfMoving ? f.w : 0.f) + mg;
+            if (lo.x - bp <= f.x && lo.y - bp <= f.y && lo.z - bp <= f.z && hi.x + bp >= e.y && hi.y + bp >= e.z &&
+                hi.z + bp >= e.w) {
+                if (m < 32)
+                    mlo |= 1u << m;
+                else
+                    mhi |= 1u << (m - 32);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        mlo |= __shfl_xor(mlo, o);
+        mhi |= __shfl_xor(mhi, o);
+    }
+    return ((uint64_t)mhi << 32) | mlo;
 }
 
 // The gravity-acceleration clamp of initItr: the fraction of gravity the last substep's acceleration

@@ -417,6 +417,9 @@ __global__ void k_root_traj(Params P, Drive Dr, int S, float4* out) {
     out[3 * (size_t)idx + 2] = stq(q);
 }
 
+// Mesh counts above which the frame kernel's broad phase splits the meshes over the lanes of a chunk.
+constexpr int CHUNK_BROADPHASE_MIN_MESHES = 16;
+
 // Pose of every (rope, mesh) slot at every substep of a frame with moving meshes (YB/sim/iteration.cu:12-33).
 __global__ void k_pose_traj(Params P, Dev D, int S, Pose* out) {
     const int M = P.numSdf;
@@ -938,7 +941,6 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 const uint32_t fl = flags_of(prm[k]);
                 const f3 d = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
                 const f3 x = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-                const Pose* envPoses = cell_poses(P, D, rope[k], s);
                 if (prm[k].x != 0.f) {
                     f3 tg = mk3(1.f, 0.f, 0.f);
                     if (P.dragQuad > 0.f) tg = V<F>::qrot_ex((fl & HAS_NEXT) ? qo[k] : ldq(sQ[sl - 1]));
@@ -948,12 +950,44 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 sX[sl] = xn.x;
                 sX[CAP + sl] = xn.y;
                 sX[2 * CAP + sl] = xn.z;
-                if (P.numSdf > 0) {
-                    if (prm[k].x != 0.f)
-                        nc[k] = build_candidates(P, envPoses, xn, 2.f * len(d) + 0.02f, cand[k]);
-                    else
-                        nc[k] = 0;
+            }
+        }
+        // mesh broad phase for the next substep (YB/sim/contact.cuh:121-136). With many meshes, chunks of 16 lanes
+        // on one rope split the meshes over their lanes first, then each vertex runs the exact test on the chunk's
+        // survivors only; with a few, each vertex tests them all.
+        if (P.numSdf > 0 && P.numSdf <= CHUNK_BROADPHASE_MIN_MESHES) {
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int sl = t + k * BLOCK;
+                if (gi[k] >= 0)
+                    nc[k] = prm[k].x != 0.f ? build_candidates(P, cell_poses(P, D, rope[k], s),
+                                                               mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
+                                                               2.f * len(mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl])) +
+                                                                   0.02f,
+                                                               cand[k])
+                                            : 0;
+            }
+        } else if (P.numSdf > 0) {
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int sl = t + k * BLOCK;
+                const bool act = gi[k] >= 0 && prm[k].x != 0.f;
+                const f3 xn = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+                const float padv = 2.f * len(mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl])) + 0.02f;
+                int r0 = act ? rope[k] : 0x7fffffff, r1 = act ? rope[k] : -1;
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) {
+                    r0 = min(r0, __shfl_xor(r0, o));
+                    r1 = max(r1, __shfl_xor(r1, o));
                 }
+                const bool oneRope = r0 == r1;  // false also for a chunk without free vertices
+                const Pose* chunkPoses = cell_poses(P, D, oneRope ? r0 : 0, s);
+                const uint64_t mask = chunk_mesh_mask(P, chunkPoses, act && oneRope, xn, padv);
+                if (act)
+                    nc[k] = oneRope ? build_candidates_masked(P, chunkPoses, xn, padv, mask, cand[k])
+                                    : build_candidates(P, cell_poses(P, D, rope[k], s), xn, padv, cand[k]);
+                else if (gi[k] >= 0)
+                    nc[k] = 0;
             }
         }
         __syncthreads();
