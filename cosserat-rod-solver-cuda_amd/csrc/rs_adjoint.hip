@@ -114,6 +114,7 @@ __device__ __forceinline__ float surfaces_adj(const Params& P, const Grid* grids
         const int cnt = ncand < 0 ? P.numSdf : ncand;
         for (int c = 0; c < cnt; c++) {
             const int m = ncand < 0 ? c : cand_get(cand, c);
+            if (ncand < 0 && !probe_near_box(envPoses + m, p + d)) continue;  // as in surfaces_at
             const float4* s = reinterpret_cast<const float4*>(envPoses + m);
             const float4 a = s[0], b = s[1];
             const q4 q{a.w, b.x, b.y, b.z};
@@ -302,6 +303,7 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
             // the ring slots written here were last read two barriers ago
         }
         __syncthreads();
+        float padv = 0.f;
         if (act) {
             const f3 d = ld3s(sD[I & 1], B, t);
             q = ldq(sQ[I & 1][t]);
@@ -311,12 +313,13 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
                 v = end_velocity<F>(P, d, tg);
             }
             x = x + d;
-            if (P.numSdf > 0) {
-                if (c.w != 0.f)
-                    nc = build_candidates(P, sPose, x, 2.f * len(d) + 0.02f, cand);
-                else
-                    nc = 0;
-            }
+            padv = 2.f * len(d) + 0.02f;
+        }
+        if (P.numSdf > CHUNK_BROADPHASE_MIN_MESHES) {  // the frame kernel's chunk broad phase (one rope per block)
+            const uint64_t mask = chunk_mesh_mask(P, sPose, act && c.w != 0.f, x, padv);
+            if (act) nc = c.w != 0.f ? build_candidates_masked(P, sPose, x, padv, mask, cand) : 0;
+        } else if (act && P.numSdf > 0) {
+            nc = c.w != 0.f ? build_candidates(P, sPose, x, padv, cand) : 0;
         }
     }
     if (act && A.nx) {
@@ -397,26 +400,52 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
         }
         __syncthreads();  // sPose of a moving frame is in place
 
-        for (int k = I - 1; k >= 0; k--) {
-            // history of iteration k
-            f3 dn0 = mk3(0.f, 0.f, 0.f), dn1 = dn0, d0 = dn0, d1 = dn0;
-            q4 q0{0.f, 0.f, 0.f, 1.f}, qp = q0, qn = q0;
-            s3 H = szero();
-            f3 dl = mk3(0.f, 0.f, 0.f);
+        // History of iteration k: d^{k+1} of vertex t and t+1 (the previous iteration's d^k, already loaded),
+        // d^k, the frames q^k of t-1..t+1, the Hessian and the Newton direction. Iteration k-1's loads are
+        // issued at the top of iteration k and land while it computes (the barriers wait only for LDS).
+        struct IterHist {
+            f3 dn0, dn1, d0, d1;
+            q4 q0, qp, qn;
+            s3 H;
+            f3 dl;
+        };
+        auto load_iter = [&](int k, f3 dn0, f3 dn1) {
+            IterHist h;
+            h.dn0 = dn0;
+            h.dn1 = dn1;
+            h.d0 = h.d1 = mk3(0.f, 0.f, 0.f);
+            h.q0 = q4{0.f, 0.f, 0.f, 1.f};
+            h.qp = h.qn = h.q0;
+            h.H = szero();
+            h.dl = mk3(0.f, 0.f, 0.f);
             if (act) {
-                dn0 = ld3(A.hD[hk(A, I, s, k + 1, li)]);
-                d0 = ld3(A.hD[hk(A, I, s, k, li)]);
-                q0 = ldq(A.hQ[hk(A, I, s, k, li)]);
-                if (hn) {
-                    dn1 = ld3(A.hD[hk(A, I, s, k + 1, li + 1)]);
-                    d1 = ld3(A.hD[hk(A, I, s, k, li + 1)]);
-                }
-                qp = hp ? ldq(A.hQ[hk(A, I, s, k, li - 1)]) : q0;
-                qn = (c.fl & HAS_NEXT_ORIENT) ? ldq(A.hQ[hk(A, I, s, k, li + 1)]) : q0;
+                h.d0 = ld3(A.hD[hk(A, I, s, k, li)]);
+                h.q0 = ldq(A.hQ[hk(A, I, s, k, li)]);
+                if (hn) h.d1 = ld3(A.hD[hk(A, I, s, k, li + 1)]);
+                h.qp = hp ? ldq(A.hQ[hk(A, I, s, k, li - 1)]) : h.q0;
+                h.qn = (c.fl & HAS_NEXT_ORIENT) ? ldq(A.hQ[hk(A, I, s, k, li + 1)]) : h.q0;
 #pragma unroll
-                for (int m = 0; m < 6; m++) H.m[m] = A.hH[hh(A, I, s, k, m, li)];
-                dl = mk3(A.hH[hh(A, I, s, k, 6, li)], A.hH[hh(A, I, s, k, 7, li)], A.hH[hh(A, I, s, k, 8, li)]);
+                for (int m = 0; m < 6; m++) h.H.m[m] = A.hH[hh(A, I, s, k, m, li)];
+                h.dl = mk3(A.hH[hh(A, I, s, k, 6, li)], A.hH[hh(A, I, s, k, 7, li)], A.hH[hh(A, I, s, k, 8, li)]);
             }
+            return h;
+        };
+        IterHist cur;
+        {
+            f3 dI0 = mk3(0.f, 0.f, 0.f), dI1 = dI0;
+            if (act) {
+                dI0 = ld3(A.hD[hk(A, I, s, I, li)]);
+                if (hn) dI1 = ld3(A.hD[hk(A, I, s, I, li + 1)]);
+            }
+            cur = load_iter(I - 1, dI0, dI1);
+        }
+        for (int k = I - 1; k >= 0; k--) {
+            IterHist nxt = cur;
+            if (k > 0) nxt = load_iter(k - 1, cur.d0, cur.d1);
+            const f3 dn0 = cur.dn0, dn1 = cur.dn1, d0 = cur.d0, d1 = cur.d1;
+            const q4 q0 = cur.q0, qp = cur.qp, qn = cur.qn;
+            const s3 H = cur.H;
+            const f3 dl = cur.dl;
             // -- orientation sweep k: q^{k+1}_t from d^{k+1} and q^k_{t-1..t+1} (plus the previous
             //    iteration's segment-term pushes)
             q4 keepQ = aQ;
@@ -541,6 +570,7 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                 st3s(ex + (EX_P + 3) * B, B, t, pX);
             }
             __syncthreads();
+            cur = nxt;
         }
         if (act && hp) {  // segment-term pushes of iteration 0
             aD = aD + ld3s(ex + EX_P * B, B, t - 1);

@@ -375,6 +375,9 @@ __device__ __forceinline__ int build_candidates(const Params& P, const Pose* env
     return n <= MAX_CAND ? n : -1;
 }
 
+// Mesh counts above which the frame kernels' broad phase splits the meshes over the lanes of a chunk.
+constexpr int CHUNK_BROADPHASE_MIN_MESHES = 16;
+
 // build_candidates over the meshes of `mask` only (ascending): the same list whenever `mask` holds every mesh
 // the full scan would accept.
 __device__ __forceinline__ int build_candidates_masked(const Params& P, const Pose* envPoses, f3 p, float padv,

@@ -417,9 +417,6 @@ __global__ void k_root_traj(Params P, Drive Dr, int S, float4* out) {
     out[3 * (size_t)idx + 2] = stq(q);
 }
 
-// Mesh counts above which the frame kernel's broad phase splits the meshes over the lanes of a chunk.
-constexpr int CHUNK_BROADPHASE_MIN_MESHES = 16;
-
 // Pose of every (rope, mesh) slot at every substep of a frame with moving meshes (YB/sim/iteration.cu:12-33).
 __global__ void k_pose_traj(Params P, Dev D, int S, Pose* out) {
     const int M = P.numSdf;
