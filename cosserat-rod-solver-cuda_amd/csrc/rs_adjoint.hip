@@ -186,19 +186,23 @@ __device__ __forceinline__ Vc load_vc(const Dev& D, int i, bool act) {
 // substep's inputs and every iteration's d^k, q^k, total H and Newton direction δ = H^-1 F go to the
 // history. The frame-end state is compared bit for bit with the next checkpoint.
 template <int B, bool F>
-__global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A, Drive Dr, int S) {
+__global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDev A, Drive Dr, int S) {
     __shared__ float4 sQ[2][B];
     __shared__ float sX[3 * B], sD[2][3 * B], sF2[9 * B];
-    __shared__ Pose sPose[MAX_SDF];
+    __shared__ Pose sPoseAll[adjoint_pose_slots(B)];
     const int I = P.numItr;
-    const int r = A.rope0 + blockIdx.x, t = threadIdx.x;
-    const int v0 = D.roff[r], n = D.roff[r + 1] - v0;
+    const int rb0 = A.blk[blockIdx.x], rb1 = A.blk[blockIdx.x + 1], t = threadIdx.x;
+    const int v0 = D.roff[rb0], n = D.roff[rb1] - v0;
     const bool act = t < n;
     const int i = v0 + t, li = i - A.hv0;
+    const int r = act ? D.ropeOf[i] : rb0;  // the vertex's rope; consecutive ropes share the workgroup
+    const bool root = act && i == D.roff[r];
+    const Pose* sPose = sPoseAll + (size_t)(r - rb0) * P.numSdf;
+    const int nPose = (rb1 - rb0) * P.numSdf;
     const float h = P.h;
     const Vc c = load_vc(D, i, act);
     if (P.numSdf > 0 && !P.sdfMoving)
-        for (int m = t; m < P.numSdf; m += B) sPose[m] = A.fposes[(size_t)r * P.numSdf + m];
+        for (int m = t; m < nPose; m += B) sPoseAll[m] = A.fposes[(size_t)rb0 * P.numSdf + m];
 
     f3 x = mk3(0.f, 0.f, 0.f), v = x, vl = x;
     q4 q{0.f, 0.f, 0.f, 1.f};
@@ -207,7 +211,7 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
         v = ld3(A.cv[i]);
         vl = ld3(A.cvl[i]);
         q = ldq(A.cq[i]);
-        if (Dr.mode == DRIVE_CONST && t == 0) {
+        if (Dr.mode == DRIVE_CONST && root) {
             x = mk3(Dr.p0[3 * r], Dr.p0[3 * r + 1], Dr.p0[3 * r + 2]);
             v = mk3(Dr.v0[3 * r], Dr.v0[3 * r + 1], Dr.v0[3 * r + 2]);
             if (Dr.hasQ0) q = qunit_or_identity(q4{Dr.q0[4 * r], Dr.q0[4 * r + 1], Dr.q0[4 * r + 2], Dr.q0[4 * r + 3]});
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
     int nc = P.numSdf > 0 ? -1 : 0;
     uint64_t cand = 0;
     for (int s = 0; s < S; s++) {
-        if (Dr.mode == DRIVE_PROGRAM && act && t == 0) {
+        if (Dr.mode == DRIVE_PROGRAM && root) {
             f3 xr, vr;
             q4 qr;
             bool setQ;
@@ -227,8 +231,8 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
         }
         __syncthreads();  // readers of the previous substep are done with sX / sD / sQ / sPose
         if (P.numSdf > 0 && P.sdfMoving)
-            for (int m = t; m < P.numSdf; m += B)
-                sPose[m] = substep_pose(A.fmotion[(size_t)r * P.numSdf + m], D.grids[m], s, S, h);
+            for (int m = t; m < nPose; m += B)
+                sPoseAll[m] = substep_pose(A.fmotion[(size_t)rb0 * P.numSdf + m], D.grids[m % P.numSdf], s, S, h);
         f3 y = mk3(0.f, 0.f, 0.f);
         if (act) {
             f3 d;
@@ -315,9 +319,21 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
             x = x + d;
             padv = 2.f * len(d) + 0.02f;
         }
-        if (P.numSdf > CHUNK_BROADPHASE_MIN_MESHES) {  // the frame kernel's chunk broad phase (one rope per block)
-            const uint64_t mask = chunk_mesh_mask(P, sPose, act && c.w != 0.f, x, padv);
-            if (act) nc = c.w != 0.f ? build_candidates_masked(P, sPose, x, padv, mask, cand) : 0;
+        if (P.numSdf > CHUNK_BROADPHASE_MIN_MESHES) {  // the frame kernel's chunk broad phase
+            const bool fv = act && c.w != 0.f;
+            int q0 = fv ? r : 0x7fffffff, q1 = fv ? r : -1;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                q0 = min(q0, __shfl_xor(q0, o));
+                q1 = max(q1, __shfl_xor(q1, o));
+            }
+            const bool oneRope = q0 == q1;
+            const Pose* cp = sPoseAll + (size_t)((oneRope ? q0 : rb0) - rb0) * P.numSdf;
+            const uint64_t mask = chunk_mesh_mask(P, cp, fv && oneRope, x, padv);
+            if (act)
+                nc = !fv ? 0
+                         : oneRope ? build_candidates_masked(P, sPose, x, padv, mask, cand)
+                                   : build_candidates(P, sPose, x, padv, cand);
         } else if (act && P.numSdf > 0) {
             nc = c.w != 0.f ? build_candidates(P, sPose, x, padv, cand) : 0;
         }
@@ -337,16 +353,20 @@ __global__ __launch_bounds__(B) void k_adjoint_forward(Params P, Dev D, AdjDev A
 template <int B, bool F>
 __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A, Drive Dr, int S) {
     __shared__ float ex[EX_N * B];
-    __shared__ Pose sPose[MAX_SDF];
+    __shared__ Pose sPoseAll[adjoint_pose_slots(B)];
     const int I = P.numItr;
-    const int r = A.rope0 + blockIdx.x, t = threadIdx.x;
-    const int v0 = D.roff[r], n = D.roff[r + 1] - v0;
+    const int rb0 = A.blk[blockIdx.x], rb1 = A.blk[blockIdx.x + 1], t = threadIdx.x;
+    const int v0 = D.roff[rb0], n = D.roff[rb1] - v0;
     const bool act = t < n;
     const int i = v0 + t, li = i - A.hv0;
+    const int r = act ? D.ropeOf[i] : rb0;  // the vertex's rope; consecutive ropes share the workgroup
+    const bool root = act && i == D.roff[r];
+    const Pose* sPose = sPoseAll + (size_t)(r - rb0) * P.numSdf;
+    const int nPose = (rb1 - rb0) * P.numSdf;
     const float h = P.h;
     const Vc c = load_vc(D, i, act);
     if (P.numSdf > 0 && !P.sdfMoving)
-        for (int m = t; m < P.numSdf; m += B) sPose[m] = A.fposes[(size_t)r * P.numSdf + m];
+        for (int m = t; m < nPose; m += B) sPoseAll[m] = A.fposes[(size_t)rb0 * P.numSdf + m];
 
     f3 ax = mk3(0.f, 0.f, 0.f), av = ax, avl = ax;
     q4 aq{0.f, 0.f, 0.f, 0.f};
@@ -373,8 +393,8 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
     for (int s = S - 1; s >= 0; s--) {
         __syncthreads();  // sPose and the exchange are free
         if (P.numSdf > 0 && P.sdfMoving)
-            for (int m = t; m < P.numSdf; m += B)
-                sPose[m] = substep_pose(A.fmotion[(size_t)r * P.numSdf + m], D.grids[m], s, S, h);
+            for (int m = t; m < nPose; m += B)
+                sPoseAll[m] = substep_pose(A.fmotion[(size_t)rb0 * P.numSdf + m], D.grids[m % P.numSdf], s, S, h);
         f3 X0 = mk3(0.f, 0.f, 0.f), X1 = X0, y = X0, vs = X0, vls = X0;
         int nc = 0;
         uint64_t cand = 0;
@@ -599,7 +619,7 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             av = vb;
             avl = vlb;
             aq = aQ;
-            if (Dr.mode == DRIVE_PROGRAM && t == 0) {  // applyRootProgramK (csrc/batch.cu:122-159)
+            if (Dr.mode == DRIVE_PROGRAM && root) {  // applyRootProgramK (csrc/batch.cu:122-159)
                 const float tau = float(s) * h, dt = Dr.dt;
                 const float c2 = 2.f / (dt * dt);
                 const float cx1 = 0.5f * tau * tau * c2, cv1 = (tau + 0.5f * h) * c2;
@@ -612,7 +632,7 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             }
         }
     }
-    if (act && Dr.mode == DRIVE_CONST && t == 0) {  // applyRootsK (csrc/batch.cu:90-104)
+    if (Dr.mode == DRIVE_CONST && root) {  // applyRootsK (csrc/batch.cu:90-104)
         gp0 = gp0 + ax;
         gv0 = gv0 + av;
         ax = mk3(0.f, 0.f, 0.f);
@@ -630,7 +650,7 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
         A.grho[4 * (size_t)i + 2] += grho[2];
         A.grho[4 * (size_t)i + 3] += grho[3];
         A.gm[i] += gm;
-        if (t == 0) {
+        if (root) {
             if (A.gp0) {
                 A.gp0[3 * r] = gp0.x;
                 A.gp0[3 * r + 1] = gp0.y;
@@ -648,14 +668,14 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             }
         }
     }
-    // drag adjoint of the rope, summed in a fixed order
+    // drag adjoint of each rope, summed over its vertices in order by its root
     __syncthreads();
     double* red = reinterpret_cast<double*>(ex);
     red[t] = act ? gdrag : 0.0;
     __syncthreads();
-    if (t == 0) {
+    if (root) {
         double sum = 0.0;
-        for (int j = 0; j < n; j++) sum += red[j];
+        for (int j = t; j < D.roff[r + 1] - v0; j++) sum += red[j];
         A.gdrag[r] += sum;
     }
 }
@@ -684,34 +704,32 @@ size_t adjoint_history_bytes(int numItr) {
            (size_t)numItr * 9 * sizeof(float);
 }
 
-int adjoint_block_for(int maxRopeVerts) {
-    for (int b = 64; b <= ADJ_MAX_BLOCK; b *= 2)
-        if (maxRopeVerts <= b) return b;
-    return -1;
-}
+bool adjoint_block_supported(int block) { return block == 64 || block == 128 || block == 192 || block == 256; }
 
 template <int B, bool F>
-static hipError_t launch_frame_f(const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S, int nropes,
+static hipError_t launch_frame_f(const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S, int nblocks,
                                  hipStream_t s) {
-    hipLaunchKernelGGL((k_adjoint_forward<B, F>), dim3(nropes), dim3(B), 0, s, P, D, A, Dr, S);
+    hipLaunchKernelGGL((k_adjoint_forward<B, F>), dim3(nblocks), dim3(B), 0, s, P, D, A, Dr, S);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_adjoint_reverse<B, F>), dim3(nropes), dim3(B), 0, s, P, D, A, Dr, S);
+    hipLaunchKernelGGL((k_adjoint_reverse<B, F>), dim3(nblocks), dim3(B), 0, s, P, D, A, Dr, S);
     return hipGetLastError();
 }
 template <int B>
-static hipError_t launch_frame_t(const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S, int nropes,
+static hipError_t launch_frame_t(const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S, int nblocks,
                                  hipStream_t s) {
-    if (P.fastMath) return launch_frame_f<B, true>(P, D, A, Dr, S, nropes, s);
-    return launch_frame_f<B, false>(P, D, A, Dr, S, nropes, s);
+    if (P.fastMath) return launch_frame_f<B, true>(P, D, A, Dr, S, nblocks, s);
+    return launch_frame_f<B, false>(P, D, A, Dr, S, nblocks, s);
 }
 
 hipError_t launch_adjoint_frame(int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
-                                int nropes, hipStream_t s) {
+                                int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
     switch (block) {
-        case 64: return launch_frame_t<64>(P, D, A, Dr, S, nropes, s);
-        case 128: return launch_frame_t<128>(P, D, A, Dr, S, nropes, s);
-        case 256: return launch_frame_t<256>(P, D, A, Dr, S, nropes, s);
+        case 64: return launch_frame_t<64>(P, D, A, Dr, S, nblocks, s);
+        case 128: return launch_frame_t<128>(P, D, A, Dr, S, nblocks, s);
+        case 192: return launch_frame_t<192>(P, D, A, Dr, S, nblocks, s);
+        case 256: return launch_frame_t<256>(P, D, A, Dr, S, nblocks, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -19,8 +19,11 @@ struct AdjDev {
     // frame checkpoint: state at the start of the frame [nv], and at the start of the next one (recompute check)
     const float4 *cx, *cv, *cvl, *cq;
     const float4 *nx, *nv, *nvl, *nq;
-    // history of the frame for the ropes [rope0, rope0 + gridDim.x) of a launch, local vertex i - hv0 of hn
+    // history of the frame for the vertices [hv0, hv0 + hn) of a launch, local vertex i - hv0
     int rope0, hv0, hn;
+    // workgroup b of a launch advances the consecutive ropes [blk[b], blk[b + 1]) (at most ADJ_MAX_BLOCK vertices
+    // and adjoint_pose_slots(block) / numSdf ropes in all)
+    const int* blk;
     float4 *hX, *hV, *hVL, *hY;  // [S][hn] substep inputs: x after the root program (w: broad-phase count bits),
                                  // v after the root program, last velocity before init, inertial target y
     uint2* hC;                   // [S][hn] packed broad-phase candidate ids
@@ -41,13 +44,15 @@ struct AdjDev {
     const Motion* fmotion;  // frame motion [N][numSdf] while meshes move, else null
 };
 
-// Largest workgroup (= longest rope) the backward kernels support; history bytes per vertex-substep.
+// Largest workgroup (= longest rope) the backward kernels support; pose slots a workgroup keeps in LDS (its ropes
+// times the meshes); history bytes per vertex-substep.
 constexpr int ADJ_MAX_BLOCK = 256;
+constexpr int adjoint_pose_slots(int block) { return block > MAX_SDF ? block : MAX_SDF; }
 size_t adjoint_history_bytes(int numItr);
-int adjoint_block_for(int maxRopeVerts);  // 64 / 128 / 256, or -1
-// Both passes for the ropes [A.rope0, A.rope0 + nropes).
+bool adjoint_block_supported(int block);  // 64, 128, 192, 256
+// Both passes over `nblocks` workgroups of `block` threads, workgroup b on the ropes [A.blk[b], A.blk[b + 1]).
 hipError_t launch_adjoint_frame(int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
-                                int nropes, hipStream_t s);
+                                int nblocks, hipStream_t s);
 hipError_t launch_adjoint_finish(const Params& P, const Dev& D, const float4* rhoUnit, const double* gk,
                                  const double* grho, const double* gm, float* gea, float* gei, float* gmass,
                                  hipStream_t s);

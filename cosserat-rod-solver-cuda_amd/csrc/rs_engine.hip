@@ -187,6 +187,8 @@ struct rs_batch {
     float* d_gp = nullptr;
     double* d_gdrag = nullptr;
     int* d_mismatch = nullptr;
+    int* d_blk = nullptr;  // the adjoint's workgroup -> rope ranges
+    size_t blkCap = 0;
 
     ~rs_batch() {
         if (stream) hipStreamSynchronize(stream);
@@ -200,7 +202,7 @@ struct rs_batch {
                         D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_hits,
                         d_grids, d_cur, d_tgt, d_tele, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
                         tape.poses, tape.motion, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
-                        d_gdrag, d_mismatch};
+                        d_gdrag, d_mismatch, d_blk};
         for (void* p : ptrs)
             if (p) hipFree(p);
         for (float* g : gridMem) hipFree(g);
@@ -776,7 +778,7 @@ struct rs_batch {
         if ((int)grids.size() != tape.numSdf) throw Invalid("meshes were added or cleared while recording");
         int maxV = 0;
         for (int r = 0; r < N; r++) maxV = std::max(maxV, roff[r + 1] - roff[r]);
-        if (adjoint_block_for(maxV) < 0) throw Invalid("the adjoint supports ropes of at most 256 vertices");
+        if (maxV > ADJ_MAX_BLOCK) throw Invalid("the adjoint supports ropes of at most 256 vertices");
     }
     void copy_state(float4* dst) {
         const size_t n = nv;
@@ -1658,7 +1660,47 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
         const size_t n = b->nv;
         int maxV = 0;
         for (int r = 0; r < N; r++) maxV = std::max(maxV, b->roff[r + 1] - b->roff[r]);
-        const int blk = adjoint_block_for(maxV);
+        // Workgroups of the adjoint: consecutive ropes packed up to `blk` vertices (and the LDS pose slots), with
+        // the size among 64..256 that issues the fewest waves.
+        const int M = T.numSdf;
+        auto pack = [&](int B, int ropesPerBlock, int r0, int r1, std::vector<int>* out) {
+            long waves = 0;
+            int r = r0;
+            while (r < r1) {
+                const int start = r;
+                int nvb = 0;
+                while (r < r1 && r - start < ropesPerBlock && nvb + (b->roff[r + 1] - b->roff[r]) <= B) {
+                    nvb += b->roff[r + 1] - b->roff[r];
+                    r++;
+                }
+                if (out) out->push_back(start);
+                waves += (nvb + 63) / 64;
+            }
+            if (out) out->push_back(r1);
+            return waves;
+        };
+        // One rope per workgroup of the smallest size that holds the longest rope, unless packing consecutive
+        // ropes issues fewer waves and still leaves every CU several workgroups (a small batch needs the
+        // parallelism more than the lanes).
+        int blk = -1, perBlock = 1;
+        long bestWaves = 0;
+        for (int B : {64, 128, 192, 256}) {
+            if (B < maxV || !adjoint_block_supported(B)) continue;
+            if (blk < 0) {
+                blk = B;
+                bestWaves = pack(B, 1, 0, N, nullptr);
+                continue;
+            }
+            std::vector<int> t;
+            const int maxRopes = M > 0 ? adjoint_pose_slots(B) / M : B;
+            const long w = pack(B, maxRopes, 0, N, &t);
+            if (w < bestWaves && (long)t.size() - 1 >= 8L * device_cus()) {
+                blk = B;
+                perBlock = maxRopes;
+                bestWaves = w;
+            }
+        }
+        if (blk < 0) throw Invalid("the adjoint supports ropes of at most 256 vertices");
         // the history of a frame is kept for a chunk of ropes at a time, within a fixed budget
         int maxI = 1;
         for (const Params& P : T.P) maxI = std::max(maxI, P.numItr);
@@ -1680,6 +1722,22 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
             }
             chunks.push_back(c);
         }
+        std::vector<int> blkTable, chunkBlk0, chunkBlks;  // per chunk: offset into the table, workgroups
+        for (const Chunk& c : chunks) {
+            std::vector<int> t;
+            pack(blk, perBlock, c.rope0, c.rope0 + c.nropes, &t);
+            chunkBlk0.push_back((int)blkTable.size());
+            chunkBlks.push_back((int)t.size() - 1);
+            blkTable.insert(blkTable.end(), t.begin(), t.end());
+        }
+        if (blkTable.size() > b->blkCap) {
+            if (b->d_blk) hipFree(b->d_blk);
+            b->d_blk = dalloc<int>(blkTable.size(), "adjoint workgroups");
+            b->blkCap = blkTable.size();
+        }
+        hcheck(hipMemcpyAsync(b->d_blk, blkTable.data(), blkTable.size() * sizeof(int), hipMemcpyHostToDevice,
+                              b->stream),
+               "adjoint workgroups");
         size_t need = 0;
         for (const Chunk& c : chunks) need = std::max(need, (size_t)c.nv * perV);
         if (need > b->histBytes) {
@@ -1714,8 +1772,10 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
             dr.p1 = prog + 6 * N;
             dr.q0 = prog + 9 * N;
             dr.q1 = prog + 13 * N;
-            for (const Chunk& c : chunks) {
+            for (size_t ci = 0; ci < chunks.size(); ci++) {
+                const Chunk& c = chunks[ci];
                 AdjDev A{};
+                A.blk = b->d_blk + chunkBlk0[ci];
                 A.cx = ck;
                 A.cv = ck + n;
                 A.cvl = ck + 2 * n;
@@ -1750,10 +1810,9 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
                 A.gv0 = g_v0 ? g_v0 + (size_t)f * 3 * N : b->d_gp + 3 * N;
                 A.gp1 = g_p1 ? g_p1 + (size_t)f * 3 * N : b->d_gp + 6 * N;
                 A.mismatch = b->d_mismatch;
-                const int M = T.numSdf;
                 A.fposes = M > 0 ? T.poses + (size_t)f * N * M : nullptr;
                 A.fmotion = (M > 0 && T.P[f].sdfMoving) ? T.motion + (size_t)f * N * M : nullptr;
-                hcheck(launch_adjoint_frame(blk, T.P[f], b->D, A, dr, S, c.nropes, st), "adjoint frame");
+                hcheck(launch_adjoint_frame(blk, T.P[f], b->D, A, dr, S, chunkBlks[ci], st), "adjoint frame");
             }
         }
         hcheck(launch_adjoint_finish(b->params(), b->D, b->d_rhoUnit, b->d_gk, b->d_grho, b->d_gm, g_ea, g_ei, g_mass,
