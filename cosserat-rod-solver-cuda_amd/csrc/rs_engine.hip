@@ -1689,10 +1689,10 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
             if (blk < 0) {
                 blk = B;
                 bestWaves = pack(B, 1, 0, N, nullptr);
-                continue;
             }
-            std::vector<int> t;
             const int maxRopes = M > 0 ? adjoint_pose_slots(B) / M : B;
+            if (maxRopes < 2) continue;
+            std::vector<int> t;
             const long w = pack(B, maxRopes, 0, N, &t);
             if (w < bestWaves && (long)t.size() - 1 >= 8L * device_cus()) {
                 blk = B;

@@ -209,3 +209,51 @@ def test_rollout_function_gradcheck(rs, fast):
     inputs = (z(nv), z(nv), z(nv), z(()), z(*p1.shape))
     assert torch.autograd.gradcheck(fn, inputs, eps=1e-2, atol=2e-4, rtol=2e-2, check_undefined_grad=False,
                                     check_batched_grad=False, nondet_tol=0.0)
+
+
+def _short_rope_batch(rs, n, F=2, V=20):
+    """n ropes of V vertices hanging from roots 2 cm apart onto a plane and one sphere mesh per rope, driven by
+    the root program; material, program and the loss weights derive from the rope index only, so the first
+    ropes of a large batch are exactly the ropes of a small one."""
+    import torch
+    from ropesim import sdf
+    idx = np.arange(n)
+    roots = np.column_stack([0.02 * (idx % 64), 0.02 * (idx // 64), np.full(n, 0.12)])
+    d = np.column_stack([0.3 * np.sin(idx), 0.3 * np.cos(idx), -np.ones(n)])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos = (roots[:, None, :] + 0.01 * np.arange(V)[None, :, None] * d[:, None, :]).astype(np.float32)
+    b = rs.RopeBatch.from_positions(pos, 0.1, 500.0, 1e-4, 1 / 1200, 8)
+    b.set_gravity(np.array([0, 0, -9.81], np.float32))
+    b.damping = 0.0
+    b.set_planes(np.array([[0, 0, 1, 0]], np.float32), radius=0.004, mu_s=0.5, mu_k=0.4)
+    g = sdf.from_function(sdf.sdf_sphere(0.03), (-0.05,) * 3, (0.05,) * 3, 0.005)
+    cen = (roots + 0.12 * d + np.array([0.02, 0.0, -0.03])).astype(np.float32)
+    b.add_mesh(g.values, g.origin, g.spacing, pos=cen, quat=np.tile(np.float32([0, 0, 0, 1]), (n, 1)))
+    nv = n * V
+    t = lambda a: torch.tensor(np.asarray(a, np.float32), device="cuda", requires_grad=True)
+    jit = 1.0 + 0.1 * np.sin(np.arange(nv) * 0.37)
+    jit2 = 1.0 + 0.1 * np.cos(np.arange(nv) * 0.53)
+    EA, EI = t(500.0 * jit), t(1e-4 * jit2)
+    M = t(np.tile(np.r_[np.full(V - 1, 0.001), 0.0005], n))
+    p0 = np.repeat(roots[None], F, 0)
+    v0 = 0.05 * np.stack([np.column_stack([np.cos(idx + f), np.sin(idx + f), np.zeros(n)]) for f in range(F)])
+    P0, V0, P1 = t(p0), t(v0), t(p0 + 0.002 * v0)
+    D = torch.tensor(0.2, device="cuda", requires_grad=True)
+    x = rs.diff.rollout(b, EA, EI, M, D, P0, V0, P1, 1 / 60, 20)
+    w = torch.tensor(np.cos(np.arange(nv * 3) * 0.11).reshape(nv, 3), device="cuda", dtype=torch.float32)
+    ((x * w).sum(0) ** 2).sum().backward()
+    return {k: v.grad.detach().cpu().numpy() for k, v in dict(ea=EA, ei=EI, m=M, p0=P0, v0=V0, p1=P1).items()}
+
+
+def test_packed_adjoint_workgroups_match_single_rope_workgroups(rs):
+    """8192 short ropes are enough for the adjoint to pack three per 64-thread workgroup (the batch still gives
+    every CU eight workgroups); 8 ropes get one each. Ropes never interact, so each rope's gradients must not
+    depend on the packing: the first 8 ropes agree bit for bit."""
+    V, k = 20, 8
+    big = _short_rope_batch(rs, 8192)
+    small = _short_rope_batch(rs, k)
+    for name in ("ea", "ei", "m"):
+        assert np.array_equal(big[name][:k * V].view(np.uint32), small[name].view(np.uint32)), name
+    for name in ("p0", "v0", "p1"):
+        assert np.array_equal(big[name][:, :k].view(np.uint32), small[name].view(np.uint32)), name
+    assert np.abs(small["ea"]).max() > 0 and np.abs(small["p1"]).max() > 0
