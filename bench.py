@@ -16,7 +16,8 @@ baseline (the float32 oracle on the host cores, a bounded sample of the same wor
 The "readout" object continues the headline batch with the contact-force readout on (float wrench sums, then
 the reproducible fixed-point sums) and reports the time per frame of each against the readout off, and does the
 same on ropes resting on a slab mesh. The "bake_mesh" object times ropesim.sdf.bake_mesh on a 32768-triangle
-torus.
+torus. The "rope_bvh" object times one long rope in self-contact, and the headline batch, through the
+per-substep kernels with the full-scan rope-rope detection and with the per-group bounding-volume hierarchy.
 """
 import argparse
 import json
@@ -145,6 +146,67 @@ def bake_bench():
             "ms": 1e3 * dt, "node_triangle_pairs_per_s": nodes * len(t) / dt,
             "max_abs_dev_from_torus_sdf_m": float(np.abs(g.values - exact).max()),
             "timing": "host to host: upload, kernel, download"}
+
+
+def rope_bvh_bench(sizes, frames=3):
+    """Rope-rope detection in one long rope with self-contact, through the per-substep kernels (what
+    RopeSim(collisions=True) runs for ropes no cell holds): a helix of n vertices (5 mm segments, turns 3.2 mm
+    apart) settling on its pinned first vertex, 20 substeps x 8 iterations per 1/60 s frame, frames captured
+    as graphs. ms per frame with the full scan and with the hierarchy, and whether their positions agree bit
+    for bit; then the same comparison on many small groups (the headline batch through the per-substep
+    kernels)."""
+    import numpy as np
+    import torch
+
+    import ropesim as rs
+    out = {"workload": "one rope coiled as a helix (5 mm segments, turns 3.2 mm apart) in self-contact, per-substep "
+                       "kernels, 20 substeps x 8 iterations per 1/60 s frame",
+           "timing": f"wall clock of {frames} frames after one warm-up frame", "sizes": []}
+    for n in sizes:
+        phi = np.arange(n) * 0.005 / 0.03
+        helix = [np.column_stack([0.03 * np.cos(phi), 0.03 * np.sin(phi), 0.0032 * phi / (2 * np.pi)])]
+        row, pos = {"vertices": n}, {}
+        for mode in ("scan", "bvh"):
+            b = rs.RopeBatch.from_positions(helix, 0.117, 11473.0, 4.48e-3, 1 / 6000, 8, pinned=None)
+            b.set_gravity(np.array([0, 0, -9.81], np.float32))
+            b.set_rope_contact(True, 0.001, 0.0008, 0.025, 0.4, 1e7, 1.2, 1)
+            b.rope_broad_phase = mode
+            b.set_dispatch(0)
+            b.set_graph(True)
+            b.step_free(1 / 60, 20)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(frames):
+                b.step_free(1 / 60, 20)
+            torch.cuda.synchronize()
+            row[f"{mode}_ms_per_frame"] = 1e3 * (time.perf_counter() - t0) / frames
+            pos[mode] = torch.as_tensor(b.positions(), device="cuda").cpu().numpy()
+        row["bvh_speedup"] = row["scan_ms_per_frame"] / row["bvh_ms_per_frame"]
+        row["bitwise_equal"] = bool(np.array_equal(pos["scan"].view(np.uint32), pos["bvh"].view(np.uint32)))
+        out["sizes"].append(row)
+        print(f"# rope bvh leg: {row}", file=sys.stderr, flush=True)
+    # many groups: the headline batch through the per-substep kernels
+    from ropesim import scenes
+    s = scenes.config3(n_ropes=4096, frames=frames + 1)
+    fis = s.frame_inputs(frames + 1)
+    row, pos = {"workload": "config3 batch (4096 ropes, contact groups of 4) through the per-substep kernels"}, {}
+    for mode in ("scan", "bvh"):
+        b = s.build(rs)
+        b.set_dispatch(0)
+        b.rope_broad_phase = mode
+        b.set_graph(True)
+        s.step(b, fis[0], lambda a: torch.as_tensor(a, device="cuda"))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(1, frames + 1):
+            s.step(b, fis[f], lambda a: torch.as_tensor(a, device="cuda"))
+        torch.cuda.synchronize()
+        row[f"{mode}_ms_per_frame"] = 1e3 * (time.perf_counter() - t0) / frames
+        pos[mode] = torch.as_tensor(b.positions(), device="cuda").cpu().numpy()
+    row["bvh_speedup"] = row["scan_ms_per_frame"] / row["bvh_ms_per_frame"]
+    row["bitwise_equal"] = bool(np.array_equal(pos["scan"].view(np.uint32), pos["bvh"].view(np.uint32)))
+    out["groups"] = row
+    return out
 
 
 def differentiation_bench(n_ropes, frames):
@@ -322,6 +384,8 @@ def main():
     ap.add_argument("--scale-adjoint-frames", type=int, default=30, help="config-5 frames of the adjoint leg")
     ap.add_argument("--no-readout", action="store_true", help="skip the contact-force readout leg")
     ap.add_argument("--no-bake", action="store_true", help="skip the mesh-distance-grid leg")
+    ap.add_argument("--bvh-sizes", default="1024,8192,32768",
+                    help="vertex counts of the long-rope self-contact leg (scan vs hierarchy; empty skips it)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -435,6 +499,8 @@ def main():
         line["readout"] = readout
     if not args.no_bake:
         line["bake_mesh"] = bake_bench()
+    if args.bvh_sizes:
+        line["rope_bvh"] = rope_bvh_bench([int(v) for v in args.bvh_sizes.split(",")])
     if args.diff_frames > 0:
         d = differentiation_bench(args.diff_ropes, args.diff_frames)
         line["differentiation"] = d

@@ -297,6 +297,22 @@ public:
         return w;
     }
     void set_dispatch(int mode, int variant) { check(rs_set_dispatch(b(), mode, variant)); }
+    // rope-rope detection of the per-substep kernels: "auto", "scan" or "bvh"
+    std::string rope_broad_phase() const {
+        int m = 0;
+        check(rs_get_rope_broad_phase(b(), &m, nullptr));
+        return m == 1 ? "scan" : m == 2 ? "bvh" : "auto";
+    }
+    void set_rope_broad_phase(const std::string& m) {
+        if (m != "auto" && m != "scan" && m != "bvh")
+            throw py::value_error("rope_broad_phase must be 'auto', 'scan' or 'bvh'");
+        check(rs_set_rope_broad_phase(b(), m == "scan" ? 1 : m == "bvh" ? 2 : 0));
+    }
+    bool rope_bvh_active() const {
+        int a = 0;
+        check(rs_get_rope_broad_phase(b(), nullptr, &a));
+        return a != 0;
+    }
     py::tuple dispatch() const {
         int m, v;
         rs_get_dispatch(b(), &m, &v);
@@ -796,6 +812,8 @@ PYBIND11_MODULE(_ropesim, m) {
         .def("set_velocities", &RopeSim::set_velocities, py::arg("velocities"))
         .def("set_dispatch", &RopeSim::set_dispatch, py::arg("mode"), py::arg("variant") = -1)
         .def_property_readonly("dispatch", &RopeSim::dispatch)
+        .def_property("rope_broad_phase", &RopeSim::rope_broad_phase, &RopeSim::set_rope_broad_phase)
+        .def_property_readonly("rope_bvh_active", &RopeSim::rope_bvh_active)
         .def_property("drag", [](RopeSim& s) { return s.param(RS_P_DRAG); }, [](RopeSim& s, float v) { s.set_param(RS_P_DRAG, v); })
         .def_property("drag_quad", [](RopeSim& s) { return s.param(RS_P_DRAG_QUAD); },
                       [](RopeSim& s, float v) { s.set_param(RS_P_DRAG_QUAD, v); })
@@ -907,6 +925,10 @@ PYBIND11_MODULE(_ropesim, m) {
         .def("set_dispatch", &RopeBatch::set_dispatch, py::arg("mode"), py::arg("variant") = -1,
              "0: one launch per kernel per substep; 1: one persistent frame kernel (variant -1: automatic).")
         .def_property_readonly("dispatch", &RopeBatch::dispatch)
+        .def_property("rope_broad_phase", &RopeBatch::rope_broad_phase, &RopeBatch::set_rope_broad_phase,
+                      "Rope-rope detection of the per-substep kernels: 'auto' (a bounding-volume hierarchy per "
+                      "contact group above 128 vertices), 'scan' or 'bvh'; all give the same lists.")
+        .def_property_readonly("rope_bvh_active", &RopeBatch::rope_bvh_active)
         .def("autotune", &RopeBatch::autotune, py::arg("dt"), py::arg("substeps"), py::arg("program") = true)
         .def("set_material", &RopeBatch::set_material, py::arg("ea") = py::none(), py::arg("ei") = py::none(),
              py::arg("mass") = py::none(),

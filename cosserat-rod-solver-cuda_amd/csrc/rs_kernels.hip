@@ -1063,7 +1063,8 @@ hipError_t launch_reset(const Params& P, const Dev& D, const float4* initX, cons
 
 // One substep of the per-substep method. dx / q ping-pong through D.dx/D.dx2 and D.q/D.q2; on return
 // D.q holds the current frames (the pointers in D are swapped as needed).
-hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s, int S, hipStream_t st) {
+hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s, int S, hipStream_t st,
+                                const Bvh* bvh) {
     const int nvb = nblk(P.nv, 256);
     if (Dr.mode == DRIVE_PROGRAM)
         hipLaunchKernelGGL(k_root_program, dim3(nblk(P.nropes, 128)), dim3(128), 0, st, P, D, Dr, D.q, s, S);
@@ -1072,7 +1073,14 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
         hipLaunchKernelGGL(k_init<true>, dim3(nblk(initThreads, 256)), dim3(256), 0, st, P, D, s, S, D.dx);
     else
         hipLaunchKernelGGL(k_init<false>, dim3(nblk(initThreads, 256)), dim3(256), 0, st, P, D, s, S, D.dx);
-    if (P.ropeContact) hipLaunchKernelGGL(k_detect, dim3(nblk(P.nv, 128)), dim3(128), 0, st, P, D);
+    if (P.ropeContact) {
+        if (bvh) {
+            const hipError_t e = launch_bvh_detect(P, D, *bvh, s == 0, st);
+            if (e != hipSuccess) return e;
+        } else {
+            hipLaunchKernelGGL(k_detect, dim3(nblk(P.nv, 128)), dim3(128), 0, st, P, D);
+        }
+    }
     float4* dIn = D.dx;
     float4* dOut = D.dx2;
     float4* qIn = D.q;
@@ -1195,7 +1203,8 @@ hipError_t launch_commit(Pose7* cur, const Pose7* tgt, uint8_t* tele, Pose* pose
                        N);
     return hipGetLastError();
 }
-hipError_t launch_check_detect(const Params& P, const Dev& D, hipStream_t s) {
+hipError_t launch_check_detect(const Params& P, const Dev& D, const Bvh* bvh, hipStream_t s) {
+    if (bvh) return launch_bvh_detect(P, D, *bvh, true, s);
     hipLaunchKernelGGL(k_detect, dim3(nblk(P.nv, 128)), dim3(128), 0, s, P, D);
     return hipGetLastError();
 }

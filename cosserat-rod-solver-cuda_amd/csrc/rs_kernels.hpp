@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "rs_bvh.hpp"
 #include "rs_types.hpp"
 
 namespace rs {
@@ -22,7 +23,9 @@ hipError_t launch_root_const(const Params& P, const Dev& D, const Drive& Dr, hip
 hipError_t launch_gather(const Params& P, const Dev& D, hipStream_t s);
 hipError_t launch_reset(const Params& P, const Dev& D, const float4* initX, const float4* initQ, const int* ids,
                         int count, hipStream_t s);
-hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s, int S, hipStream_t st);
+// bvh: rope-rope detection through the per-group hierarchy (rebuilt at s = 0, refitted after), else the full scan
+hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s, int S, hipStream_t st,
+                                const Bvh* bvh = nullptr);
 hipError_t launch_cell(int variant, const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
                        int ncells, int S, unsigned long long* stats, hipStream_t st);
 hipError_t launch_scatter_targets(Pose7* tgt, uint8_t* tele, int index, const float* pos, const float* q, int rows,
@@ -30,7 +33,7 @@ hipError_t launch_scatter_targets(Pose7* tgt, uint8_t* tele, int index, const fl
 hipError_t launch_stage_motion(Pose7* cur, const Pose7* tgt, uint8_t* tele, Motion* motion, int total, hipStream_t s);
 hipError_t launch_commit(Pose7* cur, const Pose7* tgt, uint8_t* tele, Pose* poses, const Grid* grids, int numSdf,
                          int index, int N, hipStream_t s);
-hipError_t launch_check_detect(const Params& P, const Dev& D, hipStream_t s);
+hipError_t launch_check_detect(const Params& P, const Dev& D, const Bvh* bvh, hipStream_t s);
 hipError_t launch_bake_mesh(const float4* tri, int ntri, const double* origin, double spacing, int nx, int ny, int nz,
                             int shell, float halfThick, float rnd, float* out, hipStream_t s);
 hipError_t launch_check_broadphase(const Params& P, const Dev& D, const float4* xs, const float4* dxs, int* counts,

@@ -15,7 +15,7 @@ constexpr int MAX_SDF = 64;    // meshes per batch (YarnBall.h:153 kMaxSdf)
 constexpr int MAX_CAND = 8;    // broad-phase candidates per vertex (YarnBall.h:153 kMaxSdfCand)
 constexpr int MAX_COLL = 128;  // rope-rope candidates per segment (YarnBall.h:38)
 constexpr int VERLET_CAP = 64; // skin-list entries per segment in the persistent kernel
-enum { ERR_NONE = 0, ERR_MAX_COLLISIONS = 1 };
+enum { ERR_NONE = 0, ERR_MAX_COLLISIONS = 1, ERR_BVH_DEPTH = 2 };
 
 // A signed-distance grid: phi on an (nx, ny, nz) node lattice, C order (i slowest), at
 // origin + (i, j, k) * h in the object's frame (YarnBall.h:141).

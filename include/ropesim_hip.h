@@ -163,9 +163,18 @@ int rs_error_codes(const rs_batch* b, int* last_error, int* last_warning);
 int rs_clear_error_codes(rs_batch* b);
 
 /* Dispatch: mode 0 = one launch per kernel per substep, mode 1 = one persistent kernel per frame
- * (variant -1: autotuned on the first frame). No counterpart in the reference. */
+ * (variant -1: autotuned on the first frame; when no cell holds the largest rope or contact group, mode 1
+ * runs the per-substep kernels). No counterpart in the reference. */
 int rs_set_dispatch(rs_batch* b, int mode, int variant);
 int rs_get_dispatch(const rs_batch* b, int* mode, int* variant);
+/* Rope-rope detection of the per-substep kernels (replaces YarnBall's LBVH path,
+ * third_party/yarnball/KittenEngine/YarnBall/sim/collision.cu:10-101 and KittenGpuLBVH/lbvh.cu, reached
+ * through RopeSim(collisions=True), csrc/bindings.cpp:225-235): 0 = auto (a per-group bounding-volume
+ * hierarchy for groups above 128 vertices, the full scan below), 1 = always the full scan, 2 = always the
+ * hierarchy. Both give the same candidate lists (ascending, lowest 128) and step limits. tree_active reports
+ * whether the hierarchy is used for the batch as configured. */
+int rs_set_rope_broad_phase(rs_batch* b, int mode);
+int rs_get_rope_broad_phase(const rs_batch* b, int* mode, int* tree_active);
 int rs_num_cell_variants(void);
 int rs_cell_variant_info(int variant, int* block, int* vpt, int* supported);
 /* Time one frame per supported variant on the current state (restored afterwards) and keep the fastest. */
