@@ -24,10 +24,9 @@ namespace rs {
 namespace {
 
 constexpr int EX_O = 0;    // orientation pushes: q̄ to i-1 [4], q̄ to i+1 [4], d̄ / x̄ to i+1 [3], rest Darboux to i-1 [4]
-constexpr int EX_F2 = 15;  // segment terms for the next vertex: f2 [3], H2 [6]
-constexpr int EX_U = 24;   // u [3], δ [3]
-constexpr int EX_P = 30;   // position-sweep pushes: d̄ [3], x̄ [3] to i+1
-constexpr int EX_N = 36;
+constexpr int EX_U = 15;   // u [3], δ [3]
+constexpr int EX_P = 21;   // position-sweep pushes: d̄ [3], x̄ [3] to i+1
+constexpr int EX_N = 27;
 
 __device__ __forceinline__ f3 ld3s(const float* a, int B, int t) { return mk3(a[t], a[B + t], a[2 * B + t]); }
 __device__ __forceinline__ void st3s(float* a, int B, int t, f3 v) {
@@ -189,7 +188,7 @@ template <int B, bool F>
 __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDev A, Drive Dr, int S) {
     __shared__ float4 sQ[2][B];
     __shared__ float sX[3 * B], sD[2][3 * B], sF2[9 * B];
-    __shared__ Pose sPoseAll[adjoint_pose_slots(B)];
+    extern __shared__ Pose sPoseAll[];  // the poses of the workgroup's ropes, sized at launch
     const int I = P.numItr;
     const int rb0 = A.blk[blockIdx.x], rb1 = A.blk[blockIdx.x + 1], t = threadIdx.x;
     const int v0 = D.roff[rb0], n = D.roff[rb1] - v0;
@@ -353,7 +352,7 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
 template <int B, bool F>
 __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A, Drive Dr, int S) {
     __shared__ float ex[EX_N * B];
-    __shared__ Pose sPoseAll[adjoint_pose_slots(B)];
+    extern __shared__ Pose sPoseAll[];  // the poses of the workgroup's ropes, sized at launch
     const int I = P.numItr;
     const int rb0 = A.blk[blockIdx.x], rb1 = A.blk[blockIdx.x + 1], t = threadIdx.x;
     const int v0 = D.roff[rb0], n = D.roff[rb1] - v0;
@@ -708,28 +707,31 @@ bool adjoint_block_supported(int block) { return block == 64 || block == 128 || 
 
 template <int B, bool F>
 static hipError_t launch_frame_f(const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S, int nblocks,
-                                 hipStream_t s) {
-    hipLaunchKernelGGL((k_adjoint_forward<B, F>), dim3(nblocks), dim3(B), 0, s, P, D, A, Dr, S);
+                                 size_t poseBytes, hipStream_t s) {
+    hipLaunchKernelGGL((k_adjoint_forward<B, F>), dim3(nblocks), dim3(B), poseBytes, s, P, D, A, Dr, S);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_adjoint_reverse<B, F>), dim3(nblocks), dim3(B), 0, s, P, D, A, Dr, S);
+    hipLaunchKernelGGL((k_adjoint_reverse<B, F>), dim3(nblocks), dim3(B), poseBytes, s, P, D, A, Dr, S);
     return hipGetLastError();
 }
 template <int B>
 static hipError_t launch_frame_t(const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S, int nblocks,
-                                 hipStream_t s) {
-    if (P.fastMath) return launch_frame_f<B, true>(P, D, A, Dr, S, nblocks, s);
-    return launch_frame_f<B, false>(P, D, A, Dr, S, nblocks, s);
+                                 size_t poseBytes, hipStream_t s) {
+    if (P.fastMath) return launch_frame_f<B, true>(P, D, A, Dr, S, nblocks, poseBytes, s);
+    return launch_frame_f<B, false>(P, D, A, Dr, S, nblocks, poseBytes, s);
 }
 
 hipError_t launch_adjoint_frame(int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
-                                int nblocks, hipStream_t s) {
+                                int nblocks, int ropesPerBlock, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
+    if (ropesPerBlock < 1 || (size_t)ropesPerBlock * P.numSdf > (size_t)adjoint_pose_slots(block))
+        return hipErrorInvalidValue;
+    const size_t poseBytes = (size_t)ropesPerBlock * P.numSdf * sizeof(Pose);
     switch (block) {
-        case 64: return launch_frame_t<64>(P, D, A, Dr, S, nblocks, s);
-        case 128: return launch_frame_t<128>(P, D, A, Dr, S, nblocks, s);
-        case 192: return launch_frame_t<192>(P, D, A, Dr, S, nblocks, s);
-        case 256: return launch_frame_t<256>(P, D, A, Dr, S, nblocks, s);
+        case 64: return launch_frame_t<64>(P, D, A, Dr, S, nblocks, poseBytes, s);
+        case 128: return launch_frame_t<128>(P, D, A, Dr, S, nblocks, poseBytes, s);
+        case 192: return launch_frame_t<192>(P, D, A, Dr, S, nblocks, poseBytes, s);
+        case 256: return launch_frame_t<256>(P, D, A, Dr, S, nblocks, poseBytes, s);
         default: return hipErrorInvalidValue;
     }
 }

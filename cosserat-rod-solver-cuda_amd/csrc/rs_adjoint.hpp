@@ -50,9 +50,11 @@ constexpr int ADJ_MAX_BLOCK = 256;
 constexpr int adjoint_pose_slots(int block) { return block > MAX_SDF ? block : MAX_SDF; }
 size_t adjoint_history_bytes(int numItr);
 bool adjoint_block_supported(int block);  // 64, 128, 192, 256
-// Both passes over `nblocks` workgroups of `block` threads, workgroup b on the ropes [A.blk[b], A.blk[b + 1]).
+// Both passes over `nblocks` workgroups of `block` threads, workgroup b on the ropes [A.blk[b], A.blk[b + 1]),
+// at most ropesPerBlock of them (their mesh poses take ropesPerBlock * numSdf LDS slots, at most
+// adjoint_pose_slots(block)).
 hipError_t launch_adjoint_frame(int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
-                                int nblocks, hipStream_t s);
+                                int nblocks, int ropesPerBlock, hipStream_t s);
 hipError_t launch_adjoint_finish(const Params& P, const Dev& D, const float4* rhoUnit, const double* gk,
                                  const double* grho, const double* gm, float* gea, float* gei, float* gmass,
                                  hipStream_t s);

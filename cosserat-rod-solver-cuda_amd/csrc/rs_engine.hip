@@ -1765,9 +1765,11 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
             chunks.push_back(c);
         }
         std::vector<int> blkTable, chunkBlk0, chunkBlks;  // per chunk: offset into the table, workgroups
+        int ropesPerBlock = 1;                            // the most ropes any workgroup holds (LDS pose slots)
         for (const Chunk& c : chunks) {
             std::vector<int> t;
             pack(blk, perBlock, c.rope0, c.rope0 + c.nropes, &t);
+            for (size_t k = 0; k + 1 < t.size(); k++) ropesPerBlock = std::max(ropesPerBlock, t[k + 1] - t[k]);
             chunkBlk0.push_back((int)blkTable.size());
             chunkBlks.push_back((int)t.size() - 1);
             blkTable.insert(blkTable.end(), t.begin(), t.end());
@@ -1854,7 +1856,8 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
                 A.mismatch = b->d_mismatch;
                 A.fposes = M > 0 ? T.poses + (size_t)f * N * M : nullptr;
                 A.fmotion = (M > 0 && T.P[f].sdfMoving) ? T.motion + (size_t)f * N * M : nullptr;
-                hcheck(launch_adjoint_frame(blk, T.P[f], b->D, A, dr, S, chunkBlks[ci], st), "adjoint frame");
+                hcheck(launch_adjoint_frame(blk, T.P[f], b->D, A, dr, S, chunkBlks[ci], ropesPerBlock, st),
+                       "adjoint frame");
             }
         }
         hcheck(launch_adjoint_finish(b->params(), b->D, b->d_rhoUnit, b->d_gk, b->d_grho, b->d_gm, g_ea, g_ei, g_mass,
