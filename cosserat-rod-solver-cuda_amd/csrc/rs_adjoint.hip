@@ -16,6 +16,8 @@
 // Notation: for a Newton update d' = d + a H^-1 F, the adjoint u = a H^-1 d̄' gives F̄ = u and
 // H̄ = -u δ^T with δ = H^-1 F. The pair (u, δ) stands for H̄; every H term here is s I or s n n^T, so
 // only tr H̄ = -u.δ, n^T H̄ n = -(u.n)(δ.n) and (H̄ + H̄^T) n = -(u (δ.n) + δ (u.n)) are needed.
+#include <algorithm>
+
 #include "rs_adjoint.hpp"
 #include "rs_device.hpp"
 
@@ -703,35 +705,77 @@ size_t adjoint_history_bytes(int numItr) {
            (size_t)numItr * 9 * sizeof(float);
 }
 
-bool adjoint_block_supported(int block) { return block == 64 || block == 128 || block == 192 || block == 256; }
+bool adjoint_block_supported(int block, bool reverse) {
+    return block == 64 || block == 128 || block == 192 || block == 256 || (reverse && block == 512);
+}
+
+namespace {
+// Waves per SIMD that a kernel's VGPR allocation allows (512 VGPRs per SIMD lane, granule 8, at most 8).
+int waves_of(const void* fn) {
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, fn) != hipSuccess || a.numRegs <= 0) return 1;
+    const int regs = (a.numRegs + 7) / 8 * 8;
+    return std::max(1, std::min(8, 512 / regs));
+}
+template <int B>
+const void* pass_fn(bool reverse, bool fast) {
+    if (reverse)
+        return fast ? reinterpret_cast<const void*>(&k_adjoint_reverse<B, true>)
+                    : reinterpret_cast<const void*>(&k_adjoint_reverse<B, false>);
+    return fast ? reinterpret_cast<const void*>(&k_adjoint_forward<B, true>)
+                : reinterpret_cast<const void*>(&k_adjoint_forward<B, false>);
+}
+}  // namespace
+
+int adjoint_waves_per_simd(int block, bool reverse, bool fast) {
+    switch (block) {
+        case 64: return waves_of(pass_fn<64>(reverse, fast));
+        case 128: return waves_of(pass_fn<128>(reverse, fast));
+        case 192: return waves_of(pass_fn<192>(reverse, fast));
+        case 256: return waves_of(pass_fn<256>(reverse, fast));
+        case 512:
+            if (!reverse) return 1;
+            return waves_of(fast ? reinterpret_cast<const void*>(&k_adjoint_reverse<512, true>)
+                                 : reinterpret_cast<const void*>(&k_adjoint_reverse<512, false>));
+        default: return 1;
+    }
+}
 
 template <int B, bool F>
-static hipError_t launch_frame_f(const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S, int nblocks,
-                                 size_t poseBytes, hipStream_t s) {
-    hipLaunchKernelGGL((k_adjoint_forward<B, F>), dim3(nblocks), dim3(B), poseBytes, s, P, D, A, Dr, S);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_adjoint_reverse<B, F>), dim3(nblocks), dim3(B), poseBytes, s, P, D, A, Dr, S);
+static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
+                                int nblocks, size_t poseBytes, hipStream_t s) {
+    if (reverse)
+        hipLaunchKernelGGL((k_adjoint_reverse<B, F>), dim3(nblocks), dim3(B), poseBytes, s, P, D, A, Dr, S);
+    else
+        hipLaunchKernelGGL((k_adjoint_forward<B, F>), dim3(nblocks), dim3(B), poseBytes, s, P, D, A, Dr, S);
     return hipGetLastError();
 }
 template <int B>
-static hipError_t launch_frame_t(const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S, int nblocks,
-                                 size_t poseBytes, hipStream_t s) {
-    if (P.fastMath) return launch_frame_f<B, true>(P, D, A, Dr, S, nblocks, poseBytes, s);
-    return launch_frame_f<B, false>(P, D, A, Dr, S, nblocks, poseBytes, s);
+static hipError_t launch_pass_t(bool reverse, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
+                                int nblocks, size_t poseBytes, hipStream_t s) {
+    if (P.fastMath) return launch_pass_f<B, true>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
+    return launch_pass_f<B, false>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
 }
 
-hipError_t launch_adjoint_frame(int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
-                                int nblocks, int ropesPerBlock, hipStream_t s) {
+hipError_t launch_adjoint_pass(bool reverse, int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr,
+                               int S, int nblocks, int ropesPerBlock, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    if (ropesPerBlock < 1 || (size_t)ropesPerBlock * P.numSdf > (size_t)adjoint_pose_slots(block))
+    if (!adjoint_block_supported(block, reverse) || ropesPerBlock < 1 ||
+        (size_t)ropesPerBlock * P.numSdf > (size_t)adjoint_pose_slots(block))
         return hipErrorInvalidValue;
     const size_t poseBytes = (size_t)ropesPerBlock * P.numSdf * sizeof(Pose);
     switch (block) {
-        case 64: return launch_frame_t<64>(P, D, A, Dr, S, nblocks, poseBytes, s);
-        case 128: return launch_frame_t<128>(P, D, A, Dr, S, nblocks, poseBytes, s);
-        case 192: return launch_frame_t<192>(P, D, A, Dr, S, nblocks, poseBytes, s);
-        case 256: return launch_frame_t<256>(P, D, A, Dr, S, nblocks, poseBytes, s);
+        case 64: return launch_pass_t<64>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
+        case 128: return launch_pass_t<128>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
+        case 192: return launch_pass_t<192>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
+        case 256: return launch_pass_t<256>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
+        case 512:
+            if (P.fastMath)
+                hipLaunchKernelGGL((k_adjoint_reverse<512, true>), dim3(nblocks), dim3(512), poseBytes, s, P, D, A, Dr, S);
+            else
+                hipLaunchKernelGGL((k_adjoint_reverse<512, false>), dim3(nblocks), dim3(512), poseBytes, s, P, D, A, Dr,
+                                   S);
+            return hipGetLastError();
         default: return hipErrorInvalidValue;
     }
 }

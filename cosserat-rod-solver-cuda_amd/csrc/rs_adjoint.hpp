@@ -21,8 +21,8 @@ struct AdjDev {
     const float4 *nx, *nv, *nvl, *nq;
     // history of the frame for the vertices [hv0, hv0 + hn) of a launch, local vertex i - hv0
     int rope0, hv0, hn;
-    // workgroup b of a launch advances the consecutive ropes [blk[b], blk[b + 1]) (at most ADJ_MAX_BLOCK vertices
-    // and adjoint_pose_slots(block) / numSdf ropes in all)
+    // workgroup b of a launch advances the consecutive ropes [blk[b], blk[b + 1]) (at most the launch's block size
+    // in vertices and adjoint_pose_slots(block) / numSdf ropes in all)
     const int* blk;
     float4 *hX, *hV, *hVL, *hY;  // [S][hn] substep inputs: x after the root program (w: broad-phase count bits),
                                  // v after the root program, last velocity before init, inertial target y
@@ -44,17 +44,20 @@ struct AdjDev {
     const Motion* fmotion;  // frame motion [N][numSdf] while meshes move, else null
 };
 
-// Largest workgroup (= longest rope) the backward kernels support; pose slots a workgroup keeps in LDS (its ropes
-// times the meshes); history bytes per vertex-substep.
+// Longest rope the backward kernels support; pose slots a workgroup keeps in LDS (its ropes times the meshes);
+// history bytes per vertex-substep.
 constexpr int ADJ_MAX_BLOCK = 256;
 constexpr int adjoint_pose_slots(int block) { return block > MAX_SDF ? block : MAX_SDF; }
 size_t adjoint_history_bytes(int numItr);
-bool adjoint_block_supported(int block);  // 64, 128, 192, 256
-// Both passes over `nblocks` workgroups of `block` threads, workgroup b on the ropes [A.blk[b], A.blk[b + 1]),
-// at most ropesPerBlock of them (their mesh poses take ropesPerBlock * numSdf LDS slots, at most
-// adjoint_pose_slots(block)).
-hipError_t launch_adjoint_frame(int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
-                                int nblocks, int ropesPerBlock, hipStream_t s);
+bool adjoint_block_supported(int block, bool reverse);  // 64, 128, 192, 256; 512 for the reverse pass
+// Waves per SIMD the pass's register allocation allows at this block size (the device's occupancy limit).
+int adjoint_waves_per_simd(int block, bool reverse, bool fast);
+// One pass (forward recompute, or reverse) over `nblocks` workgroups of `block` threads, workgroup b on the
+// ropes [A.blk[b], A.blk[b + 1]), at most ropesPerBlock of them (their mesh poses take ropesPerBlock * numSdf
+// LDS slots, at most adjoint_pose_slots(block)). The two passes of a frame may use different packings: they
+// share only the per-vertex history.
+hipError_t launch_adjoint_pass(bool reverse, int block, const Params& P, const Dev& D, const AdjDev& A,
+                               const Drive& Dr, int S, int nblocks, int ropesPerBlock, hipStream_t s);
 hipError_t launch_adjoint_finish(const Params& P, const Dev& D, const float4* rhoUnit, const double* gk,
                                  const double* grho, const double* gm, float* gea, float* gei, float* gmass,
                                  hipStream_t s);

@@ -1702,9 +1702,10 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
         const size_t n = b->nv;
         int maxV = 0;
         for (int r = 0; r < N; r++) maxV = std::max(maxV, b->roff[r + 1] - b->roff[r]);
-        // Workgroups of the adjoint: consecutive ropes packed up to `blk` vertices (and the LDS pose slots), with
-        // the size among 64..256 that issues the fewest waves.
+        // Workgroups of each pass: consecutive ropes packed up to B vertices (and the LDS pose slots). The passes
+        // share only the per-vertex history, so each takes its own packing.
         const int M = T.numSdf;
+        const bool fastAdj = T.P.empty() ? b->fastMath : T.P[0].fastMath != 0;  // the recorded arithmetic
         auto pack = [&](int B, int ropesPerBlock, int r0, int r1, std::vector<int>* out) {
             long waves = 0;
             int r = r0;
@@ -1721,28 +1722,42 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
             if (out) out->push_back(r1);
             return waves;
         };
-        // One rope per workgroup of the smallest size that holds the longest rope, unless packing consecutive
-        // ropes issues fewer waves and still leaves every CU several workgroups (a small batch needs the
-        // parallelism more than the lanes).
-        int blk = -1, perBlock = 1;
-        long bestWaves = 0;
-        for (int B : {64, 128, 192, 256}) {
-            if (B < maxV || !adjoint_block_supported(B)) continue;
-            if (blk < 0) {
-                blk = B;
-                bestWaves = pack(B, 1, 0, N, nullptr);
+        // One rope per workgroup of the smallest size that holds the longest rope, unless packing consecutive ropes
+        // costs less and still leaves every CU several workgroups (a small batch needs the parallelism more than the
+        // lanes). Both passes are latency-bound: a workgroup takes about as long with a few busy lanes as with
+        // all, so the cost is the number of rounds, workgroups over the workgroups a CU holds at the pass's
+        // register-limited waves per SIMD.
+        struct Packing {
+            int blk = -1, perBlock = 1;
+        };
+        auto choose = [&](bool reverse) {
+            Packing best;
+            double bestCost = 0.0;
+            for (int B : {64, 128, 192, 256, 512}) {
+                if (B < maxV || !adjoint_block_supported(B, reverse)) continue;
+                const int perCU = 4 * adjoint_waves_per_simd(B, reverse, fastAdj) / (B / 64);
+                if (perCU < 1) continue;
+                if (best.blk < 0) {
+                    std::vector<int> t;
+                    pack(B, 1, 0, N, &t);
+                    best.blk = B;
+                    bestCost = (double)(t.size() - 1) / perCU;
+                }
+                const int maxRopes = M > 0 ? adjoint_pose_slots(B) / M : B;
+                if (maxRopes < 2) continue;
+                std::vector<int> t;
+                pack(B, maxRopes, 0, N, &t);
+                const double c = (double)(t.size() - 1) / perCU;
+                if (c < bestCost && (long)t.size() - 1 >= 8L * device_cus()) {
+                    best.blk = B;
+                    best.perBlock = maxRopes;
+                    bestCost = c;
+                }
             }
-            const int maxRopes = M > 0 ? adjoint_pose_slots(B) / M : B;
-            if (maxRopes < 2) continue;
-            std::vector<int> t;
-            const long w = pack(B, maxRopes, 0, N, &t);
-            if (w < bestWaves && (long)t.size() - 1 >= 8L * device_cus()) {
-                blk = B;
-                perBlock = maxRopes;
-                bestWaves = w;
-            }
-        }
-        if (blk < 0) throw Invalid("the adjoint supports ropes of at most 256 vertices");
+            if (best.blk < 0) throw Invalid("the adjoint supports ropes of at most 256 vertices");
+            return best;
+        };
+        const Packing pk[2] = {choose(false), choose(true)};
         // the history of a frame is kept for a chunk of ropes at a time, within a fixed budget
         int maxI = 1;
         for (const Params& P : T.P) maxI = std::max(maxI, P.numItr);
@@ -1764,16 +1779,19 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
             }
             chunks.push_back(c);
         }
-        std::vector<int> blkTable, chunkBlk0, chunkBlks;  // per chunk: offset into the table, workgroups
-        int ropesPerBlock = 1;                            // the most ropes any workgroup holds (LDS pose slots)
-        for (const Chunk& c : chunks) {
-            std::vector<int> t;
-            pack(blk, perBlock, c.rope0, c.rope0 + c.nropes, &t);
-            for (size_t k = 0; k + 1 < t.size(); k++) ropesPerBlock = std::max(ropesPerBlock, t[k + 1] - t[k]);
-            chunkBlk0.push_back((int)blkTable.size());
-            chunkBlks.push_back((int)t.size() - 1);
-            blkTable.insert(blkTable.end(), t.begin(), t.end());
-        }
+        // per pass and chunk: offset into the table and workgroups; per pass the most ropes any workgroup holds
+        std::vector<int> blkTable, chunkBlk0[2], chunkBlks[2];
+        int ropesPerBlock[2] = {1, 1};
+        for (int pass = 0; pass < 2; pass++)
+            for (const Chunk& c : chunks) {
+                std::vector<int> t;
+                pack(pk[pass].blk, pk[pass].perBlock, c.rope0, c.rope0 + c.nropes, &t);
+                for (size_t k = 0; k + 1 < t.size(); k++)
+                    ropesPerBlock[pass] = std::max(ropesPerBlock[pass], t[k + 1] - t[k]);
+                chunkBlk0[pass].push_back((int)blkTable.size());
+                chunkBlks[pass].push_back((int)t.size() - 1);
+                blkTable.insert(blkTable.end(), t.begin(), t.end());
+            }
         if (blkTable.size() > b->blkCap) {
             if (b->d_blk) hipFree(b->d_blk);
             b->d_blk = dalloc<int>(blkTable.size(), "adjoint workgroups");
@@ -1819,7 +1837,6 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
             for (size_t ci = 0; ci < chunks.size(); ci++) {
                 const Chunk& c = chunks[ci];
                 AdjDev A{};
-                A.blk = b->d_blk + chunkBlk0[ci];
                 A.cx = ck;
                 A.cv = ck + n;
                 A.cvl = ck + 2 * n;
@@ -1856,8 +1873,12 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
                 A.mismatch = b->d_mismatch;
                 A.fposes = M > 0 ? T.poses + (size_t)f * N * M : nullptr;
                 A.fmotion = (M > 0 && T.P[f].sdfMoving) ? T.motion + (size_t)f * N * M : nullptr;
-                hcheck(launch_adjoint_frame(blk, T.P[f], b->D, A, dr, S, chunkBlks[ci], ropesPerBlock, st),
-                       "adjoint frame");
+                for (int pass = 0; pass < 2; pass++) {
+                    A.blk = b->d_blk + chunkBlk0[pass][ci];
+                    hcheck(launch_adjoint_pass(pass == 1, pk[pass].blk, T.P[f], b->D, A, dr, S, chunkBlks[pass][ci],
+                                               ropesPerBlock[pass], st),
+                           pass ? "adjoint reverse pass" : "adjoint forward pass");
+                }
             }
         }
         hcheck(launch_adjoint_finish(b->params(), b->D, b->d_rhoUnit, b->d_gk, b->d_grho, b->d_gm, g_ea, g_ei, g_mass,

@@ -218,7 +218,7 @@ def _short_rope_batch(rs, n, F=2, V=20):
     import torch
     from ropesim import sdf
     idx = np.arange(n)
-    roots = np.column_stack([0.02 * (idx % 64), 0.02 * (idx // 64), np.full(n, 0.12)])
+    roots = np.column_stack([0.02 * (idx % 64), 0.02 * (idx // 64), np.full(n, 0.12 if V <= 20 else 0.05 + 0.01 * V)])
     d = np.column_stack([0.3 * np.sin(idx), 0.3 * np.cos(idx), -np.ones(n)])
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     pos = (roots[:, None, :] + 0.01 * np.arange(V)[None, :, None] * d[:, None, :]).astype(np.float32)
@@ -246,9 +246,9 @@ def _short_rope_batch(rs, n, F=2, V=20):
 
 
 def test_packed_adjoint_workgroups_match_single_rope_workgroups(rs):
-    """8192 short ropes are enough for the adjoint to pack three per 64-thread workgroup (the batch still gives
-    every CU eight workgroups); 8 ropes get one each. Ropes never interact, so each rope's gradients must not
-    depend on the packing: the first 8 ropes agree bit for bit."""
+    """8192 short ropes are enough for the adjoint to pack three per 64-thread workgroup in both passes (the
+    batch still gives every CU eight workgroups); 8 ropes get one each. Ropes never interact, so each rope's
+    gradients must not depend on the packing: the first 8 ropes agree bit for bit."""
     V, k = 20, 8
     big = _short_rope_batch(rs, 8192)
     small = _short_rope_batch(rs, k)
@@ -257,3 +257,17 @@ def test_packed_adjoint_workgroups_match_single_rope_workgroups(rs):
     for name in ("p0", "v0", "p1"):
         assert np.array_equal(big[name][:, :k].view(np.uint32), small[name].view(np.uint32)), name
     assert np.abs(small["ea"]).max() > 0 and np.abs(small["p1"]).max() > 0
+
+
+def test_passes_with_different_packings_match_single_rope_workgroups(rs):
+    """10240 ropes of 96 vertices: the recompute pass packs two per 192-thread workgroup, the reverse pass (more
+    registers, fewer waves per SIMD) five per 512-thread workgroup; the first 8 ropes' gradients equal those of
+    an 8-rope batch (one rope per workgroup in both passes) bit for bit."""
+    V, k = 96, 8
+    big = _short_rope_batch(rs, 10240, V=V)
+    small = _short_rope_batch(rs, k, V=V)
+    for name in ("ea", "ei", "m"):
+        assert np.array_equal(big[name][:k * V].view(np.uint32), small[name].view(np.uint32)), name
+    for name in ("p0", "v0", "p1"):
+        assert np.array_equal(big[name][:, :k].view(np.uint32), small[name].view(np.uint32)), name
+    assert np.abs(small["ei"]).max() > 0
