@@ -5,7 +5,7 @@
 // Build, once per frame:
 // 1. scene box: partial min/max per block, reduced again by every block of the key kernel;
 // 2. keys: (group index << 30) | 30-bit Morton code of the segment midpoint, value = vertex;
-// 3. stable LSD radix sort of the keys, 8 bits per pass (digit histograms per tile, one exclusive scan,
+// 3. stable LSD radix sort of the keys, 8 bits per pass (digit histograms per tile, a scan per digit row,
 //    scatter ranked by wave ballots). The group bits keep every group at its own position range;
 // 4. one binary radix tree per group over its sorted keys, by Karras, "Maximizing parallelism in the
 //    construction of BVHs, octrees, and k-d trees" (HPG 2012), section 4: internal node p of the group
@@ -168,40 +168,54 @@ __global__ __launch_bounds__(BVH_SORT_THREADS) void k_sort_hist(const u64* key, 
     hist[t * tiles + blockIdx.x] = cnt[t];
 }
 
-// Exclusive prefix over the digit-major histogram, in place (one workgroup; each thread a contiguous run).
-__global__ __launch_bounds__(1024) void k_sort_scan(int* hist, int total) {
-    __shared__ int part[1024];
+// Exclusive prefix of each digit's row of tile counts, in place (one workgroup per digit, rows in chunks of
+// 256 tiles), and the row's total.
+__global__ __launch_bounds__(256) void k_sort_scan(int* hist, int tiles, int* digitTotal) {
+    __shared__ int part[256];
     const int t = threadIdx.x;
-    const int per = (total + 1023) / 1024;
-    const int b0 = min(total, t * per), b1 = min(total, b0 + per);
-    int sum = 0;
-    for (int i = b0; i < b1; i++) sum += hist[i];
-    part[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int v = t >= o ? part[t - o] : 0;
+    int* row = hist + (size_t)blockIdx.x * tiles;
+    int carry = 0;
+    for (int c0 = 0; c0 < tiles; c0 += 256) {
+        const int j = c0 + t;
+        const int v = j < tiles ? row[j] : 0;
+        part[t] = v;
         __syncthreads();
-        part[t] += v;
+        for (int o = 1; o < 256; o <<= 1) {
+            const int u = t >= o ? part[t - o] : 0;
+            __syncthreads();
+            part[t] += u;
+            __syncthreads();
+        }
+        if (j < tiles) row[j] = carry + part[t] - v;
+        carry += part[255];
         __syncthreads();
     }
-    int run = part[t] - sum;
-    for (int i = b0; i < b1; i++) {
-        const int v = hist[i];
-        hist[i] = run;
-        run += v;
-    }
+    if (t == 0) digitTotal[blockIdx.x] = carry;
 }
 
 // Stable scatter of one tile: items go out in (round, wave, lane) order, which is their input order.
-// A lane's rank among the earlier lanes of its wave with the same digit comes from eight ballots.
+// A lane's rank among the earlier lanes of its wave with the same digit comes from eight ballots. A digit's
+// base is the exclusive prefix of the digit totals (formed here) plus this tile's row prefix.
 __global__ __launch_bounds__(BVH_SORT_THREADS) void k_sort_scatter(const u64* kin, const int* vin, u64* kout,
                                                                    int* vout, int n, int shift, const int* hist,
-                                                                   int tiles) {
+                                                                   int tiles, const int* digitTotal) {
+    static_assert(BVH_SORT_THREADS == 256, "one digit per thread");
     constexpr int W = BVH_SORT_THREADS / 64;
     __shared__ int run[256];
     __shared__ int wcnt[W][256];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    run[t] = hist[t * tiles + blockIdx.x];
+    {
+        const int tot = digitTotal[t];
+        run[t] = tot;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int u = t >= o ? run[t - o] : 0;
+            __syncthreads();
+            run[t] += u;
+            __syncthreads();
+        }
+        run[t] = run[t] - tot + hist[t * tiles + blockIdx.x];
+    }
     const u64 below = (1ull << lane) - 1ull;
     const int base = blockIdx.x * BVH_SORT_TILE;
     for (int r = 0; r < BVH_SORT_ITEMS; r++) {
@@ -397,7 +411,7 @@ __global__ __launch_bounds__(128) void k_bvh_query(Params P, Dev D, Bvh T) {
     const float ri = fmaxf(D.pad[i], D.pad[i + 1]);
     int* row = D.coll + (size_t)i * MAX_COLL;
     int cnt = 0;
-    int stack[BVH_STACK];
+    int stack[BVH_STACK];  // private (scratch): an LDS stack of 256 B per lane would cap a CU at 10 waves
     int sp = 0;
     int node = hi - lo >= 2 ? lo : ~lo;
     for (;;) {
@@ -486,6 +500,7 @@ hipError_t bvh_alloc(Bvh& T, int nv, int numGroups) {
     get((void**)&T.val, n * 4);
     get((void**)&T.val2, n * 4);
     get((void**)&T.hist, (size_t)256 * T.tiles * 4);
+    get((void**)&T.digitTotal, 256 * 4);
     get((void**)&T.child, n * 8);
     get((void**)&T.range, n * 8);
     get((void**)&T.box, 2 * n * 16);
@@ -496,7 +511,7 @@ hipError_t bvh_alloc(Bvh& T, int nv, int numGroups) {
 }
 
 void bvh_free(Bvh& T) {
-    for (void* p : {(void*)T.key, (void*)T.key2, (void*)T.val, (void*)T.val2, (void*)T.hist, (void*)T.child,
+    for (void* p : {(void*)T.key, (void*)T.key2, (void*)T.val, (void*)T.val2, (void*)T.hist, (void*)T.digitTotal, (void*)T.child,
                     (void*)T.range, (void*)T.box, (void*)T.seg, (void*)T.bounds})
         if (p) (void)hipFree(p);
     T = Bvh{};
@@ -519,9 +534,9 @@ hipError_t launch_bvh_detect(const Params& P, const Dev& D, const Bvh& T, bool r
             const int shift = 8 * pass;
             hipLaunchKernelGGL(k_sort_hist, dim3(T.tiles), dim3(BVH_SORT_THREADS), 0, st, kin, n, shift, T.hist,
                                T.tiles);
-            hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, T.hist, 256 * T.tiles);
+            hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, st, T.hist, T.tiles, T.digitTotal);
             hipLaunchKernelGGL(k_sort_scatter, dim3(T.tiles), dim3(BVH_SORT_THREADS), 0, st, kin, vin, kout, vout, n,
-                               shift, (const int*)T.hist, T.tiles);
+                               shift, (const int*)T.hist, T.tiles, (const int*)T.digitTotal);
             const u64* k2 = kout;
             const int* v2 = vout;
             kout = const_cast<u64*>(kin);

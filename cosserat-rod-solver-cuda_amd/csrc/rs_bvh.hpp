@@ -30,7 +30,8 @@ struct Bvh {
     int segEntries = 0;            // entries of all levels
     unsigned long long *key = nullptr, *key2 = nullptr;  // [n] sort keys (ping-pong)
     int *val = nullptr, *val2 = nullptr;                 // [n] sorted position -> vertex (ping-pong)
-    int* hist = nullptr;      // [256 * tiles] digit counts, then their exclusive prefix
+    int* hist = nullptr;      // [256 * tiles] digit counts per tile, then their prefix within each digit's row
+    int* digitTotal = nullptr;  // [256] keys per digit
     int2* child = nullptr;    // [n] children of internal node p (~x: leaf at sorted position x)
     int2* range = nullptr;    // [n] first and last sorted position under internal node p
     float4* box = nullptr;    // [2n] (lo, hi) of internal node p at [2p]
