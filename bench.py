@@ -52,10 +52,30 @@ def profiled_traffic():
     return None
 
 
-def flops_per_vertex_substep(iters, pair_tests_per_vertex=0.0):
-    """Algorithmic flop estimate per vertex-substep: init + end ~60, per iteration position sweep ~200 and
-    orientation sweep ~250, rope-rope pair tests ~95 each."""
-    return 60.0 + iters * 450.0 + 95.0 * pair_tests_per_vertex
+# Flops of the method per unit, counted by running the oracle on an operation-counting scalar
+# (oracle/opcount.cpp; add, sub, mul, div, sqrt and transcendentals one flop each, unfused). Checked against a
+# fresh count by tests/test_opcount.py.
+FLOP = {
+    "init_per_vertex_substep": 41.615,
+    "position_sweep_per_vertex_iteration": 160.508,
+    "orientation_sweep_per_vertex_iteration": 243.969,
+    "end_per_vertex_substep": 11.877,
+    "plane_per_free_vertex_iteration": 16.000,
+    "mesh_broad_phase_per_vertex_substep_per_mesh": 14.769,
+    "mesh_candidate_per_iteration": 106.000,
+    "pair_test": 108.016,
+}
+
+
+def flops_per_vertex_substep(iters, planes=0, meshes=0):
+    """Counted flops per vertex-substep of the solver (init, iterations, end), the plane tests and the mesh
+    broad phase. Mesh-candidate terms and rope-rope pair terms depend on the contact state and are not
+    included."""
+    f = FLOP["init_per_vertex_substep"] + FLOP["end_per_vertex_substep"]
+    f += iters * (FLOP["position_sweep_per_vertex_iteration"] + FLOP["orientation_sweep_per_vertex_iteration"])
+    f += iters * planes * FLOP["plane_per_free_vertex_iteration"]
+    f += meshes * FLOP["mesh_broad_phase_per_vertex_substep_per_mesh"]
+    return f
 
 
 def cpu_baseline(sample_ropes, threads):
@@ -71,10 +91,17 @@ def cpu_baseline(sample_ropes, threads):
     s.step(o, fi)
     dt = time.perf_counter() - t0
     assert np.isfinite(o.positions()).all()
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": s.num_vertices * s.substeps / dt, "unit": "vertex-substeps/s", "cores": threads,
             "kind": "port",
-            "sample": f"float32 oracle, {sample_ropes} ropes ({s.num_vertices} vertices, {sample_ropes // 4} "
-                      f"environments) of the same generator, 1 frame of {s.substeps} substeps, {dt:.1f} s"}
+            "sample": f"float32 oracle (OpenMP, {threads} threads on {model}), {sample_ropes} ropes "
+                      f"({s.num_vertices} vertices, {sample_ropes // 4} environments) of the same generator, 1 frame "
+                      f"of {s.substeps} substeps, {dt:.1f} s"}
 
 
 def resting_readout_bench(n_ropes, frames=5):
@@ -390,8 +417,8 @@ def main():
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        sys.exit("bench.py measures one GPU: the rope batch runs on a single device (environments would shard "
-                 "across GPUs, see DESIGN.md); run with --gpus 1")
+        sys.exit("bench.py measures one GPU: the rope batch runs on a single device (across GPUs the batch would "
+                 "shard by environment, one contiguous range of environments per GPU); run with --gpus 1")
 
     import numpy as np
     import torch
@@ -491,10 +518,13 @@ def main():
                             "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                             "kernel": "k_cell (persistent frame kernel; the timed span includes the two per-frame precompute kernels "
                                       "launched ahead of it, < 1 % of it)", "kernel_ms": kern_ms / launches}
-        fl = flops_per_vertex_substep(s.num_itr)
+        fl = flops_per_vertex_substep(s.num_itr, planes=0 if s.planes is None else len(s.planes),
+                                      meshes=b.num_meshes)
         tf = fl * units / per_launch_s / 1e12
         line["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP32_PEAK_TFLOPS,
-                        "flop_per_vertex_substep": fl, "note": "solver flops only; rope-rope pair tests excluded"}
+                        "flop_per_vertex_substep": fl,
+                        "note": "flops counted on the oracle (oracle/opcount.cpp): solver, plane tests and mesh broad "
+                                "phase; mesh-candidate and rope-rope pair terms excluded"}
     if readout is not None:
         line["readout"] = readout
     if not args.no_bake:

@@ -75,11 +75,15 @@ def test_device_config1_trajectory_matches_golden(rs, mode, fast):
 
 
 @pytest.mark.gpu
-def test_device_rope_rope_lists_match_golden(rs):
+@pytest.mark.parametrize("broad_phase", ["scan", "bvh"])
+def test_device_rope_rope_lists_match_golden(rs, broad_phase):
+    """Both rope-rope detections of the per-substep kernels (the full scan and the per-group hierarchy)."""
     g = load("rope_rope_lists")
     ropes = tangle(np.random.default_rng(7))
     b = rs.RopeBatch.from_positions(ropes, 0.117, 11473.0, 4.48e-3, 1 / 6000, 8, pinned=None)
     b.set_rope_contact(True, 0.0036, 0.0008, 0.025, 0.4, 1e7, 1.2, 3)
+    b.rope_broad_phase = broad_phase
+    assert b.rope_bvh_active == (broad_phase == "bvh")
     nv = b.num_vertices
     x, pads = g["x"], g["pads"]
     counts = np.zeros(nv, np.int32)
