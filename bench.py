@@ -519,7 +519,7 @@ def main():
                             "kernel": "k_cell (persistent frame kernel; the timed span includes the two per-frame precompute kernels "
                                       "launched ahead of it, < 1 % of it)", "kernel_ms": kern_ms / launches}
         fl = flops_per_vertex_substep(s.num_itr, planes=0 if s.planes is None else len(s.planes),
-                                      meshes=b.num_meshes)
+                                      meshes=len(s.meshes or ()))
         tf = fl * units / per_launch_s / 1e12
         line["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP32_PEAK_TFLOPS,
                         "flop_per_vertex_substep": fl,
