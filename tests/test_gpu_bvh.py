@@ -171,3 +171,21 @@ def test_ropesim_collisions_long_rope(rs):
         r.advance(1 / 60)
     assert np.isfinite(host(r.positions())).all()
     assert r.last_error == 0
+
+
+def test_broad_phase_setting_and_errors(rs):
+    """The setting reads back, an unknown mode raises ValueError, and without rope contact no hierarchy is
+    in use."""
+    ropes = _ropes(np.random.default_rng(1), [200, 200])
+    b = rs.RopeBatch.from_positions(ropes, 0.117, 11473.0, 4.48e-3, 1 / 6000, 8, pinned=None)
+    assert b.rope_broad_phase == "auto" and not b.rope_bvh_active
+    b.set_rope_contact(True, 0.0005, 0.0004, 0.025, 0.4, 1e7, 1.2, 2)
+    assert b.rope_bvh_active  # a 400-vertex group
+    b.rope_broad_phase = "scan"
+    assert b.rope_broad_phase == "scan" and not b.rope_bvh_active
+    with pytest.raises(ValueError):
+        b.rope_broad_phase = "lbvh"
+    assert b.rope_broad_phase == "scan"
+    b.rope_broad_phase = "bvh"
+    b.set_rope_contact(False, 0.0005, 0.0004, 0.025, 0.4, 1e7, 1.2, 2)
+    assert b.rope_broad_phase == "bvh" and not b.rope_bvh_active
