@@ -370,6 +370,21 @@ def scale_bench(frames, adj_frames):
     p1 = P1.clone().requires_grad_(True)
     tips = torch.as_tensor(s.offsets[1:] - 1, device="cuda", dtype=torch.long)
     target = torch.as_tensor(s.positions[s.offsets[1:] - 1], device="cuda")[None] + 0.05
+    # warm-up: a one-frame rollout and its adjoint allocate the frame history (its size does not depend on
+    # the frame count), so the timed rollout measures the steady state, as the config-4 leg does
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    xw = ropesim.diff.rollout(b, ea, ei, mass, drag, P0[:1], V0[:1], p1[:1], s.dt, s.substeps)
+    torch.cuda.synchronize()
+    w1 = time.perf_counter()
+    ((xw[:, tips] - target) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    w2 = time.perf_counter()
+    low.append(torch.cuda.mem_get_info()[0])
+    del xw
+    for g in (ea, ei, mass, drag, p1):
+        g.grad = None
+    b.reset_envs(np.arange(b.num_envs, dtype=np.int32))  # back to the construction state
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     x = ropesim.diff.rollout(b, ea, ei, mass, drag, P0, V0, p1, s.dt, s.substeps)
@@ -386,7 +401,8 @@ def scale_bench(frames, adj_frames):
                 "rollout_forward_ms_per_frame": 1e3 * (t3 - t2) / F, "adjoint_ms_per_frame": 1e3 * (t4 - t3) / F,
                 "adjoint_forward_ratio": (t4 - t3) / (t3 - t2),
                 "device_memory_in_use_peak_gb": (free0 - min(low)) / 1e9,
-                "timing": "first rollout (includes one-time allocations)"})
+                "warmup_one_frame_rollout_ms": 1e3 * (w1 - w0), "warmup_one_frame_adjoint_ms": 1e3 * (w2 - w1),
+                "timing": "one rollout after a one-frame warm-up rollout (which allocates the frame history)"})
     return out
 
 
