@@ -95,9 +95,11 @@ __device__ __forceinline__ void surface_contact_adj(const Params& P, f3 n, float
 
 // Adjoint of the surface terms of one free vertex (surfaces_at, YB/sim/contact.cuh:141-163): adds to x̄, d̄
 // and returns the adjoint of the contact stiffness k.
+// hits: the candidates pass 1 found in contact at this iteration (bit c); the others add nothing and are skipped.
 template <bool F>
 __device__ __forceinline__ float surfaces_adj(const Params& P, const Grid* grids, const Pose* envPoses, int ncand,
-                                              uint64_t cand, f3 p, f3 d, float k, f3 fb, f3 u, f3 dl, f3& xb, f3& db) {
+                                              uint64_t cand, uint32_t hits, f3 p, f3 d, float k, f3 fb, f3 u, f3 dl,
+                                              f3& xb, f3& db) {
     using M = V<F>;
     float kcb = 0.f;
     for (int pl = 0; pl < P.numPlanes; pl++) {
@@ -114,6 +116,7 @@ __device__ __forceinline__ float surfaces_adj(const Params& P, const Grid* grids
     if (P.numSdf > 0) {
         const int cnt = ncand < 0 ? P.numSdf : ncand;
         for (int c = 0; c < cnt; c++) {
+            if (ncand >= 0 && !((hits >> c) & 1u)) continue;
             const int m = ncand < 0 ? c : cand_get(cand, c);
             if (ncand < 0 && !probe_near_box(envPoses + m, p + d)) continue;  // as in surfaces_at
             const float4* s = reinterpret_cast<const float4*>(envPoses + m);
@@ -252,6 +255,7 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
             sQ[0][t] = stq(q);
         }
         __syncthreads();
+        uint64_t hitBits = 0;
         for (int it = 0; it < I; it++) {
             const float* dk = sD[it & 1];
             float* dn = sD[(it + 1) & 1];
@@ -264,8 +268,10 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
                     p1 = ld3s(sX, B, t + 1);
                     p1dx = ld3s(dk, B, t + 1);
                 }
-                vertex_terms<F>(P, D.grids, sPose, nc, cand, c.w, c.l, c.kS, c.fl, p0, d0, y, ldq(sQ[it & 1][t]), p1,
-                                p1dx, f, H, f2, H2);
+                uint32_t hit = 0;
+                vertex_terms<F, true>(P, D.grids, sPose, nc, cand, c.w, c.l, c.kS, c.fl, p0, d0, y,
+                                      ldq(sQ[it & 1][t]), p1, p1dx, f, H, f2, H2, &hit);
+                if (it < 8) hitBits |= (uint64_t)(hit & 0xFFu) << (8 * it);
                 st3s(sF2, B, t, f2);
 #pragma unroll
                 for (int m = 0; m < 6; m++) sF2[(3 + m) * B + t] = H2.m[m];
@@ -306,6 +312,10 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
                 A.hQ[hk(A, I, s, it + 1, li)] = stq(qo);
             }
             // the ring slots written here were last read two barriers ago
+        }
+        if (act) {
+            const uint64_t mk = (nc < 0 || I > 8) ? ~0ull : hitBits;
+            A.hM[hs(A, s, li)] = make_uint2((uint32_t)mk, (uint32_t)(mk >> 32));
         }
         __syncthreads();
         float padv = 0.f;
@@ -398,7 +408,7 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                 sPoseAll[m] = substep_pose(A.fmotion[(size_t)rb0 * P.numSdf + m], D.grids[m % P.numSdf], s, S, h);
         f3 X0 = mk3(0.f, 0.f, 0.f), X1 = X0, y = X0, vs = X0, vls = X0;
         int nc = 0;
-        uint64_t cand = 0;
+        uint64_t cand = 0, hitMask = ~0ull;
         if (act) {
             const float4 xs = A.hX[hs(A, s, li)];
             X0 = ld3(xs);
@@ -409,6 +419,8 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             vls = ld3(A.hVL[hs(A, s, li)]);
             const uint2 cc = A.hC[hs(A, s, li)];
             cand = (uint64_t)cc.x | ((uint64_t)cc.y << 32);
+            const uint2 mm = A.hM[hs(A, s, li)];
+            hitMask = (uint64_t)mm.x | ((uint64_t)mm.y << 32);
         }
         // end update: x' = X + d, v' = (d / h)(1 - drag h) on free vertices
         f3 aD = ax, aX = ax, aY = mk3(0.f, 0.f, 0.f);
@@ -563,7 +575,9 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                     gm += ((double)dot(y - d0, u) + (double)trG) / ((double)h * (double)h);
                     if (P.numPlanes > 0 || P.numSdf > 0) {
                         const float kc = Af::div(kap * kap, c.w);
-                        const float kcb = surfaces_adj<F>(P, D.grids, sPose, nc, cand, X0, d0, kc, u, u, dl, aX, aD);
+                        const uint32_t hits = k < 8 ? (uint32_t)(hitMask >> (8 * k)) & 0xFFu : 0xFFu;
+                        const float kcb =
+                            surfaces_adj<F>(P, D.grids, sPose, nc, cand, hits, X0, d0, kc, u, u, dl, aX, aD);
                         gm += (double)kcb * ((double)kap * (double)kap);
                     }
                 }
@@ -701,7 +715,7 @@ __global__ void k_adjoint_finish(Params P, Dev D, const float4* rhoUnit, const d
 }  // namespace
 
 size_t adjoint_history_bytes(int numItr) {
-    return 5 * sizeof(float4) + sizeof(uint2) + (size_t)(numItr + 1) * 2 * sizeof(float4) +
+    return 5 * sizeof(float4) + 2 * sizeof(uint2) + (size_t)(numItr + 1) * 2 * sizeof(float4) +
            (size_t)numItr * 9 * sizeof(float);
 }
 

@@ -27,6 +27,8 @@ struct AdjDev {
     float4 *hX, *hV, *hVL, *hY;  // [S][hn] substep inputs: x after the root program (w: broad-phase count bits),
                                  // v after the root program, last velocity before init, inertial target y
     uint2* hC;                   // [S][hn] packed broad-phase candidate ids
+    uint2* hM;                   // [S][hn] mesh candidates in contact: bit 8k + c for iteration k < 8 and
+                                 // candidate c (all bits set on scan-all substeps and past 8 iterations)
     float4 *hD, *hQ;             // [S][I + 1][hn] iterates d^k and frames q^k
     float* hH;                   // [S][I][9][hn] total Hessian (packed 6) and Newton direction H^-1 F (3)
     // adjoint of the state at the end of the frame on entry, at its start on exit [nv]

@@ -181,10 +181,12 @@ __device__ __forceinline__ bool probe_near_box(const Pose* P, f3 q) {
 // Broad-phase candidates are kept packed: 6 bits per mesh id, up to 8 ids in 64 bits.
 __device__ __forceinline__ int cand_get(uint64_t packed, int c) { return (int)((packed >> (6 * c)) & 63u); }
 
-// Planes, then candidate meshes (YB/sim/contact.cuh:141-163). `ncand` < 0: every mesh.
-template <bool F>
+// Planes, then candidate meshes (YB/sim/contact.cuh:141-163). `ncand` < 0: every mesh. HITS: bit c of *hits
+// is set when candidate c (ncand >= 0) is in contact.
+template <bool F, bool HITS = false>
 __device__ __forceinline__ void surfaces_at(const Params& P, const Grid* grids, const Pose* envPoses, int ncand,
-                                            uint64_t cand, f3 p, f3 d, float k, f3& f, s3& H) {
+                                            uint64_t cand, f3 p, f3 d, float k, f3& f, s3& H,
+                                            uint32_t* hits = nullptr) {
     for (int pl = 0; pl < P.numPlanes; pl++) {
         const f3 n = mk3(P.planes[pl][0], P.planes[pl][1], P.planes[pl][2]);
         const float gap = ((V<F>::dot(n, p) - P.planes[pl][3]) + V<F>::dot(n, d)) - P.cRadius;
@@ -199,6 +201,7 @@ __device__ __forceinline__ void surfaces_at(const Params& P, const Grid* grids, 
             f3 n, dRel;
             float gap;
             if (!sdf_contact<F>(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
+            if constexpr (HITS) *hits |= 1u << c;
             surface_contact<F>(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, f, H);
         }
     }
@@ -550,10 +553,11 @@ __device__ __forceinline__ void collision_term(const Params& P, float dmp, f3 p0
 
 // Inertia + surface terms of vertex i and the stretch term of segment (i, i+1)
 // (YB/sim/cosserat.cu:41-88). Rope-rope terms are added by the caller.
-template <bool F>
+template <bool F, bool HITS = false>
 __device__ __forceinline__ void vertex_terms(const Params& P, const Grid* grids, const Pose* envPoses, int ncand,
                                              uint64_t cand, float invMass, float lRest, float kStretch, uint32_t fl,
-                                             f3 p0, f3 d0, f3 y, q4 qi, f3 p1, f3 p1dx, f3& f, s3& H, f3& f2, s3& H2) {
+                                             f3 p0, f3 d0, f3 y, q4 qi, f3 p1, f3 p1dx, f3& f, s3& H, f3& f2, s3& H2,
+                                             uint32_t* hits = nullptr) {
     using A = Ar<F>;
     const float h = P.h;
     const float dmp = A::div(P.damping, h);
@@ -566,7 +570,7 @@ __device__ __forceinline__ void vertex_terms(const Params& P, const Grid* grids,
         f = A::rcp(h * h * invMass) * (y - d0);
         if (P.numPlanes > 0 || P.numSdf > 0) {
             const float kap = A::div(P.cKappa, h);
-            surfaces_at<F>(P, grids, envPoses, ncand, cand, p0, d0, A::div(kap * kap, invMass), f, H);
+            surfaces_at<F, HITS>(P, grids, envPoses, ncand, cand, p0, d0, A::div(kap * kap, invMass), f, H, hits);
         }
     }
     if (fl & HAS_NEXT) {

@@ -1857,7 +1857,8 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
                 A.hD = A.hY + Sn;
                 A.hQ = A.hD + Sn * (I + 1);
                 A.hC = reinterpret_cast<uint2*>(A.hQ + Sn * (I + 1));
-                A.hH = reinterpret_cast<float*>(A.hC + Sn);
+                A.hM = A.hC + Sn;
+                A.hH = reinterpret_cast<float*>(A.hM + Sn);
                 A.ax = b->d_adj;
                 A.av = A.ax + n;
                 A.avl = A.av + n;
