@@ -201,7 +201,9 @@ __device__ __forceinline__ void surfaces_at(const Params& P, const Grid* grids, 
             f3 n, dRel;
             float gap;
             if (!sdf_contact<F>(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
-            if constexpr (HITS) *hits |= 1u << c;
+            if constexpr (HITS) {
+                if (ncand >= 0) *hits |= 1u << c;  // listed candidates only (c < MAX_CAND)
+            }
             surface_contact<F>(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, f, H);
         }
     }
