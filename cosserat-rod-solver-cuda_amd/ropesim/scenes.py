@@ -8,7 +8,7 @@ order, so both sides see identical float32 inputs.
 
 Configurations (sizes scale down through the arguments for quick checks):
   config1  single rope, 33 vertices hanging from (0, 0, 1), root moving at 0.1 m/s along +x
-  config2  N ropes x 65 vertices, lower-hemisphere directions, root program, ground plane
+  config2  N ropes x 65 vertices, lower-hemisphere directions (tips above the floor), root program, ground plane
   config3  N ropes with V ~ U{16..256}, per-segment EA / EI, contact groups of 4 with rope-rope
            contact, 8 kinematic SDF meshes per environment on SE(3) random walks, ground plane
   config4  N ropes x 65 (config-1 material), plane + one static sphere SDF per rope
@@ -64,6 +64,8 @@ class Scene:
     gravity: tuple = (0.0, 0.0, -9.81)
     drag: float = 0.0
     damping: float = 0.0
+    drag_quad: float = 0.0         # quadratic normal drag [1/m]
+    drag_quad_t: float = 0.0       # quadratic tangential drag [1/m] (acts only with drag_quad > 0)
     pinned: np.ndarray | None = None
     fix_root_orientation: bool = False
     ea: np.ndarray | None = None   # float32 [nv] per segment at its first vertex
@@ -93,6 +95,9 @@ class Scene:
         b.set_gravity(np.asarray(self.gravity, np.float32))
         b.drag = self.drag
         b.damping = self.damping
+        if self.drag_quad > 0.0 or self.drag_quad_t > 0.0:
+            b.drag_quad = self.drag_quad
+            b.drag_quad_t = self.drag_quad_t
         if self.ea is not None:
             b.set_stretch_stiffnesses(self.ea)
         if self.ei is not None:
@@ -177,6 +182,21 @@ def _lower_hemisphere(rng, n):
     return d
 
 
+def _hanging_clear_of_floor(rng, roots, length, clearance=0.05):
+    """Lower-hemisphere directions, redrawn per rope until the straight rope's tip is at least `clearance`
+    above the plane z = 0: a rope that starts below the floor is thrown out of it by the penalty force in
+    its first substeps, which no configuration asks for."""
+    d = _lower_hemisphere(rng, len(roots))
+    for r in range(len(roots)):
+        for _ in range(1000):
+            if roots[r, 2] + length * d[r, 2] > clearance:
+                break
+            d[r] = _lower_hemisphere(rng, 1)[0]
+        else:
+            d[r] = np.array([1.0, 0.0, 0.0])
+    return d
+
+
 def config1(frames=60):
     """Single rope: 33 vertices straight down from (0, 0, 1), 0.03 m segments, rho 0.1, EA 500,
     EI 1e-4, h 1/3000 (50 substeps per 1/60 s frame), 8 iterations, linear drag 0.2, root at 0.1 m/s."""
@@ -191,9 +211,9 @@ def config2(n_ropes=1024, V=65, frames=120, seed=0):
     """N ropes x V vertices, 0.02 m segments along lower-hemisphere directions, rho 0.117, EA 11473,
     EI 4.48e-3, h 1/6000 (100 substeps), 8 iterations, vertex 0 pinned, root program, plane z = 0."""
     rng = np.random.default_rng(seed)
-    d = _lower_hemisphere(rng, n_ropes)
     p0 = np.column_stack([rng.uniform(-0.5, 0.5, (n_ropes, 2)), rng.uniform(0.8, 1.2, n_ropes)])
     seg = 0.02
+    d = _hanging_clear_of_floor(rng, p0, seg * (V - 1))
     pos = (p0[:, None, :] + seg * np.arange(V)[None, :, None] * d[:, None, :]).astype(np.float32)
     contact = dict(radius=0.00635, mu_s=0.5, mu_k=0.4, v_slip=1e-3, kappa=2.0)
     return Scene("config2", pos.reshape(-1, 3), (np.arange(n_ropes + 1) * V).astype(np.int32), 0.117, 11473.0,
@@ -282,9 +302,9 @@ def config4(n_ropes=256, V=65, frames=120, seed=4, grid_n=48):
     8 iterations, roots driven by the root program; plane z = 0 and one static sphere SDF (r 0.1 m on a
     grid_n^3 lattice) per rope, placed beside the rope's midpoint so the falling rope swings into it."""
     rng = np.random.default_rng(seed)
-    d = _lower_hemisphere(rng, n_ropes)
     p0 = np.column_stack([rng.uniform(-0.5, 0.5, (n_ropes, 2)), rng.uniform(0.8, 1.2, n_ropes)])
     seg = 0.02
+    d = _hanging_clear_of_floor(rng, p0, seg * (V - 1))
     pos = (p0[:, None, :] + seg * np.arange(V)[None, :, None] * d[:, None, :]).astype(np.float32)
     R = 0.005
     r = 0.1

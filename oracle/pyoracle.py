@@ -14,24 +14,31 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "librope_oracle.so")
-_lib = None
+# arith "ieee": every float operation correctly rounded and unfused (g++ -ffp-contract=off), the default;
+# "fma": the same source with multiply-adds contracted into fused ones and correctly rounded division and
+# square root, the arithmetic nvcc compiles the reference's kernels with by default (float32 only differs).
+_LIB_PATHS = {"ieee": os.path.join(_HERE, "_build", "librope_oracle.so"),
+              "fma": os.path.join(_HERE, "_build", "librope_oracle_fma.so")}
+_LIB_PATH = _LIB_PATHS["ieee"]
+_libs = {}
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with its Makefile (g++ -fopenmp -ffp-contract=off)."""
-    if force and os.path.exists(_LIB_PATH):
-        os.remove(_LIB_PATH)
+    """Compile the oracle with its Makefile (g++ -fopenmp; -ffp-contract=off, and the contracted variant)."""
+    if force:
+        for p in _LIB_PATHS.values():
+            if os.path.exists(p):
+                os.remove(p)
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _LIB_PATH
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(arith: str = "ieee"):
+    if arith not in _libs:
+        path = _LIB_PATHS[arith]
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(path)
         vp, dp, ip, fp, u8p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_uint8)
         L.orc_last_error.restype = C.c_char_p
         L.orc_create.restype = vp
@@ -61,8 +68,8 @@ def lib():
         L.orc_detect.argtypes = [vp, dp, dp, ip, ip, dp]
         L.orc_broadphase.argtypes = [vp, dp, dp, ip, ip]
         L.orc_get_poses.argtypes = [vp, dp]
-        _lib = L
-    return _lib
+        _libs[arith] = L
+    return _libs[arith]
 
 
 def _d(a):
@@ -80,11 +87,12 @@ class OracleError(RuntimeError):
     pass
 
 
-def _check(rc):
+def _check(rc, L=None):
+    L = lib() if L is None else L
     if rc == 1:
-        raise ValueError(lib().orc_last_error().decode())
+        raise ValueError(L.orc_last_error().decode())
     if rc != 0:
-        raise OracleError(lib().orc_last_error().decode())
+        raise OracleError(L.orc_last_error().decode())
 
 
 def flatten_ropes(positions):
@@ -99,10 +107,13 @@ def flatten_ropes(positions):
 
 
 class OracleBatch:
-    """CPU restatement of RopeBatch (deterministic sweeps) in float32 (prec=32) or float64 (prec=64)."""
+    """CPU restatement of RopeBatch (deterministic sweeps) in float32 (prec=32) or float64 (prec=64), with
+    unfused correctly rounded operations (arith="ieee") or contracted multiply-adds (arith="fma")."""
 
     def __init__(self, positions, density, k_stretch, k_bend, max_h, num_itr, fix_root_orientation=False,
-                 pinned=None, straight_rest=False, prec=64, offsets=None):
+                 pinned=None, straight_rest=False, prec=64, offsets=None, arith="ieee"):
+        self._L = lib(arith)
+        self.arith = arith
         if offsets is None:
             flat, off = flatten_ropes(positions)
         else:
@@ -117,19 +128,22 @@ class OracleBatch:
         else:
             pin = np.ascontiguousarray(np.asarray(pinned, bool).reshape(-1).astype(np.uint8))
         self.prec = prec
-        self._h = lib().orc_create(prec, self.nropes, off.ctypes.data_as(C.POINTER(C.c_int)),
+        self._h = self._L.orc_create(prec, self.nropes, off.ctypes.data_as(C.POINTER(C.c_int)),
                                    flat.ctypes.data_as(C.POINTER(C.c_float)), pin.ctypes.data_as(C.POINTER(C.c_uint8)),
                                    float(density), float(k_stretch), float(k_bend), float(max_h), int(num_itr),
                                    int(bool(fix_root_orientation)), int(bool(straight_rest)))
         if not self._h:
-            raise ValueError(lib().orc_last_error().decode())
+            raise ValueError(self._L.orc_last_error().decode())
         self.num_meshes = 0
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().orc_destroy(h)
+            self._L.orc_destroy(h)
             self._h = None
+
+    def _ck(self, rc):
+        _check(rc, self._L)
 
     # -- state -------------------------------------------------------------
     def get(self, what, width=None):
@@ -141,7 +155,7 @@ class OracleBatch:
             out = np.zeros(self.nropes * 64 * 6)
         else:
             out = np.zeros(self.nv * sizes.get(what, 1))
-        _check(lib().orc_get(self._h, what.encode(), _dp(out)))
+        self._ck(self._L.orc_get(self._h, what.encode(), _dp(out)))
         if what in sizes:
             return out.reshape(-1, sizes[what])
         if what == "wrenches":
@@ -169,10 +183,10 @@ class OracleBatch:
 
     def set_state(self, what, arr):
         a = _d(arr).reshape(-1)
-        _check(lib().orc_set_state(self._h, what.encode(), _dp(a)))
+        self._ck(self._L.orc_set_state(self._h, what.encode(), _dp(a)))
 
     def set_scalar(self, name, v):
-        _check(lib().orc_set_scalar(self._h, name.encode(), float(v)))
+        self._ck(self._L.orc_set_scalar(self._h, name.encode(), float(v)))
 
     def __setattr__(self, name, value):
         if name in ("drag", "damping", "drag_quad", "drag_quad_t", "accel_ratio", "num_itr"):
@@ -182,28 +196,28 @@ class OracleBatch:
 
     def set_gravity(self, g):
         g = np.asarray(g, np.float64)
-        _check(lib().orc_set_gravity(self._h, float(g[0]), float(g[1]), float(g[2])))
+        self._ck(self._L.orc_set_gravity(self._h, float(g[0]), float(g[1]), float(g[2])))
 
     def set_planes(self, planes, radius=0.0, mu_s=0.5, mu_k=None, v_slip=1e-3, kappa=2.0):
         p = _d(planes).reshape(-1, 4)
         if len(p) > 0:
             self._mu = (mu_s, mu_s if mu_k is None else mu_k)
-        _check(lib().orc_set_planes(self._h, len(p), _dp(p), radius, mu_s, mu_s if mu_k is None else mu_k, v_slip,
+        self._ck(self._L.orc_set_planes(self._h, len(p), _dp(p), radius, mu_s, mu_s if mu_k is None else mu_k, v_slip,
                                     kappa))
 
     def set_contact(self, radius, mu_s=0.5, mu_k=None, v_slip=1e-3, kappa=2.0):
         self._mu = (mu_s, mu_s if mu_k is None else mu_k)
-        _check(lib().orc_set_contact(self._h, radius, mu_s, mu_s if mu_k is None else mu_k, v_slip, kappa))
+        self._ck(self._L.orc_set_contact(self._h, radius, mu_s, mu_s if mu_k is None else mu_k, v_slip, kappa))
 
     def set_masses(self, m):
-        _check(lib().orc_set_vertex(self._h, 0, _dp(_d(m).reshape(-1))))
+        self._ck(self._L.orc_set_vertex(self._h, 0, _dp(_d(m).reshape(-1))))
 
     def set_stretch_stiffnesses(self, ea):
         """Per-segment EA on the flat vertex index (entry at a rope's last vertex ignored)."""
-        _check(lib().orc_set_vertex(self._h, 1, _dp(_d(ea).reshape(-1))))
+        self._ck(self._L.orc_set_vertex(self._h, 1, _dp(_d(ea).reshape(-1))))
 
     def set_bend_stiffnesses(self, ei):
-        _check(lib().orc_set_vertex(self._h, 2, _dp(_d(ei).reshape(-1))))
+        self._ck(self._L.orc_set_vertex(self._h, 2, _dp(_d(ei).reshape(-1))))
 
     def add_mesh(self, values, origin, spacing, pos=(0, 0, 0), quat=(0, 0, 0, 1), mu_s=None, mu_k=None):
         """Friction defaults to the contact material's, as in the reference (csrc/bindings.cpp:474-480)."""
@@ -214,7 +228,7 @@ class OracleBatch:
         nx, ny, nz = v.shape
         p, q = _d(pos).reshape(-1, 3), _d(quat).reshape(-1, 4)
         idx = C.c_int(0)
-        _check(lib().orc_add_mesh(self._h, v.ctypes.data_as(C.POINTER(C.c_float)), nx, ny, nz, _dp(_d(origin)),
+        self._ck(self._L.orc_add_mesh(self._h, v.ctypes.data_as(C.POINTER(C.c_float)), nx, ny, nz, _dp(_d(origin)),
                                   float(spacing), mu_s, mu_k, _dp(p), _dp(q), len(p),
                                   C.byref(idx)))
         self.num_meshes += 1
@@ -223,12 +237,12 @@ class OracleBatch:
     def set_mesh_poses(self, index, pos, quat, teleport=False):
         p, q = _d(pos).reshape(-1, 3), _d(quat).reshape(-1, 4)
         tele_all, flags = self._tele(teleport)
-        _check(lib().orc_set_mesh_targets(self._h, index, _dp(p), _dp(q), len(p), flags, tele_all))
+        self._ck(self._L.orc_set_mesh_targets(self._h, index, _dp(p), _dp(q), len(p), flags, tele_all))
 
     def set_mesh_poses_all(self, pos, quat, teleport=False):
         p, q = _d(pos).reshape(-1, 3), _d(quat).reshape(-1, 4)
         tele_all, flags = self._tele(teleport)
-        _check(lib().orc_set_mesh_targets(self._h, -1, _dp(p), _dp(q), len(p), flags, tele_all))
+        self._ck(self._L.orc_set_mesh_targets(self._h, -1, _dp(p), _dp(q), len(p), flags, tele_all))
 
     def _tele(self, teleport):
         if isinstance(teleport, (bool, np.bool_)):
@@ -237,7 +251,7 @@ class OracleBatch:
         return 0, self._tele_buf.ctypes.data_as(C.POINTER(C.c_uint8))
 
     def enable_contact_forces(self, on=True):
-        _check(lib().orc_enable_forces(self._h, int(bool(on))))
+        self._ck(self._L.orc_enable_forces(self._h, int(bool(on))))
 
     def contact_forces(self):
         return self.get("contact_forces")
@@ -247,7 +261,7 @@ class OracleBatch:
 
     def set_rope_contact(self, enabled, radius=0.0, barrier=0.0, k_collision=0.0, mu=0.0, k_friction=0.0,
                          detection_scaler=1.2, group_ropes=1):
-        _check(lib().orc_set_rope_contact(self._h, int(bool(enabled)), radius, barrier, k_collision, mu, k_friction,
+        self._ck(self._L.orc_set_rope_contact(self._h, int(bool(enabled)), radius, barrier, k_collision, mu, k_friction,
                                           detection_scaler, group_ropes))
 
     # -- stepping ----------------------------------------------------------
@@ -255,22 +269,22 @@ class OracleBatch:
         p, v = _d(root_pos).reshape(-1), _d(root_vel).reshape(-1)
         if root_pos_end is None:
             q = _d(root_q)
-            _check(lib().orc_step_const(self._h, _dp(p), _dp(v), _dp(q.reshape(-1) if q is not None else None),
+            self._ck(self._L.orc_step_const(self._h, _dp(p), _dp(v), _dp(q.reshape(-1) if q is not None else None),
                                         float(dt), int(substeps)))
         else:
             p1 = _d(root_pos_end).reshape(-1)
             q0 = _d(root_q)
             q1 = _d(root_q_end)
-            _check(lib().orc_step_program(self._h, _dp(p), _dp(v), _dp(p1),
+            self._ck(self._L.orc_step_program(self._h, _dp(p), _dp(v), _dp(p1),
                                           _dp(q0.reshape(-1) if q0 is not None else None),
                                           _dp(q1.reshape(-1) if q1 is not None else None), float(dt), int(substeps)))
 
     def step_free(self, dt, substeps):
-        _check(lib().orc_step_free(self._h, float(dt), int(substeps)))
+        self._ck(self._L.orc_step_free(self._h, float(dt), int(substeps)))
 
     def reset_envs(self, ids):
         ids = np.ascontiguousarray(np.asarray(ids, np.int32))
-        _check(lib().orc_reset_envs(self._h, ids.ctypes.data_as(C.POINTER(C.c_int)), len(ids)))
+        self._ck(self._L.orc_reset_envs(self._h, ids.ctypes.data_as(C.POINTER(C.c_int)), len(ids)))
 
     # -- function-level checks --------------------------------------------
     def detect(self, last_pos, pads):
@@ -278,7 +292,7 @@ class OracleBatch:
         counts = np.zeros(self.nv, np.int32)
         lists = np.zeros(self.nv * 128, np.int32)
         lim = np.zeros(self.nv)
-        _check(lib().orc_detect(self._h, _dp(lp), _dp(pd), counts.ctypes.data_as(C.POINTER(C.c_int)),
+        self._ck(self._L.orc_detect(self._h, _dp(lp), _dp(pd), counts.ctypes.data_as(C.POINTER(C.c_int)),
                                 lists.ctypes.data_as(C.POINTER(C.c_int)), _dp(lim)))
         return counts, lists.reshape(self.nv, 128), lim
 
@@ -286,11 +300,11 @@ class OracleBatch:
         p, d = _d(pos).reshape(-1), _d(dx).reshape(-1)
         counts = np.zeros(self.nv, np.int32)
         cand = np.zeros(self.nv * 8, np.int32)
-        _check(lib().orc_broadphase(self._h, _dp(p), _dp(d), counts.ctypes.data_as(C.POINTER(C.c_int)),
+        self._ck(self._L.orc_broadphase(self._h, _dp(p), _dp(d), counts.ctypes.data_as(C.POINTER(C.c_int)),
                                     cand.ctypes.data_as(C.POINTER(C.c_int))))
         return counts, cand.reshape(self.nv, 8)
 
     def poses(self):
         out = np.zeros(self.nropes * 64 * 19)
-        _check(lib().orc_get_poses(self._h, _dp(out)))
+        self._ck(self._L.orc_get_poses(self._h, _dp(out)))
         return out.reshape(self.nropes, 64, 19)

@@ -3,11 +3,11 @@ of the device ("fast": fused multiply-add and the hardware reciprocal / square r
 "ieee": every operation correctly rounded and unfused, the float32 oracle's arithmetic).
 
 Bars: positions within 1e-5 of rope length of the float64 oracle (per frame for the single-rope config,
-after one frame for gently driven batches). For the whipped batches, IEEE: within 1e-5 of the float32 oracle
-and no further from float64 than float32 arithmetic itself; fast: no further from float64 than float32
-arithmetic strays under rounding-level perturbations of its input. Rope-rope candidate lists, step limits
-and mesh broad-phase lists bit-exact given identical positions; the per-substep kernels and the persistent
-frame kernel bit-identical within each mode.
+after one frame for the batches). On ropes where float32 arithmetic itself (the float32 oracle, with and
+without contracted multiply-adds, and under rounding-level perturbations of its input) strays past half of
+that, the rope is held to twice that float32 floor. IEEE mode also within 1e-5 of the float32 oracle after one
+frame. Rope-rope candidate lists, step limits and mesh broad-phase lists bit-exact given identical positions;
+the per-substep kernels and the persistent frame kernel bit-identical within each mode.
 """
 import ctypes as C
 import os
@@ -123,6 +123,8 @@ def _small(which, frames=1):
         s = scenes.config2(n_ropes=16, V=65, frames=frames)
         if which == "config2-free":
             s.drive = "free"
+        if which == "config2-dragquad":  # the reference's own test values (tests/test_drag_quad.py:180)
+            s.drag_quad, s.drag_quad_t = 0.3, 0.01
         return s
     if which == "config3":
         return scenes.config3(n_ropes=8, frames=frames, vmax=96, grid_n=32)
@@ -132,17 +134,19 @@ def _small(which, frames=1):
 
 
 @pytest.mark.parametrize("fast", ARITH)
-@pytest.mark.parametrize("which", ["config2", "config3", "config5"])
+@pytest.mark.parametrize("which", ["config2", "config2-dragquad", "config3", "config5"])
 def test_per_substep_kernels_equal_persistent_kernel_bitwise(rs, which, fast):
     from ropesim import scenes
-    if which == "config2":
+    if which.startswith("config2"):
         s = scenes.config2(n_ropes=12, V=65, frames=3)
+        if which == "config2-dragquad":
+            s.drag_quad, s.drag_quad_t = 0.3, 0.01
     elif which == "config3":
         s = scenes.config3(n_ropes=8, frames=2, vmax=96, grid_n=24)
     else:
         s = scenes.config5(n_ropes=8, frames=2, grid_n=24)
     a = _run(rs, s, 0, s.frames, fast=fast)
-    need = {"config2": 65, "config3": 96 * 4, "config5": 96 * 4}[which]
+    need = {"config2": 65, "config2-dragquad": 65, "config3": 96 * 4, "config5": 96 * 4}[which]
     for var in range(rs.num_cell_variants()):
         blk, vpt, ok = rs.cell_variant_info(var)
         if not ok or blk * vpt < need:
@@ -153,55 +157,88 @@ def test_per_substep_kernels_equal_persistent_kernel_bitwise(rs, which, fast):
             assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), f"{get} differs (variant {var})"
 
 
-def _fp32_oracle(oracle, scene, k_scale=1.0):
+def _fp32_oracle(oracle, scene, k_scale=1.0, arith="ieee"):
     o32 = oracle.OracleBatch(scene.positions, scene.density, scene.k_stretch * k_scale, scene.k_bend, scene.max_h,
-                             scene.num_itr, prec=32, offsets=scene.offsets)
+                             scene.num_itr, pinned=scene.pinned, fix_root_orientation=scene.fix_root_orientation,
+                             prec=32, offsets=scene.offsets, arith=arith)
     scene.apply(o32)
     return o32
 
 
-def _one_frame_three_ways(rs, oracle, scene, nframes=1, fast=True, perturbed=0):
-    """Errors of the device against the float32 and float64 oracles, the float32 oracle's own gap to
-    float64, and the gaps of `perturbed` more float32 runs whose stretch stiffness is moved by k * 1e-7
-    relative (about one float32 ulp per k): the spread of float32 arithmetic on this scene."""
+def per_rope_err(scene, x, x_ref):
+    L = rope_lengths(scene)
+    err = np.linalg.norm(x.reshape(-1, 3) - x_ref.reshape(-1, 3), axis=1)
+    rope = np.repeat(np.arange(scene.num_ropes), np.diff(scene.offsets))
+    per = np.zeros(scene.num_ropes)
+    np.maximum.at(per, rope, err)
+    return per / L
+
+
+def _one_frame_four_ways(rs, oracle, scene, nframes=1, fast=True, perturbed=0):
+    """Per-rope errors (of rope length) of the device against the float32 and float64 oracles, and the float32
+    floor: how far float32 arithmetic itself lands from float64 on this scene, as the largest per-rope gap of
+    the float32 oracle, of the float32 oracle with contracted multiply-adds (the arithmetic nvcc compiles the
+    reference with by default) and of `perturbed` more float32 runs whose stretch stiffness is moved by
+    k * 1e-7 relative (about one float32 ulp per k)."""
     b, o64 = build_pair(rs, oracle, scene, 64, 1, fast)
     o32s = [_fp32_oracle(oracle, scene, 1.0 + 1e-7 * k) for k in range(perturbed + 1)]
+    ofma = _fp32_oracle(oracle, scene, arith="fma")
     for fi in scene.frame_inputs(nframes):
         scene.step(b, fi, to_dev)
-        scene.step(o64, fi)
-        for o in o32s:
+        for o in [o64, ofma] + o32s:
             scene.step(o, fi)
     x = host(b.positions())
-    gaps = [rel_err(scene, o.positions(), o64.positions()) for o in o32s]
-    return rel_err(scene, x, o32s[0].positions()), rel_err(scene, x, o64.positions()), gaps[0], max(gaps)
+    ref = o64.positions()
+    gaps = [per_rope_err(scene, o.positions(), ref) for o in [ofma] + o32s]
+    floor = np.max(gaps, axis=0)
+    return per_rope_err(scene, x, o32s[0].positions()), per_rope_err(scene, x, ref), gaps[1], gaps[0], floor
 
 
 @pytest.mark.parametrize("fast", ARITH)
-@pytest.mark.parametrize("which", ["config2", "config2-free", "config3", "config4", "config5"])
+@pytest.mark.parametrize("which", ["config2", "config2-free", "config2-dragquad", "config3", "config4", "config5"])
 def test_one_frame_vs_oracle(rs, oracle, which, fast):
-    """Stiff ropes (EA 11473 N on 2 cm segments) lose float32 digits in the stretch constraint: the float32
-    oracle itself differs from the float64 one by 2e-5..2e-3 of rope length after one frame (most where the
-    root program whips a rope to ~7 m/s, and where rope-rope and mesh contacts make the trajectory sensitive
-    to rounding). Bars, IEEE mode: within 1e-5 of rope length of the float32 oracle (identical arithmetic),
-    and no further from the float64 oracle than twice the float32 oracle's own gap (or 1e-5). Fast mode
-    (different rounding): no further from the float64 oracle than twice the largest gap of three float32
-    runs whose inputs differ at rounding level (or 1e-5). Config 4 (6 frames) and config 5 (8 frames, groups
-    of 4 in contact with 64 mesh slots) run long enough for the one-ulp differences of the transcendentals
-    (logf of the rope-rope barrier, the root program's acos / sin) to grow through the contacts: there the
-    IEEE mode is held, like the fast mode, to the float32 oracle's own distance from float64."""
+    """Bar: 1e-5 of rope length of the float64 oracle. Where float32 arithmetic itself (the float32 oracle, and
+    the same with the reference's contracted multiply-adds) lands further than half of that from float64 on
+    some rope, that rope is held to twice the float32 floor instead. Stiff ropes (EA up to 2e4 N on 1.5-2 cm
+    segments) lose float32 digits in the stretch constraint, most where the root program whips a rope to
+    several m/s, and rope-rope contacts (the log barrier) make a trajectory sensitive to rounding: on config 3
+    every float32 arithmetic lands 1e-3 of rope length from float64 after one frame. IEEE mode also stays
+    within 1e-5 of rope length of the float32 oracle (the same operations) after one frame. Config 4 (6 frames)
+    and config 5 (8 frames, groups of 4 in contact with 64 mesh slots) run long enough for one-ulp
+    differences of the transcendentals (logf of the barrier, the root program's acos / sin) to grow through
+    the contacts: there the IEEE mode is held to the float32 oracle's own distance from float64 too."""
     s = _small(which)
-    e32, e64, gap, spread = _one_frame_three_ways(rs, oracle, s, nframes=s.frames, fast=fast,
-                                                  perturbed=0 if not fast else 2)
-    print(f"\n  {which} fast {fast}: vs float32 oracle {e32:.2e}, vs float64 {e64:.2e}, float32 oracle vs "
-          f"float64 {gap:.2e} (largest of the perturbed float32 runs {spread:.2e})")
-    if fast:
-        assert e64 <= max(TOL_L, 2 * spread)
-    elif s.frames > 1:
-        assert e32 <= max(TOL_L, gap)
-        assert e64 <= max(TOL_L, 2 * gap)
-    else:
-        assert e32 < TOL_L
-        assert e64 <= max(TOL_L, 2 * gap)
+    e32, e64, gap, gap_fma, floor = _one_frame_four_ways(rs, oracle, s, nframes=s.frames, fast=fast,
+                                                         perturbed=2 if fast else 0)
+    print(f"\n  {which} fast {fast}: vs float32 oracle {e32.max():.2e}, vs float64 {e64.max():.2e}; float32 vs "
+          f"float64: oracle {gap.max():.2e}, contracted {gap_fma.max():.2e}, largest with perturbations "
+          f"{floor.max():.2e}; ropes above 1e-5: device {int((e64 > TOL_L).sum())}, float32 oracles "
+          f"{int((gap > TOL_L).sum())} / {int((gap_fma > TOL_L).sum())} of {s.num_ropes}")
+    bar = np.where(floor < 0.5 * TOL_L, TOL_L, np.maximum(TOL_L, 2 * floor))
+    assert (e64 <= bar).all(), (e64.max(), floor.max())
+    if not fast:
+        if s.frames > 1:
+            assert (e32 <= np.maximum(TOL_L, gap)).all()
+        else:
+            assert e32.max() < TOL_L
+
+
+@pytest.mark.parametrize("fast", ARITH)
+def test_config4_full_geometry_one_frame(rs, oracle, fast):
+    """The differentiation workload's positions after one frame: all 256 ropes x 65 vertices (config-1
+    material), 50 substeps at h 1/3000, 8 iterations, plane and one sphere SDF per rope on a 48^3 lattice,
+    root program. Eight of its ropes (the first eight) within 1e-5 of rope length of the float64 oracle; every
+    rope within 1e-5, or twice the float32 floor on the ropes where float32 arithmetic itself strays past
+    half of that."""
+    from ropesim import scenes
+    s = scenes.config4(n_ropes=256, frames=1, grid_n=48)
+    e32, e64, gap, gap_fma, floor = _one_frame_four_ways(rs, oracle, s, fast=fast)
+    print(f"\n  config4 256 x 65 fast {fast}: first 8 ropes {e64[:8].max():.2e}, all ropes {e64.max():.2e}; float32 "
+          f"vs float64: oracle {gap.max():.2e}, contracted {gap_fma.max():.2e}; ropes above 1e-5: device "
+          f"{int((e64 > TOL_L).sum())}, float32 oracles {int((gap > TOL_L).sum())} / {int((gap_fma > TOL_L).sum())}")
+    assert e64[:8].max() <= TOL_L
+    bar = np.where(floor < 0.5 * TOL_L, TOL_L, np.maximum(TOL_L, 2 * floor))
+    assert (e64 <= bar).all(), (e64.max(), floor.max())
 
 
 def test_program_lands_on_endpoint_and_slerp_end(rs):
