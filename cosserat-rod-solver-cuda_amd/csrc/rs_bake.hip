@@ -14,7 +14,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
+#include "ropesim_hip.h"
 #include "rs_kernels.hpp"
 #include "rs_math.hpp"
 
@@ -108,4 +111,119 @@ hipError_t launch_bake_mesh(const float4* tri, int ntri, const double* origin, d
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ the bake without allocation
+//
+// rs_bake_mesh_sdf_noalloc stages the triangle corners in this fixed array instead of a buffer of its own.
+// Entry 0 is the index-check word (an int in its first lane), entries 1 .. 3 * RS_BAKE_NOALLOC_MAX_TRIANGLES the
+// corners, three per triangle: 12 MiB of every GPU the library runs on, held until the process ends. Host code
+// reaches it only through the address hipGetSymbolAddress returns on each device.
+constexpr int kNoAllocCorners = 3 * RS_BAKE_NOALLOC_MAX_TRIANGLES;
+static __device__ __align__(256) float4 scratch[1 + kNoAllocCorners];
+
+__global__ void k_clear_word(int* w) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) w[0] = 0;
+}
+
+// Gather the corners of every triangle from device vertex and index arrays (one thread per corner). An index
+// outside [0, nv) marks the check word and reads vertex 0 instead, so every access stays in bounds.
+__global__ void k_bake_stage(const float* vtx, int nv, const int* tri, int ntri, float4* corners, int* bad) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 3LL * ntri) return;
+    int v = tri[e];
+    if (v < 0 || v >= nv) {
+        bad[0] = 1;
+        v = 0;
+    }
+    corners[e] = make_float4(vtx[3 * (size_t)v], vtx[3 * (size_t)v + 1], vtx[3 * (size_t)v + 2], 0.f);
+}
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+float4* g_scratch[kMaxDevices] = {};  // device address of `scratch` on each GPU, looked up once per device
+
+float4* scratch_address(int dev, std::string& err) {
+    if (dev < 0 || dev >= kMaxDevices) {
+        err = "bake_mesh_noalloc: device index beyond " + std::to_string(kMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("bake_mesh_noalloc: staging array address: ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch[dev] = static_cast<float4*>(p);
+    }
+    return g_scratch[dev];
+}
+
+}  // namespace
+
 }  // namespace rs
+
+extern "C" int rs_bake_mesh_sdf_noalloc(const float* vertices, int num_vertices, const int* triangles, int num_triangles,
+                                        const float* origin3, float spacing, int nx, int ny, int nz, float thickness,
+                                        float round, void* stream, float* out) {
+    using namespace rs;
+    if (!vertices || !triangles || !origin3 || !out) {
+        set_error_message("bake_mesh_noalloc: null pointer");
+        return RS_INVALID;
+    }
+    if (num_vertices < 3 || num_triangles < 1) {
+        set_error_message("bake_mesh_noalloc needs a triangle mesh");
+        return RS_INVALID;
+    }
+    if (num_triangles > RS_BAKE_NOALLOC_MAX_TRIANGLES) {
+        set_error_message(("bake_mesh_noalloc: " + std::to_string(num_triangles) + " triangles exceed the fixed "
+                           "staging array of " + std::to_string(RS_BAKE_NOALLOC_MAX_TRIANGLES) +
+                           "; use rs_bake_mesh_sdf").c_str());
+        return RS_INVALID;
+    }
+    if (!(spacing > 0.f) || nx < 2 || ny < 2 || nz < 2 ||
+        (long long)nx * ny * nz > 0x7fffffffLL * (long long)BAKE_BLOCK) {
+        set_error_message("bake_mesh_noalloc: bad lattice");
+        return RS_INVALID;
+    }
+    if (!(thickness >= 0.f) || !(round >= 0.f)) {
+        set_error_message("bake_mesh_noalloc: thickness and round must be >= 0");
+        return RS_INVALID;
+    }
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    std::string err;
+    float4* base = e == hipSuccess ? scratch_address(dev, err) : nullptr;
+    if (!base) {
+        set_error_message(e != hipSuccess ? (std::string("bake_mesh_noalloc: ") + hipGetErrorString(e)).c_str()
+                                          : err.c_str());
+        return RS_DEVICE;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int* bad = reinterpret_cast<int*>(base);
+    float4* corners = base + 1;
+    const long long nc = 3LL * num_triangles;
+    hipLaunchKernelGGL(k_clear_word, dim3(1), dim3(64), 0, st, bad);
+    hipLaunchKernelGGL(k_bake_stage, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, vertices, num_vertices,
+                       triangles, num_triangles, corners, bad);
+    e = hipGetLastError();
+    if (e == hipSuccess) {
+        const double o[3] = {origin3[0], origin3[1], origin3[2]};
+        e = launch_bake_mesh(corners, num_triangles, o, (double)spacing, nx, ny, nz, thickness > 0.f ? 1 : 0,
+                             (float)(0.5 * (double)thickness), round, out, st);
+    }
+    int flag = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, bad, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        set_error_message((std::string("bake_mesh_noalloc: ") + hipGetErrorString(e)).c_str());
+        return RS_DEVICE;
+    }
+    if (flag) {
+        set_error_message("triangle indices out of range");
+        return RS_INVALID;
+    }
+    return RS_OK;
+}

@@ -856,6 +856,8 @@ struct rs_batch {
 
 // ==================================================================== C ABI
 
+void rs::set_error_message(const char* msg) { g_msg = msg; }
+
 extern "C" {
 
 const char* rs_last_error_message(void) { return g_msg.c_str(); }

@@ -36,6 +36,8 @@ hipError_t launch_commit(Pose7* cur, const Pose7* tgt, uint8_t* tele, Pose* pose
 hipError_t launch_check_detect(const Params& P, const Dev& D, const Bvh* bvh, hipStream_t s);
 hipError_t launch_bake_mesh(const float4* tri, int ntri, const double* origin, double spacing, int nx, int ny, int nz,
                             int shell, float halfThick, float rnd, float* out, hipStream_t s);
+// Message returned by rs_last_error_message() on the calling thread.
+void set_error_message(const char* msg);
 hipError_t launch_check_broadphase(const Params& P, const Dev& D, const float4* xs, const float4* dxs, int* counts,
                                    int* cand, hipStream_t s);
 

@@ -155,6 +155,20 @@ int rs_bake_mesh_sdf(const float* vertices, int num_vertices, const int* triangl
                      const float* origin3, float spacing, int nx, int ny, int nz, float thickness, float round,
                      int device, float* out);
 
+/* No counterpart in the reference: rs_bake_mesh_sdf for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc; it synchronises its stream, so not code under graph capture). vertices [num_vertices*3],
+ * triangles [num_triangles*3] and out [nx*ny*nz] are DEVICE pointers; origin3 is host. The triangle corners are
+ * staged in a fixed device array of the library (12 MiB per GPU, held until the process ends), so at most
+ * RS_BAKE_NOALLOC_MAX_TRIANGLES triangles are accepted (RS_INVALID beyond). Same kernel and node arithmetic as
+ * rs_bake_mesh_sdf, so the grids are identical bit for bit. stream: a hipStream_t, or NULL for the null stream;
+ * the call blocks until the grid is written (it reads back the index check: RS_INVALID if an index is outside
+ * [0, num_vertices), and out is then unspecified). Calls that overlap on one GPU share the staging array and
+ * race; calls on different GPUs do not. */
+#define RS_BAKE_NOALLOC_MAX_TRIANGLES 262144
+int rs_bake_mesh_sdf_noalloc(const float* vertices, int num_vertices, const int* triangles, int num_triangles,
+                             const float* origin3, float spacing, int nx, int ny, int nz, float thickness, float round,
+                             void* stream, float* out);
+
 /* RopeSim.reset bookkeeping (csrc/bindings.cpp:303-316): time 0, previous substep = max_h, codes cleared. */
 int rs_reset_clock(rs_batch* b);
 
