@@ -33,42 +33,69 @@ std::vector<float> vec3(const farray& a, const char* name) {
     return {a.data()[0], a.data()[1], a.data()[2]};
 }
 
-// __cuda_array_interface__ of a float32, C-contiguous device array of exactly `want`
-// (same contract as the reference's parse_cai, csrc/bindings.cpp:587-617).
-const float* parse_cai(const py::object& obj, const std::vector<py::ssize_t>& want, const char* name) {
-    if (!py::hasattr(obj, "__cuda_array_interface__"))
-        throw py::value_error(std::string(name) +
-                              " must be a GPU array (e.g. a contiguous float32 torch tensor on cuda)");
-    py::dict d = obj.attr("__cuda_array_interface__").cast<py::dict>();
-    const auto ts = d["typestr"].cast<std::string>();
-    if (ts != "<f4") throw py::value_error(std::string(name) + " must be float32 (got " + ts + ")");
-    auto shape = d["shape"].cast<std::vector<py::ssize_t>>();
-    if (shape != want) {
-        std::string s = std::string(name) + " has wrong shape; expected (";
-        for (auto v : want) s += std::to_string(v) + ",";
-        throw py::value_error(s + ")");
-    }
-    py::object st = d.attr("get")("strides", py::none());
-    if (!st.is_none()) {
-        auto strides = st.cast<std::vector<py::ssize_t>>();
-        py::ssize_t expect = 4;
-        for (size_t i = shape.size(); i-- > 0;) {
-            if (shape[i] != 1 && strides[i] != expect) throw py::value_error(std::string(name) + " must be C-contiguous");
-            expect *= shape[i];
-        }
-    }
-    return reinterpret_cast<const float*>(d["data"].cast<py::tuple>()[0].cast<uintptr_t>());
+// ---- device arrays handed in through the __cuda_array_interface__ protocol (versions 2 and 3): a dict with
+// "data" = (address, read-only flag), "shape", "typestr" and "strides" in bytes (missing or None: C order).
+// The reference accepts float32 arrays of an exact shape in C order (csrc/bindings.cpp:587-617), and teleport
+// flags as one byte per environment (:128-172); wrong kind, shape, type or layout is a ValueError.
+struct DeviceArrayIn {
+    uintptr_t address = 0;
+    std::string typestr;
+    std::vector<py::ssize_t> shape;
+    std::vector<py::ssize_t> strides;  // empty: C order
+};
+
+DeviceArrayIn read_device_array(const py::object& obj) {
+    const py::dict cai = obj.attr("__cuda_array_interface__").cast<py::dict>();
+    DeviceArrayIn a;
+    a.address = cai["data"].cast<py::tuple>()[0].cast<uintptr_t>();
+    a.typestr = cai["typestr"].cast<std::string>();
+    a.shape = cai["shape"].cast<std::vector<py::ssize_t>>();
+    if (cai.contains("strides") && !cai["strides"].is_none()) a.strides = cai["strides"].cast<std::vector<py::ssize_t>>();
+    return a;
 }
-const uint8_t* parse_cai_bool(const py::object& obj, py::ssize_t n, const char* name) {
-    py::dict d = obj.attr("__cuda_array_interface__").cast<py::dict>();
-    const auto ts = d["typestr"].cast<std::string>();
-    auto shape = d["shape"].cast<std::vector<py::ssize_t>>();
-    py::object st = d.attr("get")("strides", py::none());
-    const bool contiguous = st.is_none() || n == 1 || st.cast<std::vector<py::ssize_t>>() == std::vector<py::ssize_t>{1};
-    if ((ts != "|b1" && ts != "|u1") || shape != std::vector<py::ssize_t>{n} || !contiguous)
-        throw py::value_error(std::string(name) + ": teleport must be a bool, a [num_envs] bool array, or a contiguous "
-                                                  "[num_envs] GPU bool array");
-    return reinterpret_cast<const uint8_t*>(d["data"].cast<py::tuple>()[0].cast<uintptr_t>());
+
+// C order for elements of `itemsize` bytes: every axis of extent > 1 steps over the product of the later extents.
+bool in_c_order(const DeviceArrayIn& a, py::ssize_t itemsize) {
+    if (a.strides.empty()) return true;
+    if (a.strides.size() != a.shape.size()) return false;
+    const size_t nd = a.shape.size();
+    std::vector<py::ssize_t> packed(nd);
+    py::ssize_t span = itemsize;
+    for (size_t k = nd; k > 0; k--) {
+        packed[k - 1] = span;
+        span *= a.shape[k - 1];
+    }
+    for (size_t k = 0; k < nd; k++)
+        if (a.shape[k] > 1 && a.strides[k] != packed[k]) return false;
+    return true;
+}
+
+std::string shape_text(const std::vector<py::ssize_t>& shape) {
+    std::string t = "(";
+    for (size_t k = 0; k < shape.size(); k++) t += (k ? ", " : "") + std::to_string(shape[k]);
+    return t + (shape.size() == 1 ? ",)" : ")");
+}
+
+// A float32 device array of exactly `want`, in C order.
+const float* device_f32(const py::object& obj, const std::vector<py::ssize_t>& want, const char* name) {
+    const std::string who(name);
+    if (!py::hasattr(obj, "__cuda_array_interface__"))
+        throw py::value_error(who + " must be a GPU array (e.g. a contiguous float32 torch tensor on cuda)");
+    const DeviceArrayIn a = read_device_array(obj);
+    if (a.typestr != "<f4") throw py::value_error(who + " must be float32 (got " + a.typestr + ")");
+    if (a.shape != want) throw py::value_error(who + " has wrong shape " + shape_text(a.shape) + "; expected " + shape_text(want));
+    if (!in_c_order(a, 4)) throw py::value_error(who + " must be C-contiguous");
+    return reinterpret_cast<const float*>(a.address);
+}
+
+// Teleport flags on the device: one bool or uint8 per environment, [n], contiguous.
+const uint8_t* device_flags(const py::object& obj, py::ssize_t n, const char* who) {
+    const DeviceArrayIn a = read_device_array(obj);
+    const bool byte = a.typestr == "|b1" || a.typestr == "|u1";
+    if (!byte || a.shape != std::vector<py::ssize_t>{n} || !in_c_order(a, 1))
+        throw py::value_error(std::string(who) + ": teleport must be a bool, a [num_envs] bool array, or a contiguous "
+                                                 "[num_envs] GPU bool array");
+    return reinterpret_cast<const uint8_t*>(a.address);
 }
 
 // Device buffer exposed to Python (torch.as_tensor(view) shares the memory).
@@ -77,24 +104,31 @@ struct CudaView {
     std::vector<py::ssize_t> shape;
 };
 
-void validate_rows(const float* p, const float* q, int rows, const char* who) {
+// ---- mesh poses from the host: pos [3] with quat [4] (one pose for every environment) or [N, 3] with [N, 4]
+void check_pose_rows(const float* pos, const float* quat, int rows, const char* who) {
     for (int r = 0; r < rows; r++) {
-        const float l = std::sqrt(q[4 * r] * q[4 * r] + q[4 * r + 1] * q[4 * r + 1] + q[4 * r + 2] * q[4 * r + 2] +
-                                  q[4 * r + 3] * q[4 * r + 3]);
-        if (!std::isfinite(l) || !(l > 1e-6f)) throw py::value_error(std::string(who) + ": quat must be finite with nonzero norm");
-        if (!std::isfinite(p[3 * r]) || !std::isfinite(p[3 * r + 1]) || !std::isfinite(p[3 * r + 2]))
+        const float* p = pos + 3 * (size_t)r;
+        const float* q = quat + 4 * (size_t)r;
+        double n2 = 0.0;
+        for (int c = 0; c < 4; c++) n2 += double(q[c]) * double(q[c]);
+        if (!std::isfinite(n2) || !(std::sqrt(n2) > 1e-6))
+            throw py::value_error(std::string(who) + ": quat must be finite with nonzero norm");
+        if (!(std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2])))
             throw py::value_error(std::string(who) + ": pos must be finite");
     }
 }
-int parse_poses(const farray& pos, const farray& quat, int N, const char* who) {
-    const bool p1 = pos.ndim() == 1 && pos.shape(0) == 3, pN = pos.ndim() == 2 && pos.shape(0) == N && pos.shape(1) == 3;
-    const bool q1 = quat.ndim() == 1 && quat.shape(0) == 4, qN = quat.ndim() == 2 && quat.shape(0) == N && quat.shape(1) == 4;
-    if (!((p1 && q1) || (pN && qN)))
+int pose_rows(const farray& pos, const farray& quat, int N, const char* who) {
+    auto rows_of = [](const farray& a, int width, int N) {
+        if (a.ndim() == 1 && a.shape(0) == width) return 1;
+        if (a.ndim() == 2 && a.shape(0) == N && a.shape(1) == width) return N;
+        return -1;
+    };
+    const int rp = rows_of(pos, 3, N), rq = rows_of(quat, 4, N);
+    if (rp < 0 || rp != rq)
         throw py::value_error(std::string(who) +
                               ": pos must be [3] and quat [4] (x, y, z, w), or [num_envs, 3] and [num_envs, 4]");
-    const int rows = p1 ? 1 : N;
-    validate_rows(pos.data(), quat.data(), rows, who);
-    return rows;
+    check_pose_rows(pos.data(), quat.data(), rp, who);
+    return rp;
 }
 
 // Owns one rs_batch.
@@ -197,7 +231,7 @@ public:
                  py::object mu_s, py::object mu_k) {
         if (values.ndim() != 3) throw py::value_error("sdf values must be [nx, ny, nz] with every dim >= 2");
         auto o = vec3(origin, "origin");
-        const int rows = parse_poses(pos, quat, N, "add_mesh");
+        const int rows = pose_rows(pos, quat, N, "add_mesh");
         float c[5];
         check(rs_get_contact(b(), c));
         const float ms = mu_s.is_none() ? c[1] : mu_s.cast<float>();
@@ -222,7 +256,7 @@ public:
         if (py::isinstance<py::bool_>(teleport)) {
             teleAll = teleport.cast<bool>() ? 1 : 0;
         } else if (py::hasattr(teleport, "__cuda_array_interface__")) {
-            tele = parse_cai_bool(teleport, N, who);
+            tele = device_flags(teleport, N, who);
             teleDev = true;
         } else {
             auto a = py::array_t<bool, py::array::c_style | py::array::forcecast>::ensure(teleport);
@@ -237,8 +271,8 @@ public:
         if (dp != dq) throw py::value_error(std::string(who) + ": pos and quat must both be host arrays or both GPU arrays");
         if (dp) {
             const int rows = index < 0 ? N * M : N;
-            const float* p = index < 0 ? parse_cai(pos, {N, M, 3}, "pos") : parse_cai(pos, {N, 3}, "pos");
-            const float* q = index < 0 ? parse_cai(quat, {N, M, 4}, "quat") : parse_cai(quat, {N, 4}, "quat");
+            const float* p = index < 0 ? device_f32(pos, {N, M, 3}, "pos") : device_f32(pos, {N, 3}, "pos");
+            const float* q = index < 0 ? device_f32(quat, {N, M, 4}, "quat") : device_f32(quat, {N, 4}, "quat");
             if (tele && !teleDev) {
                 // host flags with device poses: stage through the host path of the flags only
                 throw py::value_error(std::string(who) + ": GPU poses need teleport as a bool or a GPU bool array");
@@ -256,9 +290,9 @@ public:
                 throw py::value_error(std::string(who) +
                                       ": pos must be [num_envs, num_meshes, 3] and quat [num_envs, num_meshes, 4]");
             rows = N * M;
-            validate_rows(hp.data(), hq.data(), rows, who);
+            check_pose_rows(hp.data(), hq.data(), rows, who);
         } else {
-            rows = parse_poses(hp, hq, N, who);
+            rows = pose_rows(hp, hq, N, who);
         }
         check(rs_set_mesh_targets(b(), index, hp.data(), hq.data(), rows, tele, teleAll, 0));
     }
@@ -357,9 +391,9 @@ public:
             if (!o->is_none()) (py::hasattr(*o, "__cuda_array_interface__") ? dev : host) = true;
         if (dev && host) throw py::value_error("set_material: pass every array on the GPU or every array on the host");
         if (dev) {
-            const float* a = ea.is_none() ? nullptr : parse_cai(ea, {n}, "ea");
-            const float* c = ei.is_none() ? nullptr : parse_cai(ei, {n}, "ei");
-            const float* m = mass.is_none() ? nullptr : parse_cai(mass, {n}, "mass");
+            const float* a = ea.is_none() ? nullptr : device_f32(ea, {n}, "ea");
+            const float* c = ei.is_none() ? nullptr : device_f32(ei, {n}, "ei");
+            const float* m = mass.is_none() ? nullptr : device_f32(mass, {n}, "mass");
             check(rs_set_material(b(), a, c, m, 1));
             return;
         }
@@ -387,9 +421,9 @@ public:
     py::tuple adjoint(py::object dl_dx, py::object g_ea, py::object g_ei, py::object g_mass, py::object g_p0,
                       py::object g_v0, py::object g_p1) {
         const py::ssize_t F = rs_tape_frames(b()), n = nv, R = N;
-        const float* dl = parse_cai(dl_dx, {F, n, 3}, "dl_dx");
+        const float* dl = device_f32(dl_dx, {F, n, 3}, "dl_dx");
         auto out = [&](const py::object& o, const std::vector<py::ssize_t>& shape, const char* name) -> float* {
-            return o.is_none() ? nullptr : const_cast<float*>(parse_cai(o, shape, name));
+            return o.is_none() ? nullptr : const_cast<float*>(device_f32(o, shape, name));
         };
         float* ge = out(g_ea, {n}, "g_ea");
         float* gi = out(g_ei, {n}, "g_ei");
@@ -414,12 +448,12 @@ public:
         if (!rootsPinned)
             throw py::value_error("step() drives every rope's root; ropes built with a free root "
                                   "(from_positions pinned=...) advance with step_free()");
-        const float* p = parse_cai(root_pos, {N, 3}, "root_pos");
-        const float* v = parse_cai(root_vel, {N, 3}, "root_vel");
+        const float* p = device_f32(root_pos, {N, 3}, "root_pos");
+        const float* v = device_f32(root_vel, {N, 3}, "root_vel");
         const float* q = nullptr;
         if (!root_q.is_none()) {
             if (!fixRoot) throw py::value_error("root_q requires fix_root_orientation=True");
-            q = parse_cai(root_q, {N, 4}, "root_q");
+            q = device_f32(root_q, {N, 4}, "root_q");
         }
         if (root_pos_end.is_none()) {
             if (!root_q_end.is_none()) throw py::value_error("root_q_end requires root_pos_end (the motion-program path)");
@@ -432,11 +466,11 @@ public:
             check(rc);
             return;
         }
-        const float* p1 = parse_cai(root_pos_end, {N, 3}, "root_pos_end");
+        const float* p1 = device_f32(root_pos_end, {N, 3}, "root_pos_end");
         const float* q1 = nullptr;
         if (!root_q_end.is_none()) {
             if (!q) throw py::value_error("root_q_end requires root_q");
-            q1 = parse_cai(root_q_end, {N, 4}, "root_q_end");
+            q1 = device_f32(root_q_end, {N, 4}, "root_q_end");
         }
         stepped = true;
         int rc;
@@ -628,7 +662,8 @@ public:
             check(rs_set_rope_contact(b(), 1, 1e-4f, 8e-4f, 1e-5f, 0.1f, 5.f, 1.2f, 1));
         }
     }
-    // advance(dt, substeps=0): 0 -> ceil(dt / max_h) in float32 (YB/sim/step.cpp:170-175).
+    // advance(dt, substeps=0): 0 -> ceil(dt / max_h) in float32; substeps > 0 -> exactly that many of dt / substeps
+    // (csrc/bindings.cpp:270-295).
     void advance(float dt, int substeps) {
         if (substeps < 0) throw py::value_error("substeps must be >= 0");
         if (!(dt > 0)) return;
@@ -799,7 +834,7 @@ PYBIND11_MODULE(_ropesim, m) {
         .def("set_mesh_friction", &RopeSim::set_mesh_friction, py::arg("index"), py::arg("mu_s"), py::arg("mu_k") = py::none())
         .def("set_mesh_pose",
              [](RopeSim& s, int index, const farray& pos, const farray& quat, bool teleport) {
-                 parse_poses(pos, quat, 1, "set_mesh_pose");
+                 pose_rows(pos, quat, 1, "set_mesh_pose");
                  s.set_targets(index, pos, quat, py::bool_(teleport), "set_mesh_pose", s.num_meshes());
              },
              py::arg("index"), py::arg("pos"), py::arg("quat"), py::arg("teleport") = false)

@@ -145,7 +145,6 @@ struct rs_batch {
     std::vector<float*> gridMem;
     Grid* d_grids = nullptr;
     Pose7 *d_cur = nullptr, *d_tgt = nullptr;
-    uint8_t* d_tele = nullptr;
     Motion* d_motion = nullptr;
     float* d_stage = nullptr;
     uint8_t* d_stageTele = nullptr;
@@ -209,7 +208,7 @@ struct rs_batch {
         void* ptrs[] = {D.x, D.v, D.vlast, D.q, D.q2, D.dx, D.dx2, D.prm, D.qrest, D.ropeOf, d_roff, D.pad, D.maxStep,
                         D.ncol, D.coll, D.vlist, D.clist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.wrenchFx, D.outPos,
                         D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_hits,
-                        d_grids, d_cur, d_tgt, d_tele, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
+                        d_grids, d_cur, d_tgt, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
                         tape.poses, tape.motion, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
                         d_gdrag, d_mismatch, d_blk};
         for (void* p : ptrs)
@@ -467,7 +466,6 @@ struct rs_batch {
         d_grids = dalloc<Grid>(MAX_SDF, "grids");
         d_cur = dalloc<Pose7>(total, "sdf cur");
         d_tgt = dalloc<Pose7>(total, "sdf tgt");
-        d_tele = dalloc<uint8_t>(total, "sdf tele");
         D.motion = dalloc<Motion>(total, "sdf motion");
         D.poses = dalloc<Pose>(total, "sdf poses");
         d_stage = dalloc<float>(total * 7, "sdf stage");
@@ -480,12 +478,15 @@ struct rs_batch {
         std::vector<Pose7> ident(total, Pose7{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f});
         hcheck(hipMemcpy(d_cur, ident.data(), total * sizeof(Pose7), hipMemcpyHostToDevice), "upload sdf cur");
         hcheck(hipMemcpy(d_tgt, ident.data(), total * sizeof(Pose7), hipMemcpyHostToDevice), "upload sdf tgt");
-        hcheck(hipMemset(d_tele, 0, total), "memset tele");
+        std::vector<Motion> still(total, Motion{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f});
+        hcheck(hipMemcpy(const_cast<Motion*>(D.motion), still.data(), total * sizeof(Motion), hipMemcpyHostToDevice),
+               "upload sdf motion");
         hcheck(hipMemset(D.poses, 0, total * sizeof(Pose)), "memset poses");
         hcheck(hipMemset(D.ncand, 0xFF, nv * sizeof(int)), "memset ncand");
         hcheck(hipMemset(D.force, 0, 3 * nv * sizeof(float)), "memset force");
         hcheck(hipMemset(D.wrench, 0, total * 6 * sizeof(float)), "memset wrench");
     }
+    MeshSlots slots() { return MeshSlots{d_cur, d_tgt, const_cast<Motion*>(D.motion)}; }
     void upload_grids() {
         std::vector<Grid> all(MAX_SDF);
         std::memset(all.data(), 0, sizeof(Grid) * MAX_SDF);
@@ -495,8 +496,7 @@ struct rs_batch {
         hcheck(hipStreamSynchronize(stream), "upload grids");
     }
     void set_targets_dev(int index, const float* pos, const float* q, int rows, const uint8_t* tele, bool teleAll) {
-        hcheck(launch_scatter_targets(d_tgt, d_tele, index, pos, q, rows, (int)grids.size(), tele, teleAll ? 1 : 0, N,
-                                      stream),
+        hcheck(launch_mesh_targets(slots(), index, pos, q, rows, (int)grids.size(), tele, teleAll ? 1 : 0, N, stream),
                "set_mesh_poses");
         pendingSet = true;
     }
@@ -515,14 +515,9 @@ struct rs_batch {
     }
     void begin_step() {  // SdfStore::begin_step (csrc/sdf.cu:217-232)
         if (!meshesTouched) return;
-        if (pendingSet) {
-            hcheck(launch_stage_motion(d_cur, d_tgt, d_tele, const_cast<Motion*>(D.motion), N * MAX_SDF, stream),
-                   "stage motion");
-            sdfMoving = true;
-            moving = true;
-        } else {
-            sdfMoving = false;
-        }
+        // the motions were written with the targets; without a target call this step the meshes stay put
+        sdfMoving = pendingSet;
+        if (pendingSet) moving = true;
         if (!grids.empty()) hcheck(launch_fill_i32(D.ncand, -1, nv, stream), "reset candidates");
         if (forcesOn) {
             hcheck(launch_fill_f32(D.force, 0.f, 3L * nv, stream), "zero forces");
@@ -537,7 +532,7 @@ struct rs_batch {
         lastSubsteps = substeps > 0 ? substeps : 1;
         if (!meshesTouched) return;
         if (moving) {
-            hcheck(launch_commit(d_cur, d_tgt, d_tele, D.poses, d_grids, (int)grids.size(), -1, N, stream), "commit");
+            hcheck(launch_mesh_commit(slots(), D.poses, d_grids, (int)grids.size(), -1, N, stream), "commit");
             moving = false;
             pendingSet = false;
         }
@@ -1253,8 +1248,7 @@ int rs_add_mesh(rs_batch* b, const float* values, int nx, int ny, int nz, const 
         const int idx = (int)b->grids.size() - 1;
         const bool pending = b->pendingSet;  // other meshes' targets stay pending
         b->set_targets_host(idx, pos, quat, rows, nullptr, true);
-        hcheck(launch_commit(b->d_cur, b->d_tgt, b->d_tele, b->D.poses, b->d_grids, (int)b->grids.size(), idx, b->N,
-                             b->stream),
+        hcheck(launch_mesh_commit(b->slots(), b->D.poses, b->d_grids, (int)b->grids.size(), idx, b->N, b->stream),
                "commit mesh");
         hcheck(hipStreamSynchronize(b->stream), "add_mesh");
         b->pendingSet = pending;

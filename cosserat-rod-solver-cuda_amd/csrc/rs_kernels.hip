@@ -295,75 +295,72 @@ __global__ void k_reset(Params P, Dev D, const float4* initX, const float4* init
     }
 }
 
-// ------------------------------------------------------------------ mesh pose state machine (csrc/sdf.cu)
+// ------------------------------------------------------------------ mesh pose state machine
+//
+// Every (rope, mesh) slot holds its committed pose, the target it reaches at the end of the next step, and that
+// step's motion (start pose, displacement, rotation vector: the form of YB/YarnBall.h:152), which follows from
+// the two. Setting a target writes the target and the motion together; a teleported slot starts its motion at
+// the target, with no displacement or rotation. The commit after a step makes the targets the committed poses
+// and the motions static again. The reference keeps a teleport flag and derives the motions at the start of the
+// step (csrc/sdf.cu:37-87); the motions are the same, including the last-call-wins order of several target calls
+// before one step, since a target call reads the committed pose, which only the commit changes.
 
-__device__ __forceinline__ f3 rotvec(q4 q0, q4 q1) {  // sdf.cu:25-33
-    const f3 a = qvec(q1), b = -qvec(q0);
-    f3 v = (q1.w * b + q0.w * a) + cross(a, b);
-    float w = q1.w * q0.w - dot(a, b);
-    if (w < 0.f) {
-        v = -v;
-        w = -w;
-    }
-    const float s = len(v);
-    if (!(s > 1e-9f)) return mk3(0.f, 0.f, 0.f);
-    return v * (2.f * atan2f(s, w) / s);
+__device__ __forceinline__ Motion static_motion(const Pose7& p) {
+    return Motion{p.px, p.py, p.pz, p.qx, p.qy, p.qz, p.qw, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 }
 
-// Targets for mesh `index` (rows 1 or nropes) or, with index < 0, a dense [nropes, M] block (sdf.cu:37-51).
-__global__ void k_scatter_targets(Pose7* tgt, uint8_t* tele, int index, const float* pos, const float* q, int rows,
-                                  int M, const uint8_t* teleFlags, int teleAll, int N) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int per = index < 0 ? M : 1;
-    if (i >= N * per) return;
-    const int e = i / per, m = index < 0 ? i % per : index;
-    const int r = index < 0 ? i : (rows == 1 ? 0 : e);
-    q4 qq{q[4 * r], q[4 * r + 1], q[4 * r + 2], q[4 * r + 3]};
-    const float l = qlen(qq);
-    qq = l > 0.f ? q4{qq.x / l, qq.y / l, qq.z / l, qq.w / l} : q4{0.f, 0.f, 0.f, 1.f};
-    const size_t slot = (size_t)e * MAX_SDF + m;
-    tgt[slot] = Pose7{pos[3 * r], pos[3 * r + 1], pos[3 * r + 2], qq.x, qq.y, qq.z, qq.w, 0.f};
-    tele[slot] = teleAll ? 1 : (teleFlags ? teleFlags[e] : 0);
+// Motion from pose a to pose b. The rotation vector is the logarithm of the relative rotation q_b conj(q_a),
+// taken on the w >= 0 sheet, and zero once the vector part is below 1e-9 (csrc/sdf.cu:25-33).
+__device__ __forceinline__ Motion motion_between(const Pose7& a, const Pose7& b) {
+    q4 r = V<false>::qmul(q4{b.qx, b.qy, b.qz, b.qw}, qconj(q4{a.qx, a.qy, a.qz, a.qw}));
+    if (r.w < 0.f) r = q4{-r.x, -r.y, -r.z, -r.w};
+    const float s = V<false>::len(qvec(r));
+    const f3 rv = s > 1e-9f ? qvec(r) * (2.f * atan2f(s, r.w) / s) : mk3(0.f, 0.f, 0.f);
+    return Motion{a.px, a.py, a.pz, a.qx, a.qy, a.qz, a.qw, b.px - a.px, b.py - a.py, b.pz - a.pz, rv.x, rv.y, rv.z, 0.f};
 }
 
-// This frame's motion from the current pose to the target; teleported slots jump (sdf.cu:55-63).
-__global__ void k_stage_motion(Pose7* cur, const Pose7* tgt, uint8_t* tele, Motion* motion, int total) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    if (tele[i]) {
-        cur[i] = tgt[i];
-        tele[i] = 0;
+// Targets of rope e's slots: mesh `index` from row 0 (rows == 1) or row e, or with index < 0 every mesh from the
+// dense [nropes, M] rows. Quaternions are normalised, a zero one becoming the identity. One thread per rope.
+__global__ void k_mesh_targets(MeshSlots S, int index, const float* pos, const float* quat, int rows, int M,
+                               const uint8_t* teleFlags, int teleAll, int nropes) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nropes) return;
+    const bool tele = teleAll != 0 || (teleFlags != nullptr && teleFlags[e] != 0);
+    const bool every = index < 0;
+    for (int m = every ? 0 : index; m < (every ? M : index + 1); m++) {
+        const size_t row = every ? (size_t)e * M + m : (rows == 1 ? 0 : (size_t)e);
+        const float* qr = quat + 4 * row;
+        const float l = qlen(q4{qr[0], qr[1], qr[2], qr[3]});
+        Pose7 t{pos[3 * row], pos[3 * row + 1], pos[3 * row + 2], 0.f, 0.f, 0.f, 1.f, 0.f};
+        if (l > 0.f) {
+            t.qx = qr[0] / l;
+            t.qy = qr[1] / l;
+            t.qz = qr[2] / l;
+            t.qw = qr[3] / l;
+        }
+        const size_t slot = (size_t)e * MAX_SDF + m;
+        S.tgt[slot] = t;
+        S.motion[slot] = tele ? static_motion(t) : motion_between(S.cur[slot], t);
     }
-    const Pose7 c = cur[i], t = tgt[i];
-    const f3 rv = rotvec(q4{c.qx, c.qy, c.qz, c.qw}, q4{t.qx, t.qy, t.qz, t.qw});
-    motion[i] = Motion{c.px, c.py, c.pz, c.qx, c.qy, c.qz, c.qw, t.px - c.px, t.py - c.py, t.pz - c.pz,
-                       rv.x, rv.y, rv.z, 0.f};
 }
 
-// Commit targets as static current poses with their world AABBs (sdf.cu:67-87). index >= 0: that mesh only.
-__global__ void k_commit(Pose7* cur, const Pose7* tgt, uint8_t* tele, Pose* poses, const Grid* grids, int numSdf,
-                         int index, int N) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int slot, m;
-    if (index >= 0) {
-        if (i >= N) return;
-        m = index;
-        slot = i * MAX_SDF + index;
-    } else {
-        if (i >= N * MAX_SDF) return;
-        m = i % MAX_SDF;
-        slot = i;
-        if (m >= numSdf) return;
+// Commit rope e's targets as static committed poses with their world boxes (csrc/sdf.cu:67-87): mesh `index`, or
+// every mesh with index < 0. One thread per rope.
+__global__ void k_mesh_commit(MeshSlots S, Pose* poses, const Grid* grids, int numSdf, int index, int nropes) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nropes) return;
+    for (int m = index < 0 ? 0 : index; m < (index < 0 ? numSdf : index + 1); m++) {
+        const size_t slot = (size_t)e * MAX_SDF + m;
+        const Pose7 c = S.tgt[slot];
+        S.cur[slot] = c;
+        S.motion[slot] = static_motion(c);
+        const f3 p = mk3(c.px, c.py, c.pz);
+        const q4 q{c.qx, c.qy, c.qz, c.qw};
+        f3 lo, hi;
+        world_box(grids[m], p, q, lo, hi);
+        store_pose(poses + slot, Pose{p.x, p.y, p.z, q.x, q.y, q.z, q.w, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, lo.x, lo.y,
+                                      lo.z, hi.x, hi.y, hi.z, 0.f});
     }
-    cur[slot] = tgt[slot];
-    tele[slot] = 0;
-    const Pose7 c = cur[slot];
-    const f3 pos = mk3(c.px, c.py, c.pz);
-    const q4 q{c.qx, c.qy, c.qz, c.qw};
-    f3 lo, hi;
-    world_box(grids[m], pos, q, lo, hi);
-    store_pose(poses + slot, Pose{pos.x, pos.y, pos.z, q.x, q.y, q.z, q.w, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, lo.x, lo.y,
-                                  lo.z, hi.x, hi.y, hi.z, 0.f});
 }
 
 // ------------------------------------------------------------------ function-level check kernels
@@ -1184,23 +1181,17 @@ hipError_t launch_cell(int variant, const Params& P, const Dev& D, const Drive& 
     }
 }
 
-hipError_t launch_scatter_targets(Pose7* tgt, uint8_t* tele, int index, const float* pos, const float* q, int rows,
-                                  int M, const uint8_t* teleFlags, int teleAll, int N, hipStream_t s) {
-    const int threads = index < 0 ? N * M : N;
-    if (threads <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter_targets, dim3(nblk(threads, 256)), dim3(256), 0, s, tgt, tele, index, pos, q, rows, M,
-                       teleFlags, teleAll, N);
+hipError_t launch_mesh_targets(const MeshSlots& S, int index, const float* pos, const float* quat, int rows, int M,
+                               const uint8_t* teleFlags, int teleAll, int nropes, hipStream_t s) {
+    if (nropes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mesh_targets, dim3(nblk(nropes, 128)), dim3(128), 0, s, S, index, pos, quat, rows, M, teleFlags,
+                       teleAll, nropes);
     return hipGetLastError();
 }
-hipError_t launch_stage_motion(Pose7* cur, const Pose7* tgt, uint8_t* tele, Motion* motion, int total, hipStream_t s) {
-    hipLaunchKernelGGL(k_stage_motion, dim3(nblk(total, 256)), dim3(256), 0, s, cur, tgt, tele, motion, total);
-    return hipGetLastError();
-}
-hipError_t launch_commit(Pose7* cur, const Pose7* tgt, uint8_t* tele, Pose* poses, const Grid* grids, int numSdf,
-                         int index, int N, hipStream_t s) {
-    const int threads = index >= 0 ? N : N * MAX_SDF;
-    hipLaunchKernelGGL(k_commit, dim3(nblk(threads, 256)), dim3(256), 0, s, cur, tgt, tele, poses, grids, numSdf, index,
-                       N);
+hipError_t launch_mesh_commit(const MeshSlots& S, Pose* poses, const Grid* grids, int numSdf, int index, int nropes,
+                              hipStream_t s) {
+    if (nropes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mesh_commit, dim3(nblk(nropes, 128)), dim3(128), 0, s, S, poses, grids, numSdf, index, nropes);
     return hipGetLastError();
 }
 hipError_t launch_check_detect(const Params& P, const Dev& D, const Bvh* bvh, hipStream_t s) {

@@ -28,11 +28,19 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
                                 const Bvh* bvh = nullptr);
 hipError_t launch_cell(int variant, const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
                        int ncells, int S, unsigned long long* stats, hipStream_t st);
-hipError_t launch_scatter_targets(Pose7* tgt, uint8_t* tele, int index, const float* pos, const float* q, int rows,
-                                  int M, const uint8_t* teleFlags, int teleAll, int N, hipStream_t s);
-hipError_t launch_stage_motion(Pose7* cur, const Pose7* tgt, uint8_t* tele, Motion* motion, int total, hipStream_t s);
-hipError_t launch_commit(Pose7* cur, const Pose7* tgt, uint8_t* tele, Pose* poses, const Grid* grids, int numSdf,
-                         int index, int N, hipStream_t s);
+// Mesh pose slots [nropes * MAX_SDF]: committed pose, target of the next step, and that step's motion.
+struct MeshSlots {
+    Pose7* cur;
+    Pose7* tgt;
+    Motion* motion;
+};
+// Targets of mesh `index` (rows 1 or nropes of pos [3] / quat [4]) or, index < 0, of every mesh from dense
+// [nropes, M] rows; teleported ropes (teleAll, or teleFlags[rope] != 0) start the step at their target.
+hipError_t launch_mesh_targets(const MeshSlots& S, int index, const float* pos, const float* quat, int rows, int M,
+                               const uint8_t* teleFlags, int teleAll, int nropes, hipStream_t s);
+// Targets become the committed poses (mesh `index`, or every mesh with index < 0), with world boxes in `poses`.
+hipError_t launch_mesh_commit(const MeshSlots& S, Pose* poses, const Grid* grids, int numSdf, int index, int nropes,
+                              hipStream_t s);
 hipError_t launch_check_detect(const Params& P, const Dev& D, const Bvh* bvh, hipStream_t s);
 hipError_t launch_bake_mesh(const float4* tri, int ntri, const double* origin, double spacing, int nx, int ny, int nz,
                             int shell, float halfThick, float rnd, float* out, hipStream_t s);
