@@ -7,12 +7,16 @@ segments, per-segment EA / EI, contact groups of 4 with rope-rope contact, 8 kin
 of 1/6000 s per 1/60 s frame, 8 iterations. A step is one frame. Inputs (geometry, per-frame root
 programs and mesh targets) are resident on the GPU before the timed region.
 
-One JSON line: metric, value, ms_per_step, roofline of the persistent frame kernel (algorithmic bytes
-per launch over its HIP-event-timed duration on the solver stream), a VALU figure, and the CPU
-baseline (the float32 oracle on the host cores, a bounded sample of the same workload). The
-"differentiation" object is the second half of the metric, on the config-4 workload:
+One JSON line: metric, value, ms_per_step over frames 5..24 by default (the per-frame times in
+"frame_times"), roofline of the persistent frame kernel (algorithmic bytes per launch over its
+HIP-event-timed duration on the solver stream), a VALU figure, the same window in the other arithmetic mode
+("other_arithmetic"), and the CPU baseline (the oracle on the host cores in float32 and float64, a bounded
+sample of the same workload, with the device's one-frame position error against float64 on samples of every
+configuration). "plain" times the plain method on configs 1 and 2. The "differentiation" object is the second
+half of the metric, on the config-4 workload:
 - forward time of the per-substep kernels and of the persistent frame kernel;
-- a differentiable rollout's forward (with per-frame checkpoints) and adjoint time, and their ratio.
+- a differentiable rollout's forward (with per-frame checkpoints) and adjoint time, and their ratio, with the
+  tips tracking a trajectory made with hidden per-vertex EA / EI.
 The "readout" object continues the headline batch with the contact-force readout on (float wrench sums, then
 the reproducible fixed-point sums) and reports the time per frame of each against the readout off, and does the
 same on ropes resting on a slab mesh. The "bake_mesh" object times ropesim.sdf.bake_mesh on a 32768-triangle
@@ -78,30 +82,153 @@ def flops_per_vertex_substep(iters, planes=0, meshes=0):
     return f
 
 
+def per_rope_error(scene, x, ref):
+    """Largest vertex distance of each rope over the rope's length."""
+    import numpy as np
+    p = scene.positions.astype(np.float64)
+    off = scene.offsets
+    d = np.linalg.norm(np.asarray(x, np.float64).reshape(-1, 3) - np.asarray(ref, np.float64).reshape(-1, 3), axis=1)
+    out = np.zeros(scene.num_ropes)
+    for r in range(scene.num_ropes):
+        L = np.linalg.norm(np.diff(p[off[r]:off[r + 1]], axis=0), axis=1).sum()
+        out[r] = d[off[r]:off[r + 1]].max() / L
+    return out
+
+
+def device_frames(scene, fast, frames=1):
+    """Positions of the HIP batch after `frames` frames of the scene, in the given arithmetic mode."""
+    import torch
+
+    import ropesim
+    b = scene.build(ropesim)
+    b.fast_math = fast
+    for fi in scene.frame_inputs(frames):
+        scene.step(b, fi, lambda a: torch.as_tensor(a, device="cuda"))
+    return torch.as_tensor(b.positions(), device="cuda").cpu().numpy().reshape(-1, 3)
+
+
+def one_frame_errors(scene, o64, o32):
+    """Largest per-rope position error (of rope length) after one frame against the float64 oracle: the device
+    in fast and IEEE arithmetic, and the float32 oracle itself (the floor float32 arithmetic sets)."""
+    import numpy as np
+    ref = o64.positions()
+    out = {"ropes": scene.num_ropes}
+    for name, fast in (("device_fast", True), ("device_ieee", False)):
+        e = per_rope_error(scene, device_frames(scene, fast), ref)
+        out[name] = float(e.max())
+        out[name + "_ropes_above_1e-5"] = int((e > 1e-5).sum())
+    e32 = per_rope_error(scene, o32.positions(), ref)
+    out["float32_oracle"] = float(e32.max())
+    out["float32_oracle_ropes_above_1e-5"] = int((e32 > 1e-5).sum())
+    return out
+
+
 def cpu_baseline(sample_ropes, threads):
+    """The oracle (OpenMP on the host cores) on a bounded sample of the headline workload, float32 (the
+    reference's precision) and float64, one frame each; and, from the same runs, the one-frame position error of
+    the device against float64 there and on samples of the other configurations."""
     import numpy as np
 
     from oracle.pyoracle import OracleBatch
     from ropesim import scenes
+
+    def oracle_frame(scene, prec):
+        o = OracleBatch(scene.positions, scene.density, scene.k_stretch, scene.k_bend, scene.max_h, scene.num_itr,
+                        pinned=scene.pinned, prec=prec, offsets=scene.offsets)
+        scene.apply(o)
+        fi = scene.frame_inputs(1)[0]
+        t0 = time.perf_counter()
+        scene.step(o, fi)
+        dt = time.perf_counter() - t0
+        assert np.isfinite(o.positions()).all()
+        return o, dt
+
     s = scenes.config3(n_ropes=sample_ropes, frames=1)
-    o = OracleBatch(s.positions, s.density, s.k_stretch, s.k_bend, s.max_h, s.num_itr, prec=32, offsets=s.offsets)
-    s.apply(o)
-    fi = s.frame_inputs(1)[0]
-    t0 = time.perf_counter()
-    s.step(o, fi)
-    dt = time.perf_counter() - t0
-    assert np.isfinite(o.positions()).all()
+    o32, t32 = oracle_frame(s, 32)
+    o64, t64 = oracle_frame(s, 64)
+    units = s.num_vertices * s.substeps
     model = "unknown CPU"
     try:
         with open("/proc/cpuinfo") as fh:
             model = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
     except (OSError, StopIteration):
         pass
-    return {"value": s.num_vertices * s.substeps / dt, "unit": "vertex-substeps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"float32 oracle (OpenMP, {threads} threads on {model}), {sample_ropes} ropes "
-                      f"({s.num_vertices} vertices, {sample_ropes // 4} environments) of the same generator, 1 frame "
-                      f"of {s.substeps} substeps, {dt:.1f} s"}
+    errors = {"config3_sample": one_frame_errors(s, o64, o32)}
+    others = {"config1": scenes.config1(frames=1), "config2_sample": scenes.config2(n_ropes=64, frames=1),
+              "config4": scenes.config4(n_ropes=256, frames=1), "config5_sample": scenes.config5(n_ropes=64, frames=1)}
+    for name, sc in others.items():
+        errors[name] = one_frame_errors(sc, oracle_frame(sc, 64)[0], oracle_frame(sc, 32)[0])
+    return {"value": units / t32, "unit": "vertex-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"the oracle (OpenMP, {threads} threads on {model}) on {sample_ropes} ropes ({s.num_vertices} "
+                      f"vertices, {sample_ropes // 4} environments) of the headline generator, 1 frame of "
+                      f"{s.substeps} substeps: float32 {t32:.1f} s (value), float64 {t64:.1f} s",
+            "float64": {"value": units / t64, "unit": "vertex-substeps/s", "seconds": t64},
+            "one_frame_error_vs_float64": errors,
+            "one_frame_error_note": "largest per-rope vertex error over rope length after one frame; samples: "
+                                    "config 1 whole, config 2 64 ropes, config 3 the CPU sample, config 4 all 256 "
+                                    "ropes, config 5 64 ropes with its 64 mesh slots"}
+
+
+def plain_legs(frames1=60, frames2=120, warm=5):
+    """The plain method on configs 1 and 2 (the reference's throughput harness shape, examples/throughput.py:
+    a warm-up frame that captures the graph, then frames timed back to back): config 1 through the drop-in
+    RopeSim (set_root + advance(1/60, substeps=50) per frame) and config 2 through RopeBatch with the root
+    program, each with the per-substep kernels (graph-replayed) and with the persistent frame kernel."""
+    import numpy as np
+    import torch
+
+    import ropesim
+    from ropesim import scenes
+    out = {}
+    c1 = scenes.config1(frames=frames1 + warm)
+    row = {"workload": "config1: one rope of 33 vertices (seg 0.03 m, rho 0.1, EA 500, EI 1e-4), drag 0.2, root at "
+                       "0.1 m/s, RopeSim.set_root + advance(1/60, substeps=50) per frame", "frames": frames1}
+    for mode in (0, 1):
+        r = ropesim.RopeSim(num_verts=33, seg_len=0.03, dir=np.array([0, 0, -1], np.float32), density=0.1,
+                            k_stretch=500.0, k_bend=1e-4, max_h=1 / 3000)
+        r.set_gravity(np.array([0, 0, -9.81], np.float32))
+        r.drag = 0.2
+        r.damping = 0.0
+        r.set_dispatch(mode)
+        t0 = None
+        for f in range(frames1 + warm):
+            if f == warm:
+                t0 = time.perf_counter()
+            r.set_root(np.array([0.1 * f / 60, 0, 0], np.float32), np.array([0.1, 0, 0], np.float32))
+            r.advance(1 / 60, substeps=50)
+        wall = time.perf_counter() - t0
+        assert np.isfinite(np.asarray(r.positions())).all()
+        key = "per_substep_kernels" if mode == 0 else "persistent_kernel"
+        row[key + "_frames_per_s"] = frames1 / wall
+        row[key + "_vertex_substeps_per_s"] = 33 * 50 * frames1 / wall
+    out["config1"] = row
+    s = scenes.config2(frames=frames2 + warm)
+    fis = s.frame_inputs(frames2 + warm)
+    dev = [{k: torch.as_tensor(v, device="cuda") for k, v in fi.items() if v is not None} for fi in fis]
+    row = {"workload": f"config2: {s.num_ropes} ropes x 65 vertices, root program, plane, 100 substeps x 8 iterations "
+                       "per 1/60 s frame, RopeBatch.step", "frames": frames2}
+    for mode in (0, 1):
+        b = s.build(ropesim)
+        b.set_dispatch(mode, -1)
+        if mode == 1:
+            b.autotune(s.dt, s.substeps, True)
+        b.set_graph(True)
+        for f in range(warm):
+            s.step(b, dev[f])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(warm, warm + frames2):
+            s.step(b, dev[f])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        assert bool(torch.isfinite(torch.as_tensor(b.positions(), device="cuda")).all())
+        key = "per_substep_kernels" if mode == 0 else "persistent_kernel"
+        row[key + "_ms_per_frame"] = 1e3 * wall / frames2
+        row[key + "_vertex_substeps_per_s"] = s.num_vertices * s.substeps * frames2 / wall
+        del b
+    row["persistent_speedup"] = row["per_substep_kernels_ms_per_frame"] / row["persistent_kernel_ms_per_frame"]
+    out["config2"] = row
+    return out
 
 
 def resting_readout_bench(n_ropes, frames=5):
@@ -284,7 +411,12 @@ def differentiation_bench(n_ropes, frames):
     drag = torch.tensor(s.drag, device="cuda", requires_grad=True)
     p1 = P1.clone().requires_grad_(True)
     tips = torch.as_tensor(s.offsets[1:] - 1, device="cuda", dtype=torch.long)
-    target = torch.as_tensor(s.positions[s.offsets[1:] - 1], device="cuda")[None] + 0.05
+    # target trajectory: the same solver with hidden per-vertex EA / EI x U(0.5, 2); gradients are taken at the
+    # nominal material
+    ea_h, ei_h = (torch.as_tensor(a, device="cuda") for a in s.hidden_material())
+    with torch.no_grad():
+        target = ropesim.diff.rollout(b, ea_h, ei_h, mass.detach(), float(s.drag), P0, V0, P1, s.dt,
+                                      s.substeps)[:, tips].clone()
     fwds, bwds = [], []
     for rep in range(5):  # two warm-ups (first-use allocations of torch's caching allocator), three timed
         for t in (ea, ei, mass, drag, p1):
@@ -305,7 +437,7 @@ def differentiation_bench(n_ropes, frames):
     fwd, bwd = float(np.median(fwds)), float(np.median(bwds))
     return {"workload": f"config4: {n_ropes} ropes x 65 vertices (config-1 material), plane + one static sphere SDF "
                         f"(48^3) per rope, root program, {frames} frames x {s.substeps} substeps x {s.num_itr} "
-                        "iterations, tip-tracking loss",
+                        "iterations, tips tracking a hidden-material trajectory",
             "frames": frames, "vertex_substeps": units,
             "plain_kernels_ms_per_frame": 1e3 * t_plain / frames,
             "plain_kernels_graph_ms_per_frame": 1e3 * t_plain_graph / frames,
@@ -315,7 +447,10 @@ def differentiation_bench(n_ropes, frames):
             "rollout_forward_ms_per_frame": 1e3 * fwd / frames,
             "rollout_forward_vertex_substeps_per_s": units / fwd,
             "adjoint_ms_per_frame": 1e3 * bwd / frames,
-            "adjoint_forward_ratio": bwd / fwd, "timing": "median of 3 rollouts after 2 warm-ups"}
+            "adjoint_forward_ratio": bwd / fwd,
+            "adjoint_vs_persistent_forward_ratio": bwd / t_persist,
+            "loss": "sum over frames of squared tip distances to the trajectory of hidden per-vertex EA / EI x U(0.5, 2)",
+            "timing": "median of 3 rollouts after 2 warm-ups"}
 
 
 def scale_bench(frames, adj_frames):
@@ -406,21 +541,74 @@ def scale_bench(frames, adj_frames):
     return out
 
 
+def timed_window(scene, b, dev, W, K, kernel_timing=False):
+    """W untimed frames, then K timed frames back to back. Each frame with rope-rope contact returns after its
+    error word is read (one host synchronisation per frame, in the reference too), so the host clock between
+    frames gives per-frame times without adding a synchronisation. Returns the wall time of the K frames, the
+    per-frame times in ms and, with kernel_timing, the frame kernels' summed HIP-event time and launch count."""
+    import torch
+    for k in range(W):
+        scene.step(b, dev[k], lambda a: a)
+    torch.cuda.synchronize()
+    if kernel_timing:
+        b.set_kernel_timing(True)
+        b.kernel_time()
+    marks = [time.perf_counter()]
+    for k in range(W, W + K):
+        scene.step(b, dev[k], lambda a: a)
+        marks.append(time.perf_counter())
+    torch.cuda.synchronize()
+    marks[-1] = time.perf_counter()
+    kern = b.kernel_time() if kernel_timing else (0.0, 0)
+    if kernel_timing:
+        b.set_kernel_timing(False)
+    x = torch.as_tensor(b.positions(), device="cuda")
+    assert bool(torch.isfinite(x).all()), "non-finite state"
+    ms = [1e3 * (b_ - a_) for a_, b_ in zip(marks[:-1], marks[1:])]
+    return marks[-1] - marks[0], ms, kern
+
+
+def series(ms):
+    return {"first_ms": ms[0], "last_ms": ms[-1], "min_ms": min(ms), "max_ms": max(ms),
+            "ms": [round(v, 3) for v in ms]}
+
+
+def build_headline(args, frames, fast):
+    import torch
+
+    import ropesim
+    from ropesim import scenes
+    s = scenes.config3(n_ropes=args.ropes, frames=frames)
+    b = s.build(ropesim)
+    b.fast_math = fast
+    fis = s.frame_inputs(frames)
+    dev = [{k: torch.as_tensor(v, device="cuda") for k, v in fi.items() if v is not None} for fi in fis]
+    torch.cuda.synchronize()
+    tuned_ms = None
+    b.set_dispatch(args.mode, args.variant if args.mode == 1 else -1)
+    if args.mode == 1 and args.variant < 0:
+        tuned_ms = b.autotune(s.dt, s.substeps, True)
+    b.set_graph(not args.no_graph)  # the frame's kernel sequence is captured once and replayed
+    return s, b, dev, tuned_ms
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--ropes", type=int, default=4096)
     ap.add_argument("--mode", type=int, default=1, help="0: per-substep kernels, 1: persistent frame kernel")
     ap.add_argument("--variant", type=int, default=-1, help="frame-kernel variant (-1: autotune)")
     ap.add_argument("--no-graph", action="store_true", help="launch each frame's kernels directly instead of "
                     "replaying the frame captured as a hipGraph")
     ap.add_argument("--ieee", action="store_true",
-                    help="correctly rounded, unfused arithmetic (the float32 oracle's) instead of the default fast "
-                         "arithmetic (fused multiply-add, hardware reciprocal / square root / log)")
-    ap.add_argument("--cpu-sample-ropes", type=int, default=768, help="config-3 ropes timed on the CPU oracle (about 10 s)")
+                    help="headline in correctly rounded, unfused arithmetic (the float32 oracle's) instead of the "
+                         "default fast arithmetic (fused multiply-add, hardware reciprocal / square root / log)")
+    ap.add_argument("--no-ieee-leg", action="store_true", help="skip the headline window rerun in the other arithmetic")
+    ap.add_argument("--cpu-sample-ropes", type=int, default=512, help="config-3 ropes timed on the CPU oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the config-1 / config-2 plain-method legs")
     ap.add_argument("--diff-ropes", type=int, default=256, help="config-4 ropes of the differentiation leg")
     ap.add_argument("--diff-frames", type=int, default=120, help="config-4 frames (0 skips the leg)")
     ap.add_argument("--scale-frames", type=int, default=3, help="config-5 forward frames timed (0 skips the leg)")
@@ -436,26 +624,12 @@ def main():
         sys.exit("bench.py measures one GPU: the rope batch runs on a single device (across GPUs the batch would "
                  "shard by environment, one contiguous range of environments per GPU); run with --gpus 1")
 
-    import numpy as np
     import torch
-
-    import ropesim
-    from ropesim import scenes
 
     t_setup = time.perf_counter()
     K, W = args.steps, args.warmup
-    RK = 0 if args.no_readout or K < 2 else K  # frames per readout mode (one warm-up), after the headline frames
-    s = scenes.config3(n_ropes=args.ropes, frames=K + W + 2 * RK)
-    b = s.build(ropesim)
-    b.fast_math = not args.ieee
-    fis = s.frame_inputs(K + W + 2 * RK)
-    dev = [{k: torch.as_tensor(v, device="cuda") for k, v in fi.items() if v is not None} for fi in fis]
-    torch.cuda.synchronize()
-    tuned_ms = None
-    b.set_dispatch(args.mode, args.variant if args.mode == 1 else -1)
-    if args.mode == 1 and args.variant < 0:
-        tuned_ms = b.autotune(s.dt, s.substeps, True)
-    b.set_graph(not args.no_graph)  # the frame's kernel sequence is captured once and replayed
+    RK = 0 if args.no_readout or K < 2 else min(K, 10)  # frames per readout mode (one warm-up), after the headline
+    s, b, dev, tuned_ms = build_headline(args, K + W + 2 * RK, not args.ieee)
     setup_s = time.perf_counter() - t_setup
 
     if args.copy_calibration_mb > 0:  # a plain copy of known size: reads and writes n bytes each
@@ -465,20 +639,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    for k in range(W):
-        s.step(b, dev[k], lambda a: a)
-    torch.cuda.synchronize()
-    b.set_kernel_timing(True)
-    b.kernel_time()
-    t0 = time.perf_counter()
-    for k in range(W, W + K):
-        s.step(b, dev[k], lambda a: a)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms, launches = b.kernel_time()
-    b.set_kernel_timing(False)
-    x = torch.as_tensor(b.positions(), device="cuda")
-    assert bool(torch.isfinite(x).all()), "non-finite state"
+    wall, frame_ms, (kern_ms, launches) = timed_window(s, b, dev, W, K, kernel_timing=True)
 
     readout = None
     if RK > 0:  # the same batch continues with the contact-force readout on: float sums, then fixed-point sums
@@ -500,9 +661,12 @@ def main():
         for name in ("float", "fixed_point"):
             readout[name + "_overhead"] = readout[name + "_ms_per_frame"] / readout["off_ms_per_frame"] - 1.0
         readout["resting"] = resting_readout_bench(args.ropes)
+    dispatch, captures = list(b.dispatch), b.graph_captures
+    del b
 
     units = s.num_vertices * s.substeps          # vertex-substeps per frame
     value = units * K / wall
+    arith = "ieee" if args.ieee else "fast"
     line = {
         "metric": "vertex-substeps/s (4096 uneven ropes, contact on)",
         "value": value,
@@ -520,9 +684,10 @@ def main():
                                f"groups of 4 with rope-rope contact, 8 SDF meshes/env (64^3), plane, root program, "
                                f"{s.substeps} substeps x {s.num_itr} iterations per 1/60 s frame",
                    "ropes": args.ropes, "vertices": s.num_vertices, "substeps": s.substeps, "iterations": s.num_itr,
-                   "dispatch": list(b.dispatch), "arithmetic": "ieee" if args.ieee else "fast",
-                   "graph_captures": b.graph_captures,
-                   "autotune_ms": tuned_ms, "setup_s": round(setup_s, 1)},
+                   "dispatch": dispatch, "arithmetic": arith, "graph_captures": captures,
+                   "autotune_ms": tuned_ms, "setup_s": round(setup_s, 1),
+                   "timed_frames": f"frames {W}..{W + K - 1} of the run (frames slow as the ropes tangle)"},
+        "frame_times": series(frame_ms),
     }
     if launches > 0:
         per_launch_s = kern_ms / launches * 1e-3
@@ -541,8 +706,21 @@ def main():
                         "flop_per_vertex_substep": fl,
                         "note": "flops counted on the oracle (oracle/opcount.cpp): solver, plane tests and mesh broad "
                                 "phase; mesh-candidate and rope-rope pair terms excluded"}
+    if not args.no_ieee_leg:
+        other = "fast" if args.ieee else "ieee"
+        s2, b2, dev2, _ = build_headline(args, K + W, args.ieee)
+        wall2, ms2, _ = timed_window(s2, b2, dev2, W, K)
+        line["other_arithmetic"] = {"arithmetic": other, "value": units * K / wall2, "ms_per_step": 1e3 * wall2 / K,
+                                    "frame_times": series(ms2),
+                                    "note": "the same workload and frame window in the other arithmetic mode: ieee "
+                                            "is every operation correctly rounded and unfused (the float32 oracle's "
+                                            "operations); fast is fused multiply-add with the hardware reciprocal, "
+                                            "square root and log"}
+        del b2
     if readout is not None:
         line["readout"] = readout
+    if not args.no_plain:
+        line["plain"] = plain_legs()
     if not args.no_bake:
         line["bake_mesh"] = bake_bench()
     if args.bvh_sizes:

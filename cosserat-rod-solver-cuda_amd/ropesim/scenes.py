@@ -122,6 +122,15 @@ class Scene:
                                                    offsets=self.offsets, **kw)
         return self.apply(b)
 
+    def hidden_material(self, seed=7, lo=0.5, hi=2.0):
+        """Per-vertex EA and EI (float32 [nv]) scaled by U(lo, hi) from the scene's own: the material a target
+        trajectory is generated with when the scene's material is to be identified from it."""
+        rng = np.random.default_rng(seed)
+        nv = self.num_vertices
+        ea = np.full(nv, self.k_stretch, np.float64) if self.ea is None else self.ea.astype(np.float64)
+        ei = np.full(nv, self.k_bend, np.float64) if self.ei is None else self.ei.astype(np.float64)
+        return (ea * rng.uniform(lo, hi, nv)).astype(np.float32), (ei * rng.uniform(lo, hi, nv)).astype(np.float32)
+
     # ---- driving -------------------------------------------------------------------------
     def frame_inputs(self, nframes=None):
         """Per-frame float32 arrays: dict of p0 [N,3], v0 [N,3], p1 [N,3] and, with meshes, mesh_pos /
@@ -300,7 +309,7 @@ def config4(n_ropes=256, V=65, frames=120, seed=4, grid_n=48):
     """Differentiation form: n_ropes x V with config-1 material (rho 0.1 kg/m, EA 500 N, EI 1e-4 N m^2, linear
     drag 0.2 1/s) on 0.02 m segments along lower-hemisphere directions, h 1/3000 (50 substeps per 1/60 s frame),
     8 iterations, roots driven by the root program; plane z = 0 and one static sphere SDF (r 0.1 m on a
-    grid_n^3 lattice) per rope, placed beside the rope's midpoint so the falling rope swings into it."""
+    grid_n^3 lattice) per rope whose top lies 2 cm below the rope's midpoint, so the falling rope lands on it."""
     rng = np.random.default_rng(seed)
     p0 = np.column_stack([rng.uniform(-0.5, 0.5, (n_ropes, 2)), rng.uniform(0.8, 1.2, n_ropes)])
     seg = 0.02
@@ -315,7 +324,9 @@ def config4(n_ropes=256, V=65, frames=120, seed=4, grid_n=48):
     side = np.cross(d, np.array([0.0, 0.0, 1.0]))
     ns = np.linalg.norm(side, axis=1, keepdims=True)
     side = np.where(ns > 1e-3, side / np.maximum(ns, 1e-30), np.array([1.0, 0.0, 0.0]))
-    c = pos[:, V // 2].astype(np.float64) + (r + 0.03) * side
+    # the sphere's top 2 cm below the rope's midpoint: the middle of the falling rope lands on it within a few
+    # frames while the rest swings on
+    c = pos[:, V // 2].astype(np.float64) - np.array([0.0, 0.0, r + R + 0.02])
     c[:, 2] = np.maximum(c[:, 2], r + 0.01)
     for i in range(n_ropes):  # keep the start clear of the sphere
         while np.linalg.norm(pos[i].astype(np.float64) - c[i], axis=1).min() < r + 0.02:

@@ -111,3 +111,15 @@ def test_device_mesh_broadphase_matches_golden(rs):
                                         counts.ctypes.data_as(ip), cand.ctypes.data_as(ip)) == 0
     np.testing.assert_array_equal(counts, g["counts"])
     np.testing.assert_array_equal(cand.reshape(nv, 8), g["cand"])
+
+
+def test_config4_gradient_references_agree_with_central_differences():
+    """The float64 autograd gradients of the differentiation workload (tests/golden/config4_gradients.npz)
+    project onto the float64 oracle's central differences along each family's direction within 1e-3."""
+    import os
+    ref = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config4_gradients.npz"))
+    for k in ("ea", "ei", "m", "drag", "p0", "v0", "p1"):
+        ad = float(np.sum(ref["g64_" + k] * ref["dir_" + k]))
+        fd = float(ref["fd_" + k])
+        assert abs(ad - fd) <= 1e-3 * abs(fd), (k, ad, fd)
+    assert ref["tgt"].shape == (6, 8, 3) and np.isfinite(ref["g64_ea"]).all()

@@ -271,3 +271,44 @@ def test_passes_with_different_packings_match_single_rope_workgroups(rs):
     for name in ("p0", "v0", "p1"):
         assert np.array_equal(big[name][:, :k].view(np.uint32), small[name].view(np.uint32)), name
     assert np.abs(small["ei"]).max() > 0
+
+
+def _config4_reference():
+    import os
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config4_gradients.npz"))
+
+
+@pytest.mark.parametrize("fast", [True, False], ids=["fast", "ieee"])
+def test_config4_workload_gradients(rs, fast):
+    """The differentiation workload itself: 8 ropes of config 4 (65 vertices, 50 substeps of 1/3000 s, 8
+    iterations, plane and a 48^3 sphere lattice per rope, root program), 6 frames, the tips tracking a target
+    trajectory made with hidden per-vertex EA / EI x U(0.5, 2), gradients at the nominal material. References
+    (tests/golden/make_config4_grads.py): float64 torch autograd of the transcription, float32 autograd of it at
+    two rounding-level EA perturbations, and central differences of the float64 oracle along each family's
+    gradient. Bar per family: 1e-3 relative, or twice what float32 arithmetic alone moves the family (the
+    largest float32 autograd gap); central differences the same, or twice their own gap to float64 autograd."""
+    from ropesim import scenes
+    ref = _config4_reference()
+    s = scenes.config4(n_ropes=8, frames=6, grid_n=48)
+    b = s.build(rs)
+    b.fast_math = fast
+    mat = (ref["ea"], ref["ei"], ref["m"])
+    prog = (ref["p0"], ref["v0"], ref["p1"])
+    g, loss, _ = hip_grads(rs, s, b, mat, float(ref["drag"]), prog, ref["tips"], ref["tgt"])
+    assert abs(loss - float(ref["loss"])) <= 1e-3 * float(ref["loss"]), (loss, float(ref["loss"]))
+    rows = []
+    for k in ("ea", "ei", "m", "drag", "p0", "v0", "p1"):
+        g64 = ref["g64_" + k]
+        floor = max(rel(ref[f"g32_{j}_" + k], g64) for j in (0, 1))
+        e = rel(g[k], g64)
+        v, fd = ref["dir_" + k], float(ref["fd_" + k])
+        ad = float(np.sum(g[k] * v))
+        fd_floor = max([abs(float(np.sum(ref[f"g32_{j}_" + k] * v)) - fd) / abs(fd) for j in (0, 1)] +
+                       [abs(float(np.sum(g64 * v)) - fd) / abs(fd)])
+        rows.append((k, e, floor, abs(ad - fd) / abs(fd), fd_floor))
+    print(f"\n  config4 gradients, fast {fast}: family: vs float64 autograd (float32 gap) | vs central difference "
+          f"(float32 / float64-autograd gap)\n    " +
+          "\n    ".join(f"{k}: {e:.1e} ({fl:.1e}) | {efd:.1e} ({ffl:.1e})" for k, e, fl, efd, ffl in rows))
+    for k, e, fl, efd, ffl in rows:
+        assert e <= max(TOL, 2 * fl), (k, e, fl)
+        assert efd <= max(TOL, 2 * ffl), (k, efd, ffl)
