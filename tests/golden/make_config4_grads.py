@@ -12,11 +12,12 @@ Stored:
 - the hidden material, the nominal material (EA, EI, node masses), the drag and the root program;
 - float64 torch autograd gradients through oracle/torch_rope.py (the transcription pinned by
   tests/test_torch_transcription.py) for EA, EI, mass, drag, p0, v0, p1;
-- float32 autograd of the same transcription, at EA and at EA (1 + 1e-7): how far float32 arithmetic alone moves
+- float32 autograd of the same transcription at EA (1 + k 1e-7), k = 0..3: how far float32 arithmetic alone moves
   each gradient family;
-- central differences of the float64 C++ oracle's loss along each family's float64 gradient direction, with the
-  step sizes of tests/test_torch_transcription.py (relative 1e-6 for EA, drag and v0, 1e-5 for EI, 1e-7 for the
-  masses, 1e-9 m for root positions).
+- central differences of the float64 C++ oracle per rope (the ropes never interact): for each per-vertex or
+  per-rope family along the float64 gradient restricted to that rope, and for the drag, which every rope shares,
+  from the per-rope loss terms of one pair of runs; step sizes of tests/test_torch_transcription.py (relative
+  1e-6 for EA, drag and v0, 1e-5 for EI, 1e-7 for the masses, 1e-9 m for root positions).
 """
 import os
 import sys
@@ -62,13 +63,16 @@ def main():
     g64, loss = torch_grads(pyoracle, s, mat, s.drag, prog, tips, tgt)
     print(f"float64 autograd: loss {loss:.6e} ({time.time() - t0:.0f} s)", flush=True)
     g32 = []
-    for p in (0.0, 1e-7):
+    for p in (0.0, 1e-7, 2e-7, 3e-7):
         g, _ = torch_grads(pyoracle, s, (mat[0] * (1 + p), mat[1], mat[2]), s.drag, prog, tips, tgt,
                            dtype=torch.float32)
         g32.append(g)
-    print(f"float32 autograd x2 ({time.time() - t0:.0f} s)", flush=True)
+    print(f"float32 autograd x4 ({time.time() - t0:.0f} s)", flush=True)
 
-    def loss_at(name, h, v):
+    V = s.num_vertices // ROPES
+
+    def rope_losses(name, h, v):
+        """Per-rope tip losses of the float64 oracle with family `name` moved by h v."""
         mm, pp, dd = list(mat), list(prog), s.drag
         if name in ("ea", "ei", "m"):
             i = ("ea", "ei", "m").index(name)
@@ -78,7 +82,16 @@ def main():
         else:
             i = ("p0", "v0", "p1").index(name)
             pp[i] = prog[i] + h * v
-        return oracle_loss(pyoracle, s, mm, dd, pp, tips, tgt)[0]
+        o = make_oracle(pyoracle, s)
+        o.set_stretch_stiffnesses(mm[0])
+        o.set_bend_stiffnesses(mm[1])
+        o.set_masses(mm[2])
+        o.drag = dd
+        per = np.zeros(ROPES)
+        for f in range(FRAMES):
+            o.step(pp[0][f], pp[1][f], s.dt, s.substeps, root_pos_end=pp[2][f])
+            per += ((o.positions()[tips] - tgt[f]) ** 2).sum(1)
+        return per
 
     out = dict(ea_hidden=ea_h, ei_hidden=ei_h, ea=mat[0], ei=mat[1], m=mat[2], drag=np.float64(s.drag),
                p0=prog[0], v0=prog[1], p1=prog[2], tgt=tgt, tips=tips, loss=np.float64(loss))
@@ -86,22 +99,33 @@ def main():
         out["g64_" + k] = np.asarray(g64[k])
         for j, g in enumerate(g32):
             out[f"g32_{j}_" + k] = np.asarray(g[k])
-        # direction: the float64 gradient, scaled by the parameter for the material (relative steps)
-        if k in ("ea", "ei", "m"):
-            scale = mat[("ea", "ei", "m").index(k)]
-            v = scale * (g64[k] * scale)
-            v = v / np.linalg.norm(g64[k] * scale)
-        elif k == "drag":
-            v = np.float64(np.sign(g64[k]))
-        else:
-            v = g64[k] / np.linalg.norm(g64[k])
-        eps = STEPS[k]
-        fd = (loss_at(k, eps, v) - loss_at(k, -eps, v)) / (2 * eps)
-        out["dir_" + k] = np.asarray(v, np.float64)
-        out["fd_" + k] = np.float64(fd)
-        ad = float(np.sum(g64[k] * v))
-        print(f"  {k}: float64 autograd projection {ad:.8e}  central difference {fd:.8e}  rel {abs(ad - fd) / abs(fd):.1e}",
+    # central differences per rope (ropes never interact): for a per-vertex or per-rope family, along the float64
+    # gradient restricted to the rope (material scaled by its value: relative steps); for the drag, shared by
+    # every rope, the per-rope loss terms of one pair of runs
+    for k in ("ea", "ei", "m", "p0", "v0", "p1"):
+        dirs, fds = [], []
+        for r in range(ROPES):
+            g = np.asarray(g64[k])
+            v = np.zeros_like(g)
+            if k in ("ea", "ei", "m"):
+                scale = mat[("ea", "ei", "m").index(k)]
+                sl = slice(r * V, (r + 1) * V)
+                v[sl] = scale[sl] * (g[sl] * scale[sl])
+                v /= np.linalg.norm(g[sl] * scale[sl])
+            else:
+                v[:, r] = g[:, r] / np.linalg.norm(g[:, r])
+            eps = STEPS[k]
+            fds.append((rope_losses(k, eps, v)[r] - rope_losses(k, -eps, v)[r]) / (2 * eps))
+            dirs.append(v)
+        out["dir_" + k] = np.asarray(dirs)
+        out["fd_" + k] = np.asarray(fds)
+        ad = np.array([float(np.sum(np.asarray(g64[k]) * dirs[r])) for r in range(ROPES)])
+        print(f"  {k}: float64 autograd vs central difference per rope, worst {np.max(np.abs(ad - fds) / np.abs(fds)):.1e}",
               flush=True)
+    eps = STEPS["drag"]
+    out["fd_drag"] = (rope_losses("drag", eps, 1.0) - rope_losses("drag", -eps, 1.0)) / (2 * eps)
+    print(f"  drag: per-rope central differences sum {out['fd_drag'].sum():.6e}, float64 autograd {float(g64['drag']):.6e}",
+          flush=True)
     np.savez_compressed(os.path.join(HERE, "config4_gradients.npz"), **out)
     print(f"wrote config4_gradients.npz ({time.time() - t0:.0f} s)")
 
