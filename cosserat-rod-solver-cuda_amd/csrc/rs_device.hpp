@@ -657,6 +657,12 @@ __device__ __forceinline__ f3 end_velocity(const Params& P, f3 d, f3 t) {
     return v * (1.f - P.drag * P.h);
 }
 
+// Mesh poses of a rope during substep s of a persistent frame kernel: the precomputed per-substep poses while
+// meshes move, the committed poses otherwise.
+__device__ __forceinline__ const Pose* cell_poses(const Params& P, const Dev& D, int rope, int s) {
+    return P.sdfMoving ? D.posesS + ((size_t)s * P.nropes + rope) * P.numSdf : D.poses + (size_t)rope * MAX_SDF;
+}
+
 // SLERP of the root program (csrc/batch.cu:106-114).
 __device__ __forceinline__ q4 slerp4(q4 a, q4 b, float t) {
     float d = qdot(a, b);

@@ -550,7 +550,25 @@ struct rs_batch {
     // Cells of the persistent kernel: whole units (a contact group, or a rope without rope-rope contact) packed
     // best-fit-decreasing into cells of the variant's capacity, so the fewest workgroups carry the batch. A cell's
     // units are laid out in vertex order; cellMap gives the vertex of every slot.
+    // The strand kernel takes one rope per wave: a cell per rope, launched longest first.
+    void build_strand_order(int var) {
+        cells.assign(N, Cell{0, 0, 1, 0});
+        std::vector<int> order(N);
+        for (int r = 0; r < N; r++) {
+            cells[r] = Cell{roff[r], roff[r + 1] - roff[r], 1, 0};
+            order[r] = r;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a].nv > cells[b].nv; });
+        if (d_cells) hipFree(d_cells);
+        if (d_order) hipFree(d_order);
+        d_cells = dalloc<Cell>(N, "cells");
+        d_order = dalloc<int>(N, "rope order");
+        hcheck(hipMemcpy(d_cells, cells.data(), N * sizeof(Cell), hipMemcpyHostToDevice), "upload cells");
+        hcheck(hipMemcpy(d_order, order.data(), N * sizeof(int), hipMemcpyHostToDevice), "upload order");
+        cellsFor = var;
+    }
     void build_cells(int var) {
+        if (kCellVariants[var].strand) return build_strand_order(var);
         const int cap = kCellVariants[var].block * kCellVariants[var].vpt;
         const int G = ropeContact ? groupRopes : 1;
         struct Unit {
@@ -640,15 +658,25 @@ struct rs_batch {
         return m;
     }
     bool variant_supported(int var) const {
-        const int cap = kCellVariants[var].block * kCellVariants[var].vpt;
-        return cap >= max_unit() && cell_lds_bytes(kCellVariants[var].block, kCellVariants[var].vpt) <= device_max_lds();
+        const CellVariant& cv = kCellVariants[var];
+        if (variant_lds_bytes(var) > device_max_lds()) return false;
+        if (cv.strand) {  // one rope per wave, no rope-rope contact
+            if (ropeContact) return false;
+            int m = 0;
+            for (int r = 0; r < N; r++) m = std::max(m, roff[r + 1] - roff[r]);
+            return m <= 64 * cv.vpt;
+        }
+        return cv.block * cv.vpt >= max_unit();
     }
     int default_variant() const {
-        // smallest supported cell that still fills the machine with work; the autotuner refines it
+        // the strand kernel with the fewest blocks per lane that holds every rope, else the smallest supported
+        // cell that still fills the machine with work; the autotuner refines it
+        for (int v = 0; v < kNumCellVariants; v++)
+            if (kCellVariants[v].strand && variant_supported(v)) return v;
         int best = -1;
         for (int v = 0; v < kNumCellVariants; v++) {
             if (!variant_supported(v)) continue;
-            if (kCellVariants[v].vpt != 1) continue;
+            if (kCellVariants[v].strand || kCellVariants[v].vpt != 1) continue;
             best = v;
             break;
         }
@@ -1401,7 +1429,7 @@ int rs_cell_variant_info(int variant, int* block, int* vpt, int* supported) {
     if (variant < 0 || variant >= kNumCellVariants) return RS_INVALID;
     *block = kCellVariants[variant].block;
     *vpt = kCellVariants[variant].vpt;
-    if (supported) *supported = cell_lds_bytes(*block, *vpt) <= device_max_lds() ? 1 : 0;
+    if (supported) *supported = variant_lds_bytes(variant) <= device_max_lds() ? 1 : 0;
     return RS_OK;
 }
 

@@ -393,12 +393,6 @@ __global__ void k_check_broadphase(Params P, Dev D, const float4* xs, const floa
 //   q^k of the previous orientation sweep] | [orientation(k): read dx^(k+1), neighbours' q^k -> own q^(k+1)
 //   kept in a register] | ...
 // so nobody reads a slot while its owner overwrites it, and the results equal the per-substep kernels'.
-// Mesh poses of a rope during substep s of the persistent kernel: the precomputed per-substep poses while
-// meshes move, the committed poses otherwise.
-__device__ __forceinline__ const Pose* cell_poses(const Params& P, const Dev& D, int rope, int s) {
-    return P.sdfMoving ? D.posesS + ((size_t)s * P.nropes + rope) * P.numSdf : D.poses + (size_t)rope * MAX_SDF;
-}
-
 // Root program of every rope at every substep of the frame (csrc/batch.cu:122-159), ahead of the persistent
 // kernel so its per-vertex threads only load the result.
 __global__ void k_root_traj(Params P, Drive Dr, int S, float4* out) {
@@ -881,7 +875,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         sD[CAP + sl] = d.y;
                         sD[2 * CAP + sl] = d.z;
                     }
-                    if (it > 0) sQ[sl] = stq(qo[k]);  // publish the frames of the previous orientation sweep
+                    // publish the frames of the previous orientation sweep (at it = 0 those of the previous substep
+                    // or of the root program)
+                    sQ[sl] = stq(qo[k]);
                 }
             }
             __syncthreads();
@@ -903,11 +899,13 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 }
             }
         }
-        __syncthreads();
+        __syncthreads();  // the last sweep's neighbour reads are done before the end phase moves the positions
+        if (P.dragQuad > 0.f) {  // the tips' drag tangent: the previous segment's frame after the last sweep
 #pragma unroll
-        for (int k = 0; k < VPT; k++)
-            if (gi[k] >= 0) sQ[t + k * BLOCK] = stq(qo[k]);  // frames of the last orientation sweep
-        __syncthreads();
+            for (int k = 0; k < VPT; k++)
+                if (gi[k] >= 0) sQ[t + k * BLOCK] = stq(qo[k]);
+            __syncthreads();
+        }
 
         // end of substep: force readout (every lane, for the wave-wide wrench sums), then velocity and commit
         if (FORCES) {
@@ -984,7 +982,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     nc[k] = 0;
             }
         }
-        __syncthreads();
+        // no barrier here: the end phase touched only each vertex's own slots, and the next substep reads a
+        // neighbour's slot only after its init barrier
     }
 
     // write the state back, fill the views
@@ -1115,6 +1114,11 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
     return hipGetLastError();
 }
 
+size_t variant_lds_bytes(int variant) {
+    const CellVariant& v = kCellVariants[variant];
+    return v.strand ? strand_lds_bytes(v.vpt) : cell_lds_bytes(v.block, v.vpt);
+}
+
 size_t cell_lds_bytes(int block, int vpt) {
     const size_t cap = (size_t)block * vpt;
     return cap * (4 + 4 + 3 + 3 + 9 + 1 + 1 + 3 + 1 + 3 + 3 + 3 + 1) * sizeof(float) + (cap / 16) * (6 + 2 + 2) * sizeof(float);
@@ -1134,9 +1138,8 @@ static hipError_t launch_cell_f(const Params& P, const Dev& D, const Drive& Dr, 
     hipLaunchKernelGGL((k_cell<B, V, FORCES, F>), dim3(ncells), dim3(B), lds, st, P, D, Dr, cells, order, S, stats);
     return hipGetLastError();
 }
-template <int B, int V>
-static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
-                                int ncells, int S, unsigned long long* stats, hipStream_t st) {
+// The per-frame inputs of the persistent kernels: the root program and the moving-mesh poses at every substep.
+static hipError_t launch_frame_precompute(const Params& P, const Dev& D, const Drive& Dr, int S, hipStream_t st) {
     if (Dr.mode == DRIVE_PROGRAM) {
         hipLaunchKernelGGL(k_root_traj, dim3(nblk(S * P.nropes, 256)), dim3(256), 0, st, P, Dr, S,
                            const_cast<float4*>(D.rootTraj));
@@ -1150,6 +1153,14 @@ static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, 
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
+    return hipSuccess;
+}
+
+template <int B, int V>
+static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
+                                int ncells, int S, unsigned long long* stats, hipStream_t st) {
+    const hipError_t pe = launch_frame_precompute(P, D, Dr, S, st);
+    if (pe != hipSuccess) return pe;
     const bool forces = P.forcesOn && (P.numPlanes > 0 || P.numSdf > 0);
     if (P.fastMath) {
         if (forces) return launch_cell_f<B, V, true, true>(P, D, Dr, cells, order, ncells, S, stats, st);
@@ -1159,13 +1170,19 @@ static hipError_t launch_cell_t(const Params& P, const Dev& D, const Drive& Dr, 
     return launch_cell_f<B, V, false, false>(P, D, Dr, cells, order, ncells, S, stats, st);
 }
 
-const CellVariant kCellVariants[] = {{64, 1},  {128, 1}, {256, 1}, {384, 1}, {512, 1},
-                                     {768, 1}, {1024, 1}, {256, 2}, {256, 4}, {512, 2}};
+const CellVariant kCellVariants[] = {{64, 1, 0},  {128, 1, 0}, {256, 1, 0}, {384, 1, 0}, {512, 1, 0}, {768, 1, 0},
+                                     {1024, 1, 0}, {256, 2, 0},  {256, 4, 0}, {512, 2, 0}, {64, 1, 1},  {64, 2, 1},
+                                     {64, 4, 1}};
 const int kNumCellVariants = sizeof(kCellVariants) / sizeof(kCellVariants[0]);
 
 hipError_t launch_cell(int variant, const Params& P, const Dev& D, const Drive& Dr, const Cell* cells, const int* order,
                        int ncells, int S, unsigned long long* stats, hipStream_t st) {
     if (ncells <= 0) return hipSuccess;
+    if (variant >= 0 && variant < kNumCellVariants && kCellVariants[variant].strand) {
+        const hipError_t pe = launch_frame_precompute(P, D, Dr, S, st);
+        if (pe != hipSuccess) return pe;
+        return launch_strand(kCellVariants[variant].vpt, P, D, Dr, order, ncells, S, st);
+    }
     switch (variant) {
         case 0: return launch_cell_t<64, 1>(P, D, Dr, cells, order, ncells, S, stats, st);
         case 1: return launch_cell_t<128, 1>(P, D, Dr, cells, order, ncells, S, stats, st);

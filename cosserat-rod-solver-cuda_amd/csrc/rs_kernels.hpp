@@ -10,10 +10,16 @@ namespace rs {
 
 struct CellVariant {
     int block, vpt;  // workgroup size and vertex slots per thread: a cell holds up to block * vpt vertices
+    int strand;      // 1: the strand kernel, one wave per rope of up to 64 * vpt vertices (block 64)
 };
 extern const CellVariant kCellVariants[];
 extern const int kNumCellVariants;
 size_t cell_lds_bytes(int block, int vpt);
+size_t strand_lds_bytes(int nb);
+size_t variant_lds_bytes(int variant);
+// The strand kernel (rs_strand.hip): one wave per rope, `order` [nropes] the launch order of the ropes.
+hipError_t launch_strand(int nb, const Params& P, const Dev& D, const Drive& Dr, const int* order, int nropes, int S,
+                         hipStream_t st);
 
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, long n, hipStream_t s);

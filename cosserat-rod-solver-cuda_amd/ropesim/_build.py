@@ -22,7 +22,7 @@ ARCH = os.environ.get("ROPESIM_ARCH", "gfx950")
 LIB = os.path.join(PKG, "librope_hip.so")
 EXT = os.path.join(PKG, "_ropesim" + sysconfig.get_config_var("EXT_SUFFIX"))
 
-HIP_SOURCES = ["rs_kernels.hip", "rs_adjoint.hip", "rs_bake.hip", "rs_bvh.hip", "rs_engine.hip"]
+HIP_SOURCES = ["rs_kernels.hip", "rs_strand.hip", "rs_adjoint.hip", "rs_bake.hip", "rs_bvh.hip", "rs_engine.hip"]
 HIP_HEADERS = ["rs_math.hpp", "rs_types.hpp", "rs_device.hpp", "rs_kernels.hpp", "rs_adjoint.hpp", "rs_bvh.hpp"]
 
 # -ffp-contract=off: both dispatch forms and the checker follow one operation order; the packed-FP32

@@ -108,7 +108,10 @@ def test_config1_fp32_oracle_tracks_device(rs, oracle):
 
 def _run(rs, scene, mode, nframes, variant=-1, fast=True):
     b = scene.build(rs)
-    b.set_dispatch(mode, variant)
+    try:
+        b.set_dispatch(mode, variant)
+    except ValueError:  # a variant that does not take this batch (rope length, contact groups)
+        return None
     b.fast_math = fast
     for fi in scene.frame_inputs(nframes):
         scene.step(b, fi, to_dev)
@@ -147,14 +150,19 @@ def test_per_substep_kernels_equal_persistent_kernel_bitwise(rs, which, fast):
         s = scenes.config5(n_ropes=8, frames=2, grid_n=24)
     a = _run(rs, s, 0, s.frames, fast=fast)
     need = {"config2": 65, "config2-dragquad": 65, "config3": 96 * 4, "config5": 96 * 4}[which]
+    ran = 0
     for var in range(rs.num_cell_variants()):
         blk, vpt, ok = rs.cell_variant_info(var)
         if not ok or blk * vpt < need:
             continue
         b = _run(rs, s, 1, s.frames, var, fast=fast)
+        if b is None:
+            continue
+        ran += 1
         for get in ("positions", "velocities", "quaternions"):
             x, y = host(getattr(a, get)()), host(getattr(b, get)())
             assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), f"{get} differs (variant {var})"
+    assert ran >= 3
 
 
 def _fp32_oracle(oracle, scene, k_scale=1.0, arith="ieee"):
