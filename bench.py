@@ -579,9 +579,21 @@ def build_headline(args, frames, fast):
     import ropesim
     from ropesim import scenes
     s = scenes.config3(n_ropes=args.ropes, frames=frames)
+    drop = {w for w in args.without.split(",") if w}
+    unknown = drop - {"rope_contact", "meshes", "mesh_motion", "planes"}
+    if unknown:
+        sys.exit(f"--without: unknown feature(s) {sorted(unknown)}")
+    if "rope_contact" in drop:
+        s.rope_contact = None
+    if "meshes" in drop:
+        s.meshes, s.mesh_pos0, s.mesh_quat0 = [], None, None
+    if "planes" in drop:
+        s.planes = None
     b = s.build(ropesim)
     b.fast_math = fast
     fis = s.frame_inputs(frames)
+    if "mesh_motion" in drop:
+        fis = [{k: v for k, v in fi.items() if not k.startswith("mesh_")} for fi in fis]
     dev = [{k: torch.as_tensor(v, device="cuda") for k, v in fi.items() if v is not None} for fi in fis]
     torch.cuda.synchronize()
     tuned_ms = None
@@ -617,6 +629,9 @@ def main():
     ap.add_argument("--no-bake", action="store_true", help="skip the mesh-distance-grid leg")
     ap.add_argument("--bvh-sizes", default="1024,8192,32768",
                     help="vertex counts of the long-rope self-contact leg (scan vs hierarchy; empty skips it)")
+    ap.add_argument("--without", default="",
+                    help="comma-separated features to leave out of the headline workload, for cost breakdowns: "
+                         "rope_contact, meshes, mesh_motion (meshes stay where they start), planes")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -686,7 +701,8 @@ def main():
                    "ropes": args.ropes, "vertices": s.num_vertices, "substeps": s.substeps, "iterations": s.num_itr,
                    "dispatch": dispatch, "arithmetic": arith, "graph_captures": captures,
                    "autotune_ms": tuned_ms, "setup_s": round(setup_s, 1),
-                   "timed_frames": f"frames {W}..{W + K - 1} of the run (frames slow as the ropes tangle)"},
+                   "timed_frames": f"frames {W}..{W + K - 1} of the run (frames slow as the ropes tangle)",
+                   "without": args.without or None},
         "frame_times": series(frame_ms),
     }
     if launches > 0:

@@ -458,6 +458,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
     const int t = threadIdx.x;
 
     int gi[VPT], rope[VPT], nc[VPT], ncol[VPT];
+    uint32_t pairs[VPT];  // the first two rope-rope partners of the substep (16 bits each), kept for the iterations
     int grp[VPT];   // packed local [begin, end) of the contact group (16 bits each)
     int vcnt[VPT];  // skin-list length of the vertex's segment (-1: overflow, scan the whole group)
     float4 prm[VPT];
@@ -473,6 +474,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
         nc[k] = P.numSdf > 0 ? -1 : 0;  // a frame starts by visiting every mesh (sdf.cu:226-227)
         cand[k] = 0;
         ncol[k] = 0;
+        pairs[k] = 0;
         vcnt[k] = 0;
         limEff[k] = INFINITY;
         if (gi[k] >= 0) {
@@ -731,6 +733,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                 const int sl = t + k * BLOCK;
                 float minDist = INFINITY;
                 ncol[k] = 0;
+                pairs[k] = 0;
                 if (gi[k] >= 0 && (flags_of(prm[k]) & HAS_NEXT)) {
                     const int i = gi[k];
                     const f3 a0 = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
@@ -752,31 +755,35 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     }
                     const int n = full ? g1 - g0 : vcnt[k];
                     int cnt = 0, tests = 0;
-                    int jn = (!full && n > 0) ? (int)vl[0] : 0;  // the next skin entry is loaded one step ahead
-                    for (int c = 0; c < n; c++) {
-                        int j;
-                        if (full) {
-                            j = g0 + c;
-                        } else {
-                            j = jn;
-                            if (c + 1 < n) jn = (int)vl[(size_t)(c + 1) * CAP];
-                        }
-                        if (full && (!(sFl[j] & HAS_NEXT) || (j >= rs0 && j < rs1 && abs(j - sl) <= 2))) continue;
-                        const f3 c0 = mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]);
-                        const f3 c1 = mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]);
-                        const float rj = fmaxf(sPad[j], sPad[j + 1]);
-                        const f3 blo = vmin(c0, c1), bhi = vmax(c0, c1);
-                        if (blo.x - rj > hi.x || blo.y - rj > hi.y || blo.z - rj > hi.z || bhi.x + rj < lo.x ||
-                            bhi.y + rj < lo.y || bhi.z + rj < lo.z)
-                            continue;
-                        tests++;
-                        float d2;
-                        if (pair_hit(a1, c0 - a0, c1 - a0, ri + rj, d2)) {
-                            if (cnt < MAX_COLL) {
-                                cl[(size_t)cnt * CAP] = (uint16_t)j;
-                                minDist = fminf(minDist, 0.45f * sqrtf(d2));
+                    // skin entries come from global memory in batches of four loads issued together, so a long
+                    // list waits once per batch instead of once per entry
+                    for (int cb = 0; cb < n; cb += 4) {
+                        int jb[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++)
+                            jb[u] = full ? g0 + cb + u : (cb + u < n ? (int)vl[(size_t)(cb + u) * CAP] : 0);
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const int j = jb[u];
+                            if (cb + u >= n) break;
+                            if (full && (!(sFl[j] & HAS_NEXT) || (j >= rs0 && j < rs1 && abs(j - sl) <= 2))) continue;
+                            const f3 c0 = mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]);
+                            const f3 c1 = mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]);
+                            const float rj = fmaxf(sPad[j], sPad[j + 1]);
+                            const f3 blo = vmin(c0, c1), bhi = vmax(c0, c1);
+                            if (blo.x - rj > hi.x || blo.y - rj > hi.y || blo.z - rj > hi.z || bhi.x + rj < lo.x ||
+                                bhi.y + rj < lo.y || bhi.z + rj < lo.z)
+                                continue;
+                            tests++;
+                            float d2;
+                            if (pair_hit(a1, c0 - a0, c1 - a0, ri + rj, d2)) {
+                                if (cnt < MAX_COLL) {
+                                    cl[(size_t)cnt * CAP] = (uint16_t)j;
+                                    minDist = fminf(minDist, 0.45f * sqrtf(d2));
+                                }
+                                if (cnt < 2) pairs[k] |= (uint32_t)j << (16 * cnt);
+                                cnt++;
                             }
-                            cnt++;
                         }
                     }
                     if (stats) {  // statistics only: [skin entries at builds, exact tests, list entries, builds]
@@ -839,7 +846,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         const float dmp = Ar<F>::div(P.damping, h);
                         const uint16_t* lst = D.clist + (size_t)C.v0 * MAX_COLL + sl;
                         for (int c = 0; c < ncol[k]; c++) {
-                            const int j = lst[(size_t)c * CAP];
+                            const int j = c < 2 ? (int)((pairs[k] >> (16 * c)) & 0xFFFFu) : (int)lst[(size_t)c * CAP];
                             collision_term<F>(P, dmp, p0, p1, d0, p1dx, mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]),
                                               mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]),
                                               mk3(sD[j], sD[CAP + j], sD[2 * CAP + j]),
