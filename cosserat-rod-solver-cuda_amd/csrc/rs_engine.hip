@@ -31,6 +31,9 @@ constexpr size_t kAdjointHistoryBudget = size_t(2) << 30;
 // kernels (the full scan costs the square of the group size per substep; measured on MI355X the hierarchy is
 // already 1.3x faster on one 256-vertex rope and 4x on 1024 groups of ~550 vertices).
 constexpr int kBvhMinGroup = 128;
+// Batches of up to this many cells order each frame's launch by the previous frame's cell durations (the
+// ranking visits every pair of cells, so it stays well under a microsecond per cell up to this count).
+constexpr int kRankMaxCells = 16384;
 
 struct Invalid : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -130,6 +133,7 @@ struct rs_batch {
     std::vector<Cell> cells;
     Cell* d_cells = nullptr;
     int* d_order = nullptr;
+    unsigned* d_cellTime = nullptr;  // [cells] last frame's duration of each cell (orders the next launch)
     unsigned long long* d_hits = nullptr;  // rope-rope statistics [4]
     bool countHits = false;
     // rope-rope detection of the per-substep kernels: 0 the hierarchy for groups above kBvhMinGroup
@@ -207,7 +211,7 @@ struct rs_batch {
         }
         void* ptrs[] = {D.x, D.v, D.vlast, D.q, D.q2, D.dx, D.dx2, D.prm, D.qrest, D.ropeOf, d_roff, D.pad, D.maxStep,
                         D.ncol, D.coll, D.vlist, D.clist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.wrenchFx, D.outPos,
-                        D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_hits,
+                        D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_cellTime, d_hits,
                         d_grids, d_cur, d_tgt, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
                         tape.poses, tape.motion, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
                         d_gdrag, d_mismatch, d_blk};
@@ -561,6 +565,7 @@ struct rs_batch {
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a].nv > cells[b].nv; });
         if (d_cells) hipFree(d_cells);
         if (d_order) hipFree(d_order);
+        D.cellTime = nullptr;  // the strand kernel keeps its rope order
         d_cells = dalloc<Cell>(N, "cells");
         d_order = dalloc<int>(N, "rope order");
         hcheck(hipMemcpy(d_cells, cells.data(), N * sizeof(Cell), hipMemcpyHostToDevice), "upload cells");
@@ -626,6 +631,15 @@ struct rs_batch {
         hcheck(hipMemcpy(d_cells, cells.data(), nc * sizeof(Cell), hipMemcpyHostToDevice), "upload cells");
         hcheck(hipMemcpy(d_order, order.data(), nc * sizeof(int), hipMemcpyHostToDevice), "upload order");
         hcheck(hipMemcpy(dm, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice), "upload cell map");
+        // the first frame runs the cells largest first; every later frame longest first by measured duration
+        if (d_cellTime) hipFree(d_cellTime);
+        d_cellTime = nullptr;
+        D.cellTime = nullptr;
+        if (nc <= kRankMaxCells) {
+            d_cellTime = dalloc<unsigned>(nc, "cell durations");
+            hcheck(hipMemset(d_cellTime, 0, nc * sizeof(unsigned)), "clear cell durations");
+            D.cellTime = d_cellTime;
+        }
         // the persistent kernel's skin and candidate lists are indexed by cell slot
         if (ropeContact && map.size() > vlistCap) {
             if (D.vlist) hipFree(D.vlist);
@@ -724,6 +738,8 @@ struct rs_batch {
             hcheck(launch_cell(var, P, D, dr, d_cells, d_order, (int)cells.size(), S, countHits ? d_hits : nullptr,
                                stream),
                    "frame kernel");
+            if (D.cellTime && !kCellVariants[var].strand)
+                hcheck(launch_rank_cells(D.cellTime, (int)cells.size(), d_order, stream), "cell order");
         }
     }
     void run_substeps(const Drive& dr, int S) {
@@ -758,13 +774,14 @@ struct rs_batch {
     // Graph form of the frame: the kernel sequence is captured once into a hipGraph and replayed while
     // every argument it was captured with (parameters, buffers, staged drive, substeps, dispatch) is the same.
     void launch_graph(const Params& P, const Drive& dr, int S, int var, const Bvh* tree) {
-        std::vector<char> key(sizeof(Params) + sizeof(Dev) + sizeof(Drive) + 7 * sizeof(long long));
+        std::vector<char> key(sizeof(Params) + sizeof(Dev) + sizeof(Drive) + 8 * sizeof(long long));
         char* k = key.data();
         std::memcpy(k, &P, sizeof(Params));
         std::memcpy(k + sizeof(Params), &D, sizeof(Dev));
         std::memcpy(k + sizeof(Params) + sizeof(Dev), &dr, sizeof(Drive));
-        const long long tail[7] = {mode, var, S, (long long)(uintptr_t)d_cells, (long long)cells.size(),
-                                   countHits ? 1 : 0, (long long)(uintptr_t)(tree ? tree->key : nullptr)};
+        const long long tail[8] = {mode, var, S, (long long)(uintptr_t)d_cells, (long long)cells.size(),
+                                   countHits ? 1 : 0, (long long)(uintptr_t)(tree ? tree->key : nullptr),
+                                   (long long)(uintptr_t)d_order};
         std::memcpy(k + sizeof(Params) + sizeof(Dev) + sizeof(Drive), tail, sizeof(tail));
         if (!gexec || key != gkey) {
             drop_graph();

@@ -434,8 +434,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
     float* sPad = sF + 9 * CAP;                          // [CAP]
     uint32_t* sFl = reinterpret_cast<uint32_t*>(sPad + CAP);  // [CAP]
     // during a skin-list build sF holds each segment's padded box and, after it, the list lengths
-    float* sBlo = sF;            // [3][CAP]
-    float* sBhi = sF + 3 * CAP;  // [3][CAP]
+    // ([CAP][3] pairs (lo.x, lo.y), (lo.z, hi.x), (hi.y, hi.z): one box is three 8-byte reads)
+    float2* sBx = reinterpret_cast<float2*>(sF);  // [CAP][3]
     int* sNcol = reinterpret_cast<int*>(sF + 8 * CAP);  // [CAP]
     float* sBox = reinterpret_cast<float*>(sFl + CAP);   // [6][NCH]
     uint32_t* sOv = reinterpret_cast<uint32_t*>(sBox + 6 * NCH);  // [2][NCH] chunk-overlap bit masks (lo, hi words)
@@ -447,6 +447,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
     float* sVl = sVel + 3 * CAP;  // [3][CAP] velocity at the previous substep start
     float* sY = sVl + 3 * CAP;    // [3][CAP] inertial target of the substep
     float* sLim = sY + 3 * CAP;   // [CAP] step limit of each segment this substep
+    int* sWork = reinterpret_cast<int*>(sLim + CAP);  // [4] next unclaimed chunk of a skin-list build
     auto ld3L = [&](const float* a, int sl) { return mk3(a[sl], a[CAP + sl], a[2 * CAP + sl]); };
     auto st3L = [&](float* a, int sl, f3 v) {
         a[sl] = v.x;
@@ -454,8 +455,10 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
         a[2 * CAP + sl] = v.z;
     };
 
-    const Cell C = cells[order[blockIdx.x]];
+    const int cellIdx = order[blockIdx.x];
+    const Cell C = cells[cellIdx];
     const int t = threadIdx.x;
+    const unsigned long long startTick = __builtin_amdgcn_s_memrealtime();
 
     int gi[VPT], rope[VPT], nc[VPT], ncol[VPT];
     uint32_t pairs[VPT];  // the first two rope-rope partners of the substep (16 bits each), kept for the iterations
@@ -612,12 +615,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                             hi = vmax(a, b) + mk3(rb, rb, rb);
                         }
                     }
-                    sBlo[sl] = lo.x;
-                    sBlo[CAP + sl] = lo.y;
-                    sBlo[2 * CAP + sl] = lo.z;
-                    sBhi[sl] = hi.x;
-                    sBhi[CAP + sl] = hi.y;
-                    sBhi[2 * CAP + sl] = hi.z;
+                    sBx[3 * sl] = make_float2(lo.x, lo.y);
+                    sBx[3 * sl + 1] = make_float2(lo.z, hi.x);
+                    sBx[3 * sl + 2] = make_float2(hi.y, hi.z);
 #pragma unroll
                     for (int o = 1; o < 16; o <<= 1) {
                         lo = vmin(lo, mk3(__shfl_xor(lo.x, o), __shfl_xor(lo.y, o), __shfl_xor(lo.z, o)));
@@ -637,6 +637,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         sChG[NCH + ch] = cg1;
                     }
                 }
+                if (t == 0) sWork[0] = BLOCK / 64;
                 __syncthreads();
                 const int nch = (C.nv + 15) >> 4;
                 // (2) chunk A meets chunk B (B within A's contact groups): 16 lanes per A, OR-reduced masks
@@ -669,13 +670,15 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                     }
                 }
                 __syncthreads();
-                // (3) one wave per chunk A: lane (a, g) holds segment i = 16 A + a and tests partners 4g..4g+3
-                //     of every overlapping chunk B in ascending order; the four lanes of a segment append their
-                //     hits in partner order (cross-lane prefix), so each skin list ascends.
+                // (3) one wave per chunk A (its own first, then the next unclaimed one): lane (a, g) holds
+                //     segment i = 16 A + a and tests partners 4g..4g+3 of every overlapping chunk B in ascending
+                //     order; the four lanes of a segment append their hits in partner order (cross-lane prefix),
+                //     so each skin list ascends. The four partner boxes are read before any is compared, so the
+                //     twelve reads of a step are in flight together.
                 {
                     const int lane = t & 63, wave = t >> 6;
                     const int a = lane & 15, g = lane >> 4;
-                    for (int A = wave; A < nch; A += BLOCK / 64) {
+                    for (int A = wave; A < nch;) {
                         const int i = A * 16 + a;
                         const uint32_t fi = i < C.nv ? sFl[i] : 0u;
                         const bool seg = (fi & HAS_NEXT) != 0;
@@ -684,25 +687,35 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         const bool prevHN = seg && (fi & HAS_PREV);
                         f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = -lo;
                         if (seg) {
-                            lo = mk3(sBlo[i], sBlo[CAP + i], sBlo[2 * CAP + i]);
-                            hi = mk3(sBhi[i], sBhi[CAP + i], sBhi[2 * CAP + i]);
+                            const float2 b0 = sBx[3 * i], b1 = sBx[3 * i + 1], b2 = sBx[3 * i + 2];
+                            lo = mk3(b0.x, b0.y, b1.x);
+                            hi = mk3(b1.y, b2.x, b2.y);
                         }
                         int cnt = 0;
-                        uint64_t mask = ((uint64_t)sOv[NCH + A] << 32) | sOv[A];
+                        // A is the same on every lane, so the mask is too: the loop below runs on the scalar unit
+                        uint64_t mask = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)sOv[NCH + A]) << 32) |
+                                        (uint32_t)__builtin_amdgcn_readfirstlane((int)sOv[A]);
                         uint16_t* vl = D.vlist + (size_t)C.v0 * VERLET_CAP + i;  // entry e at vl[e * CAP]
                         while (mask) {
                             const int B = __builtin_ctzll(mask);
                             mask &= mask - 1;
+                            float2 e0[4], e1[4], e2[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const int jc = min(B * 16 + 4 * g + u, C.nv - 1);
+                                e0[u] = sBx[3 * jc];
+                                e1[u] = sBx[3 * jc + 1];
+                                e2[u] = sBx[3 * jc + 2];
+                            }
                             uint32_t hm = 0;
 #pragma unroll
                             for (int u = 0; u < 4; u++) {
                                 const int j = B * 16 + 4 * g + u;
-                                const int jc = min(j, C.nv - 1);
-                                const bool ov = !(sBlo[jc] > hi.x || sBlo[CAP + jc] > hi.y || sBlo[2 * CAP + jc] > hi.z ||
-                                                  sBhi[jc] < lo.x || sBhi[CAP + jc] < lo.y || sBhi[2 * CAP + jc] < lo.z);
+                                const bool apart = (e0[u].x > hi.x) | (e0[u].y > hi.y) | (e1[u].x > hi.z) |
+                                                   (e1[u].y < lo.x) | (e2[u].x < lo.y) | (e2[u].y < lo.z);
                                 const int dj = j - i;
-                                const bool self = dj == 0 || dj == 1 || dj == -1 || (dj == 2 && nextHN) || (dj == -2 && prevHN);
-                                if (seg && ov && !self && j < C.nv && j >= g0 && j < g1) hm |= 1u << u;
+                                const bool self = ((unsigned)(dj + 1) <= 2u) | ((dj == 2) & nextHN) | ((dj == -2) & prevHN);
+                                if (seg & !apart & !self & (j < C.nv) & (j >= g0) & (j < g1)) hm |= 1u << u;
                             }
                             const int c = __builtin_popcount(hm);
                             const int c0 = __shfl(c, a), c1 = __shfl(c, a + 16), c2 = __shfl(c, a + 32), c3 = __shfl(c, a + 48);
@@ -718,6 +731,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         }
                         if (g == 0 && i < C.nv) sNcol[i] = cnt <= VERLET_CAP ? cnt : -1;
                         if (stats && g == 0 && seg) atomicAdd(stats, (unsigned long long)cnt);
+                        int claim = 0;
+                        if (lane == 0) claim = atomicAdd(sWork, 1);
+                        A = __builtin_amdgcn_readfirstlane(claim);
                     }
                 }
                 __syncthreads();
@@ -1023,6 +1039,34 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             }
         }
     }
+    // the cell's duration orders the next frame's launch (k_rank_cells)
+    if (D.cellTime) {
+        __syncthreads();
+        if (t == 0) D.cellTime[cellIdx] = (unsigned)(__builtin_amdgcn_s_memrealtime() - startTick);
+    }
+}
+
+// Launch order of the cells for the next frame, longest first by the durations of this frame: cell c goes
+// to position #{c' : time[c'] > time[c], or time[c'] == time[c] and c' < c}, a permutation. Workgroups start
+// in launch order, so the slowest cells start first and the short ones fill in behind them (longest
+// processing time first). The order decides only where and when a cell runs, never its results.
+__global__ __launch_bounds__(256) void k_rank_cells(const unsigned* time, int n, int* order) {
+    __shared__ unsigned tile[256];
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const unsigned mine = c < n ? time[c] : 0u;
+    int rank = 0;
+    for (int base = 0; base < n; base += 256) {
+        const int j = base + threadIdx.x;
+        tile[threadIdx.x] = j < n ? time[j] : 0u;
+        __syncthreads();
+        const int m = min(256, n - base);
+        for (int k = 0; k < m; k++) {
+            const unsigned o = tile[k];
+            rank += (o > mine || (o == mine && base + k < c)) ? 1 : 0;
+        }
+        __syncthreads();
+    }
+    if (c < n) order[rank] = c;
 }
 
 // ------------------------------------------------------------------ host launchers
@@ -1121,6 +1165,12 @@ hipError_t launch_plain_substep(const Params& P, Dev& D, const Drive& Dr, int s,
     return hipGetLastError();
 }
 
+hipError_t launch_rank_cells(const unsigned* time, int n, int* order, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rank_cells, dim3((n + 255) / 256), dim3(256), 0, st, time, n, order);
+    return hipGetLastError();
+}
+
 size_t variant_lds_bytes(int variant) {
     const CellVariant& v = kCellVariants[variant];
     return v.strand ? strand_lds_bytes(v.vpt) : cell_lds_bytes(v.block, v.vpt);
@@ -1128,7 +1178,8 @@ size_t variant_lds_bytes(int variant) {
 
 size_t cell_lds_bytes(int block, int vpt) {
     const size_t cap = (size_t)block * vpt;
-    return cap * (4 + 4 + 3 + 3 + 9 + 1 + 1 + 3 + 1 + 3 + 3 + 3 + 1) * sizeof(float) + (cap / 16) * (6 + 2 + 2) * sizeof(float);
+    return cap * (4 + 4 + 3 + 3 + 9 + 1 + 1 + 3 + 1 + 3 + 3 + 3 + 1) * sizeof(float) + (cap / 16) * (6 + 2 + 2) * sizeof(float) +
+           4 * sizeof(int);
 }
 
 template <int B, int V, bool FORCES, bool F>

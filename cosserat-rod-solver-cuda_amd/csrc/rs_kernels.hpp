@@ -17,6 +17,8 @@ extern const int kNumCellVariants;
 size_t cell_lds_bytes(int block, int vpt);
 size_t strand_lds_bytes(int nb);
 size_t variant_lds_bytes(int variant);
+// Longest-first launch order of `n` cells from their last durations (k_cell writes Dev::cellTime).
+hipError_t launch_rank_cells(const unsigned* time, int n, int* order, hipStream_t st);
 // The strand kernel (rs_strand.hip): one wave per rope, `order` [nropes] the launch order of the ropes.
 hipError_t launch_strand(int nb, const Params& P, const Dev& D, const Drive& Dr, const int* order, int nropes, int S,
                          hipStream_t st);

@@ -103,6 +103,7 @@ struct Dev {
     const float4* rootTraj;  // [S][nropes][3]: root program x (w = 1 when it sets the frame), v, q
     const Pose* posesS;      // [S][nropes][numSdf] mesh poses while meshes move
     const int* cellMap;      // [cells * cell capacity] vertex of each cell slot (-1 past the cell's end)
+    unsigned* cellTime;      // [cells] each cell's duration in its last frame (100 MHz ticks), or null
 };
 
 // What drives the roots during one frame.
