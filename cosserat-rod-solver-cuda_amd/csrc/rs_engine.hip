@@ -134,6 +134,7 @@ struct rs_batch {
     Cell* d_cells = nullptr;
     int* d_order = nullptr;
     unsigned* d_cellTime = nullptr;  // [cells] last frame's duration of each cell (orders the next launch)
+    int* d_cellQueue = nullptr;      // [1] the frame kernel's claim counter
     unsigned long long* d_hits = nullptr;  // rope-rope statistics [4]
     bool countHits = false;
     // rope-rope detection of the per-substep kernels: 0 the hierarchy for groups above kBvhMinGroup
@@ -211,7 +212,7 @@ struct rs_batch {
         }
         void* ptrs[] = {D.x, D.v, D.vlast, D.q, D.q2, D.dx, D.dx2, D.prm, D.qrest, D.ropeOf, d_roff, D.pad, D.maxStep,
                         D.ncol, D.coll, D.vlist, D.clist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.wrenchFx, D.outPos,
-                        D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_cellTime, d_hits,
+                        D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_cellTime, d_cellQueue, d_hits,
                         d_grids, d_cur, d_tgt, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
                         tape.poses, tape.motion, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
                         d_gdrag, d_mismatch, d_blk};
@@ -565,7 +566,8 @@ struct rs_batch {
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a].nv > cells[b].nv; });
         if (d_cells) hipFree(d_cells);
         if (d_order) hipFree(d_order);
-        D.cellTime = nullptr;  // the strand kernel keeps its rope order
+        D.cellTime = nullptr;  // the strand kernel keeps its rope order and one workgroup per rope
+        D.cellQueue = nullptr;
         d_cells = dalloc<Cell>(N, "cells");
         d_order = dalloc<int>(N, "rope order");
         hcheck(hipMemcpy(d_cells, cells.data(), N * sizeof(Cell), hipMemcpyHostToDevice), "upload cells");
@@ -640,6 +642,8 @@ struct rs_batch {
             hcheck(hipMemset(d_cellTime, 0, nc * sizeof(unsigned)), "clear cell durations");
             D.cellTime = d_cellTime;
         }
+        if (!d_cellQueue) d_cellQueue = dalloc<int>(1, "cell queue");
+        D.cellQueue = d_cellQueue;
         // the persistent kernel's skin and candidate lists are indexed by cell slot
         if (ropeContact && map.size() > vlistCap) {
             if (D.vlist) hipFree(D.vlist);

@@ -10,6 +10,8 @@
 // Both give bit-identical results (same arithmetic, no FP contraction, same list order).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rs_device.hpp"
 #include "rs_kernels.hpp"
 
@@ -420,8 +422,8 @@ __global__ void k_pose_traj(Params P, Dev D, int S, Pose* out) {
 }
 
 template <int BLOCK, int VPT, bool FORCES, bool F>
-__global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const Cell* cells, const int* order, int S,
-                                                unsigned long long* stats) {
+__global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const Cell* cells, const int* order,
+                                                int ncells, int S, unsigned long long* stats) {
     constexpr int CAP = BLOCK * VPT;
     constexpr int NCH = CAP / 16;
     static_assert(NCH <= 64 && BLOCK % 64 == 0, "chunk masks are 64 bits; waves are 64 lanes");
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
     float* sVl = sVel + 3 * CAP;  // [3][CAP] velocity at the previous substep start
     float* sY = sVl + 3 * CAP;    // [3][CAP] inertial target of the substep
     float* sLim = sY + 3 * CAP;   // [CAP] step limit of each segment this substep
-    int* sWork = reinterpret_cast<int*>(sLim + CAP);  // [4] next unclaimed chunk of a skin-list build
+    int* sWork = reinterpret_cast<int*>(sLim + CAP);  // [4]: next unclaimed chunk of a skin-list build, claimed cell
     auto ld3L = [&](const float* a, int sl) { return mk3(a[sl], a[CAP + sl], a[2 * CAP + sl]); };
     auto st3L = [&](float* a, int sl, f3 v) {
         a[sl] = v.x;
@@ -455,600 +457,611 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
         a[2 * CAP + sl] = v.z;
     };
 
-    const int cellIdx = order[blockIdx.x];
-    const Cell C = cells[cellIdx];
-    const int t = threadIdx.x;
-    const unsigned long long startTick = __builtin_amdgcn_s_memrealtime();
-
-    int gi[VPT], rope[VPT], nc[VPT], ncol[VPT];
-    uint32_t pairs[VPT];  // the first two rope-rope partners of the substep (16 bits each), kept for the iterations
-    int grp[VPT];   // packed local [begin, end) of the contact group (16 bits each)
-    int vcnt[VPT];  // skin-list length of the vertex's segment (-1: overflow, scan the whole group)
-    float4 prm[VPT];
-    q4 qo[VPT];
-    float limEff[VPT];
-    uint64_t cand[VPT];
-    bool isRoot[VPT];
-
-#pragma unroll
-    for (int k = 0; k < VPT; k++) {
-        const int sl = t + k * BLOCK;
-        gi[k] = sl < C.nv ? D.cellMap[C.v0 + sl] : -1;
-        nc[k] = P.numSdf > 0 ? -1 : 0;  // a frame starts by visiting every mesh (sdf.cu:226-227)
-        cand[k] = 0;
-        ncol[k] = 0;
-        pairs[k] = 0;
-        vcnt[k] = 0;
-        limEff[k] = INFINITY;
-        if (gi[k] >= 0) {
-            const int i = gi[k];
-            prm[k] = D.prm[i];
-            sQr[sl] = D.qrest[i];
-            const uint32_t fl = flags_of(prm[k]);
-            rope[k] = D.ropeOf[i];
-            isRoot[k] = D.roff[rope[k]] == i;
-            {  // a rope contact group occupies consecutive slots, so slot = vertex - (i - sl) inside it
-                const int G = P.groupRopes;
-                const int g = rope[k] / G;
-                const int base = i - sl;
-                grp[k] = (D.roff[min(g * G, P.nropes)] - base) | ((D.roff[min((g + 1) * G, P.nropes)] - base) << 16);
-            }
-            f3 x = ld3(D.x[i]);
-            q4 q = ldq(D.q[i]);
-            f3 v = ld3(D.v[i]);
-            st3L(sVl, sl, ld3(D.vlast[i]));
-            if (Dr.mode == DRIVE_CONST && isRoot[k]) {
-                const int e = rope[k];
-                x = mk3(Dr.p0[3 * e], Dr.p0[3 * e + 1], Dr.p0[3 * e + 2]);
-                v = mk3(Dr.v0[3 * e], Dr.v0[3 * e + 1], Dr.v0[3 * e + 2]);
-                if (Dr.hasQ0) q = qunit_or_identity(q4{Dr.q0[4 * e], Dr.q0[4 * e + 1], Dr.q0[4 * e + 2], Dr.q0[4 * e + 3]});
-            }
-            sX[sl] = x.x;
-            sX[CAP + sl] = x.y;
-            sX[2 * CAP + sl] = x.z;
-            st3L(sVel, sl, v);
-            qo[k] = q;
-            sQ[sl] = stq(q);
-            sFl[sl] = fl | ((uint32_t)(grp[k] & 0xFFFF) << 4) | ((uint32_t)(grp[k] >> 16) << 15);
-        } else {
-            prm[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            rope[k] = 0;
-            grp[k] = 0;
-            isRoot[k] = false;
-            const f3 z = mk3(0.f, 0.f, 0.f);
-            st3L(sVel, sl, z);
-            st3L(sVl, sl, z);
-            st3L(sY, sl, z);
-            qo[k] = q4{0.f, 0.f, 0.f, 1.f};
-            sQr[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
-            sX[sl] = sX[CAP + sl] = sX[2 * CAP + sl] = 0.f;
-            sQ[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
-            sFl[sl] = 0u;
+    // Cells come from a queue in launch order (longest first): a workgroup that finishes a cell claims the next
+    // one at once, so no compute unit idles while cells remain. The grid is at most what the device holds at
+    // once; without the queue every workgroup runs cell order[blockIdx.x].
+    for (int pass = 0;; pass++) {
+        if (D.cellQueue) {
+            if (threadIdx.x == 0) sWork[1] = atomicAdd(D.cellQueue, 1);
+            __syncthreads();
         }
-    }
-    const float h = P.h;
+        const int claim = D.cellQueue ? sWork[1] : (pass == 0 ? (int)blockIdx.x : ncells);
+        if (claim >= ncells) break;
+        const int cellIdx = order[claim];
+        const Cell C = cells[cellIdx];
+        const int t = threadIdx.x;
+        const unsigned long long startTick = __builtin_amdgcn_s_memrealtime();
 
-    for (int s = 0; s < S; s++) {
-        // root program of this substep and kinematic mesh poses
-        if (Dr.mode == DRIVE_PROGRAM) {
+        int gi[VPT], rope[VPT], nc[VPT], ncol[VPT];
+        uint32_t pairs[VPT];  // the first two rope-rope partners of the substep (16 bits each), kept for the iterations
+        int grp[VPT];   // packed local [begin, end) of the contact group (16 bits each)
+        int vcnt[VPT];  // skin-list length of the vertex's segment (-1: overflow, scan the whole group)
+        float4 prm[VPT];
+        q4 qo[VPT];
+        float limEff[VPT];
+        uint64_t cand[VPT];
+        bool isRoot[VPT];
+
 #pragma unroll
-            for (int k = 0; k < VPT; k++) {
-                if (gi[k] >= 0 && isRoot[k]) {
-                    const int sl = t + k * BLOCK;
-                    f3 x, v;
-                    q4 q;
-                    bool setQ;
-                    const float4* rt = D.rootTraj + 3 * ((size_t)s * P.nropes + rope[k]);
-                    const float4 r0 = rt[0];
-                    x = ld3(r0);
-                    v = ld3(rt[1]);
-                    setQ = r0.w != 0.f;
-                    if (setQ) q = ldq(rt[2]);
-                    sX[sl] = x.x;
-                    sX[CAP + sl] = x.y;
-                    sX[2 * CAP + sl] = x.z;
-                    st3L(sVel, sl, v);
-                    if (setQ) {
-                        qo[k] = q;
-                        sQ[sl] = stq(q);
+        for (int k = 0; k < VPT; k++) {
+            const int sl = t + k * BLOCK;
+            gi[k] = sl < C.nv ? D.cellMap[C.v0 + sl] : -1;
+            nc[k] = P.numSdf > 0 ? -1 : 0;  // a frame starts by visiting every mesh (sdf.cu:226-227)
+            cand[k] = 0;
+            ncol[k] = 0;
+            pairs[k] = 0;
+            vcnt[k] = 0;
+            limEff[k] = INFINITY;
+            if (gi[k] >= 0) {
+                const int i = gi[k];
+                prm[k] = D.prm[i];
+                sQr[sl] = D.qrest[i];
+                const uint32_t fl = flags_of(prm[k]);
+                rope[k] = D.ropeOf[i];
+                isRoot[k] = D.roff[rope[k]] == i;
+                {  // a rope contact group occupies consecutive slots, so slot = vertex - (i - sl) inside it
+                    const int G = P.groupRopes;
+                    const int g = rope[k] / G;
+                    const int base = i - sl;
+                    grp[k] = (D.roff[min(g * G, P.nropes)] - base) | ((D.roff[min((g + 1) * G, P.nropes)] - base) << 16);
+                }
+                f3 x = ld3(D.x[i]);
+                q4 q = ldq(D.q[i]);
+                f3 v = ld3(D.v[i]);
+                st3L(sVl, sl, ld3(D.vlast[i]));
+                if (Dr.mode == DRIVE_CONST && isRoot[k]) {
+                    const int e = rope[k];
+                    x = mk3(Dr.p0[3 * e], Dr.p0[3 * e + 1], Dr.p0[3 * e + 2]);
+                    v = mk3(Dr.v0[3 * e], Dr.v0[3 * e + 1], Dr.v0[3 * e + 2]);
+                    if (Dr.hasQ0) q = qunit_or_identity(q4{Dr.q0[4 * e], Dr.q0[4 * e + 1], Dr.q0[4 * e + 2], Dr.q0[4 * e + 3]});
+                }
+                sX[sl] = x.x;
+                sX[CAP + sl] = x.y;
+                sX[2 * CAP + sl] = x.z;
+                st3L(sVel, sl, v);
+                qo[k] = q;
+                sQ[sl] = stq(q);
+                sFl[sl] = fl | ((uint32_t)(grp[k] & 0xFFFF) << 4) | ((uint32_t)(grp[k] >> 16) << 15);
+            } else {
+                prm[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                rope[k] = 0;
+                grp[k] = 0;
+                isRoot[k] = false;
+                const f3 z = mk3(0.f, 0.f, 0.f);
+                st3L(sVel, sl, z);
+                st3L(sVl, sl, z);
+                st3L(sY, sl, z);
+                qo[k] = q4{0.f, 0.f, 0.f, 1.f};
+                sQr[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
+                sX[sl] = sX[CAP + sl] = sX[2 * CAP + sl] = 0.f;
+                sQ[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
+                sFl[sl] = 0u;
+            }
+        }
+        const float h = P.h;
+
+        for (int s = 0; s < S; s++) {
+            // root program of this substep and kinematic mesh poses
+            if (Dr.mode == DRIVE_PROGRAM) {
+#pragma unroll
+                for (int k = 0; k < VPT; k++) {
+                    if (gi[k] >= 0 && isRoot[k]) {
+                        const int sl = t + k * BLOCK;
+                        f3 x, v;
+                        q4 q;
+                        bool setQ;
+                        const float4* rt = D.rootTraj + 3 * ((size_t)s * P.nropes + rope[k]);
+                        const float4 r0 = rt[0];
+                        x = ld3(r0);
+                        v = ld3(rt[1]);
+                        setQ = r0.w != 0.f;
+                        if (setQ) q = ldq(rt[2]);
+                        sX[sl] = x.x;
+                        sX[CAP + sl] = x.y;
+                        sX[2 * CAP + sl] = x.z;
+                        st3L(sVel, sl, v);
+                        if (setQ) {
+                            qo[k] = q;
+                            sQ[sl] = stq(q);
+                        }
                     }
                 }
             }
-        }
-        // init
-#pragma unroll
-        for (int k = 0; k < VPT; k++) {
-            if (gi[k] >= 0) {
-                const int sl = t + k * BLOCK;
-                f3 d, y;
-                float padding;
-                const f3 v = ld3L(sVel, sl);
-                init_vertex<F>(P, prm[k].x, v, ld3L(sVl, sl), d, y, padding);
-                st3L(sY, sl, y);
-                st3L(sVl, sl, v);
-                sD[sl] = d.x;
-                sD[CAP + sl] = d.y;
-                sD[2 * CAP + sl] = d.z;
-                sPad[sl] = padding;
-            }
-        }
-        __syncthreads();
-
-        if (P.ropeContact) {
-            // Rope-rope broad phase with skin (Verlet) lists. At a rebuild every segment records, in ascending
-            // order, the partners whose box (padded by the build radius R + skin) meets its own: a superset of
-            // every pair the exact test can accept until some vertex has moved more than the skin from its
-            // build position or some detection radius has grown past R (then the next substep rebuilds).
-            // Each substep the exact test of the reference runs over that list in order, so the candidate
-            // lists are the ones a full ascending scan of the contact group produces.
-            const float skin = 2.f * P.scaledDetR, slackR = 0.25f * P.scaledDetR;
-            bool viol = s == 0;
+            // init
 #pragma unroll
             for (int k = 0; k < VPT; k++) {
-                const int sl = t + k * BLOCK;
-                if (gi[k] >= 0 && s > 0) {
-                    const f3 dxv = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]) - mk3(sX0[sl], sX0[CAP + sl], sX0[2 * CAP + sl]);
-                    if (len2(dxv) > skin * skin) viol = true;
-                    if ((flags_of(prm[k]) & HAS_NEXT) && fmaxf(sPad[sl], sPad[sl + 1]) > sR[sl]) viol = true;
+                if (gi[k] >= 0) {
+                    const int sl = t + k * BLOCK;
+                    f3 d, y;
+                    float padding;
+                    const f3 v = ld3L(sVel, sl);
+                    init_vertex<F>(P, prm[k].x, v, ld3L(sVl, sl), d, y, padding);
+                    st3L(sY, sl, y);
+                    st3L(sVl, sl, v);
+                    sD[sl] = d.x;
+                    sD[CAP + sl] = d.y;
+                    sD[2 * CAP + sl] = d.z;
+                    sPad[sl] = padding;
                 }
             }
-            if (__syncthreads_or(viol)) {
-                // (1) build boxes padded by the build radius, chunk boxes and chunk group ranges
+            if (!P.ropeContact) __syncthreads();  // with rope contact the rebuild vote below is this barrier
+
+            if (P.ropeContact) {
+                // Rope-rope broad phase with skin (Verlet) lists. At a rebuild every segment records, in ascending
+                // order, the partners whose box (padded by the build radius R + skin) meets its own: a superset of
+                // every pair the exact test can accept until some vertex has moved more than the skin from its
+                // build position or some detection radius has grown past R (then the next substep rebuilds).
+                // Each substep the exact test of the reference runs over that list in order, so the candidate
+                // lists are the ones a full ascending scan of the contact group produces.
+                const float skin = 2.f * P.scaledDetR, slackR = 0.25f * P.scaledDetR;
+                // The vote reads only the vertex's own slots and the build radii (stable since the build): a
+                // segment's radius max(pad_i, pad_i+1) outgrows its bound exactly when one of its end vertices
+                // does, so each vertex checks its own padding against the segments on either side.
+                bool viol = s == 0;
 #pragma unroll
                 for (int k = 0; k < VPT; k++) {
                     const int sl = t + k * BLOCK;
-                    f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = mk3(-INFINITY, -INFINITY, -INFINITY);
-                    int cg0 = 1 << 30, cg1 = 0;
-                    if (gi[k] >= 0) {
-                        cg0 = grp[k] & 0xFFFF;
-                        cg1 = grp[k] >> 16;
-                        const f3 a = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-                        sX0[sl] = a.x;
-                        sX0[CAP + sl] = a.y;
-                        sX0[2 * CAP + sl] = a.z;
-                        if (flags_of(prm[k]) & HAS_NEXT) {
-                            const f3 b = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
-                            const float R = fmaxf(sPad[sl], sPad[sl + 1]) + slackR;
-                            sR[sl] = R;
-                            const float rb = R + skin + 1e-4f + 2e-6f * fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z)));
-                            lo = vmin(a, b) - mk3(rb, rb, rb);
-                            hi = vmax(a, b) + mk3(rb, rb, rb);
-                        }
-                    }
-                    sBx[3 * sl] = make_float2(lo.x, lo.y);
-                    sBx[3 * sl + 1] = make_float2(lo.z, hi.x);
-                    sBx[3 * sl + 2] = make_float2(hi.y, hi.z);
-#pragma unroll
-                    for (int o = 1; o < 16; o <<= 1) {
-                        lo = vmin(lo, mk3(__shfl_xor(lo.x, o), __shfl_xor(lo.y, o), __shfl_xor(lo.z, o)));
-                        hi = vmax(hi, mk3(__shfl_xor(hi.x, o), __shfl_xor(hi.y, o), __shfl_xor(hi.z, o)));
-                        cg0 = min(cg0, __shfl_xor(cg0, o));
-                        cg1 = max(cg1, __shfl_xor(cg1, o));
-                    }
-                    if ((t & 15) == 0) {
-                        const int ch = sl >> 4;
-                        sBox[ch] = lo.x;
-                        sBox[NCH + ch] = lo.y;
-                        sBox[2 * NCH + ch] = lo.z;
-                        sBox[3 * NCH + ch] = hi.x;
-                        sBox[4 * NCH + ch] = hi.y;
-                        sBox[5 * NCH + ch] = hi.z;
-                        sChG[ch] = cg0;
-                        sChG[NCH + ch] = cg1;
+                    if (gi[k] >= 0 && s > 0) {
+                        const f3 dxv = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]) - mk3(sX0[sl], sX0[CAP + sl], sX0[2 * CAP + sl]);
+                        if (len2(dxv) > skin * skin) viol = true;
+                        const uint32_t fl = flags_of(prm[k]);
+                        const float pad = sPad[sl];
+                        if ((fl & HAS_NEXT) && pad > sR[sl]) viol = true;
+                        if ((fl & HAS_PREV) && pad > sR[sl - 1]) viol = true;
                     }
                 }
-                if (t == 0) sWork[0] = BLOCK / 64;
-                __syncthreads();
-                const int nch = (C.nv + 15) >> 4;
-                // (2) chunk A meets chunk B (B within A's contact groups): 16 lanes per A, OR-reduced masks
-                for (int base = 0; base < nch * 16; base += BLOCK) {
-                    const int A = (base + t) >> 4, sub = t & 15;
-                    uint32_t mlo = 0, mhi = 0;
-                    if (A < nch && sChG[A] < sChG[NCH + A]) {
-                        const float lx = sBox[A], ly = sBox[NCH + A], lz = sBox[2 * NCH + A];
-                        const float hx = sBox[3 * NCH + A], hy = sBox[4 * NCH + A], hz = sBox[5 * NCH + A];
-                        const int b0 = sChG[A] >> 4, b1 = (sChG[NCH + A] - 1) >> 4;
-                        for (int B = b0 + sub; B <= b1; B += 16) {
-                            const bool ov = !(sBox[B] > hx || sBox[NCH + B] > hy || sBox[2 * NCH + B] > hz ||
-                                              sBox[3 * NCH + B] < lx || sBox[4 * NCH + B] < ly || sBox[5 * NCH + B] < lz);
-                            if (ov) {
-                                if (B < 32)
-                                    mlo |= 1u << B;
-                                else
-                                    mhi |= 1u << (B - 32);
+                if (__syncthreads_or(viol)) {  // also publishes the init phase
+                    // (1) build boxes padded by the build radius, chunk boxes and chunk group ranges
+#pragma unroll
+                    for (int k = 0; k < VPT; k++) {
+                        const int sl = t + k * BLOCK;
+                        f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = mk3(-INFINITY, -INFINITY, -INFINITY);
+                        int cg0 = 1 << 30, cg1 = 0;
+                        if (gi[k] >= 0) {
+                            cg0 = grp[k] & 0xFFFF;
+                            cg1 = grp[k] >> 16;
+                            const f3 a = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+                            sX0[sl] = a.x;
+                            sX0[CAP + sl] = a.y;
+                            sX0[2 * CAP + sl] = a.z;
+                            if (flags_of(prm[k]) & HAS_NEXT) {
+                                const f3 b = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
+                                const float R = fmaxf(sPad[sl], sPad[sl + 1]) + slackR;
+                                sR[sl] = R;
+                                const float rb = R + skin + 1e-4f + 2e-6f * fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z)));
+                                lo = vmin(a, b) - mk3(rb, rb, rb);
+                                hi = vmax(a, b) + mk3(rb, rb, rb);
                             }
                         }
-                    }
+                        sBx[3 * sl] = make_float2(lo.x, lo.y);
+                        sBx[3 * sl + 1] = make_float2(lo.z, hi.x);
+                        sBx[3 * sl + 2] = make_float2(hi.y, hi.z);
 #pragma unroll
-                    for (int o = 1; o < 16; o <<= 1) {
-                        mlo |= __shfl_xor(mlo, o);
-                        mhi |= __shfl_xor(mhi, o);
-                    }
-                    if (A < nch && sub == 0) {
-                        sOv[A] = mlo;
-                        sOv[NCH + A] = mhi;
-                    }
-                }
-                __syncthreads();
-                // (3) one wave per chunk A (its own first, then the next unclaimed one): lane (a, g) holds
-                //     segment i = 16 A + a and tests partners 4g..4g+3 of every overlapping chunk B in ascending
-                //     order; the four lanes of a segment append their hits in partner order (cross-lane prefix),
-                //     so each skin list ascends. The four partner boxes are read before any is compared, so the
-                //     twelve reads of a step are in flight together.
-                {
-                    const int lane = t & 63, wave = t >> 6;
-                    const int a = lane & 15, g = lane >> 4;
-                    for (int A = wave; A < nch;) {
-                        const int i = A * 16 + a;
-                        const uint32_t fi = i < C.nv ? sFl[i] : 0u;
-                        const bool seg = (fi & HAS_NEXT) != 0;
-                        const int g0 = (fi >> 4) & 0x7FF, g1 = (fi >> 15) & 0x7FF;
-                        const bool nextHN = seg && (sFl[i + 1] & HAS_NEXT);
-                        const bool prevHN = seg && (fi & HAS_PREV);
-                        f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = -lo;
-                        if (seg) {
-                            const float2 b0 = sBx[3 * i], b1 = sBx[3 * i + 1], b2 = sBx[3 * i + 2];
-                            lo = mk3(b0.x, b0.y, b1.x);
-                            hi = mk3(b1.y, b2.x, b2.y);
+                        for (int o = 1; o < 16; o <<= 1) {
+                            lo = vmin(lo, mk3(__shfl_xor(lo.x, o), __shfl_xor(lo.y, o), __shfl_xor(lo.z, o)));
+                            hi = vmax(hi, mk3(__shfl_xor(hi.x, o), __shfl_xor(hi.y, o), __shfl_xor(hi.z, o)));
+                            cg0 = min(cg0, __shfl_xor(cg0, o));
+                            cg1 = max(cg1, __shfl_xor(cg1, o));
                         }
-                        int cnt = 0;
-                        // A is the same on every lane, so the mask is too: the loop below runs on the scalar unit
-                        uint64_t mask = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)sOv[NCH + A]) << 32) |
-                                        (uint32_t)__builtin_amdgcn_readfirstlane((int)sOv[A]);
-                        uint16_t* vl = D.vlist + (size_t)C.v0 * VERLET_CAP + i;  // entry e at vl[e * CAP]
-                        while (mask) {
-                            const int B = __builtin_ctzll(mask);
-                            mask &= mask - 1;
-                            float2 e0[4], e1[4], e2[4];
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                const int jc = min(B * 16 + 4 * g + u, C.nv - 1);
-                                e0[u] = sBx[3 * jc];
-                                e1[u] = sBx[3 * jc + 1];
-                                e2[u] = sBx[3 * jc + 2];
-                            }
-                            uint32_t hm = 0;
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                const int j = B * 16 + 4 * g + u;
-                                const bool apart = (e0[u].x > hi.x) | (e0[u].y > hi.y) | (e1[u].x > hi.z) |
-                                                   (e1[u].y < lo.x) | (e2[u].x < lo.y) | (e2[u].y < lo.z);
-                                const int dj = j - i;
-                                const bool self = ((unsigned)(dj + 1) <= 2u) | ((dj == 2) & nextHN) | ((dj == -2) & prevHN);
-                                if (seg & !apart & !self & (j < C.nv) & (j >= g0) & (j < g1)) hm |= 1u << u;
-                            }
-                            const int c = __builtin_popcount(hm);
-                            const int c0 = __shfl(c, a), c1 = __shfl(c, a + 16), c2 = __shfl(c, a + 32), c3 = __shfl(c, a + 48);
-                            int idx = cnt + (g > 0 ? c0 : 0) + (g > 1 ? c1 : 0) + (g > 2 ? c2 : 0);
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                if (hm & (1u << u)) {
-                                    if (idx < VERLET_CAP) vl[(size_t)idx * CAP] = (uint16_t)(B * 16 + 4 * g + u);
-                                    idx++;
+                        if ((t & 15) == 0) {
+                            const int ch = sl >> 4;
+                            sBox[ch] = lo.x;
+                            sBox[NCH + ch] = lo.y;
+                            sBox[2 * NCH + ch] = lo.z;
+                            sBox[3 * NCH + ch] = hi.x;
+                            sBox[4 * NCH + ch] = hi.y;
+                            sBox[5 * NCH + ch] = hi.z;
+                            sChG[ch] = cg0;
+                            sChG[NCH + ch] = cg1;
+                        }
+                    }
+                    if (t == 0) sWork[0] = BLOCK / 64;
+                    __syncthreads();
+                    const int nch = (C.nv + 15) >> 4;
+                    // (2) chunk A meets chunk B (B within A's contact groups): 16 lanes per A, OR-reduced masks
+                    for (int base = 0; base < nch * 16; base += BLOCK) {
+                        const int A = (base + t) >> 4, sub = t & 15;
+                        uint32_t mlo = 0, mhi = 0;
+                        if (A < nch && sChG[A] < sChG[NCH + A]) {
+                            const float lx = sBox[A], ly = sBox[NCH + A], lz = sBox[2 * NCH + A];
+                            const float hx = sBox[3 * NCH + A], hy = sBox[4 * NCH + A], hz = sBox[5 * NCH + A];
+                            const int b0 = sChG[A] >> 4, b1 = (sChG[NCH + A] - 1) >> 4;
+                            for (int B = b0 + sub; B <= b1; B += 16) {
+                                const bool ov = !(sBox[B] > hx || sBox[NCH + B] > hy || sBox[2 * NCH + B] > hz ||
+                                                  sBox[3 * NCH + B] < lx || sBox[4 * NCH + B] < ly || sBox[5 * NCH + B] < lz);
+                                if (ov) {
+                                    if (B < 32)
+                                        mlo |= 1u << B;
+                                    else
+                                        mhi |= 1u << (B - 32);
                                 }
                             }
-                            cnt += c0 + c1 + c2 + c3;
                         }
-                        if (g == 0 && i < C.nv) sNcol[i] = cnt <= VERLET_CAP ? cnt : -1;
-                        if (stats && g == 0 && seg) atomicAdd(stats, (unsigned long long)cnt);
-                        int claim = 0;
-                        if (lane == 0) claim = atomicAdd(sWork, 1);
-                        A = __builtin_amdgcn_readfirstlane(claim);
+#pragma unroll
+                        for (int o = 1; o < 16; o <<= 1) {
+                            mlo |= __shfl_xor(mlo, o);
+                            mhi |= __shfl_xor(mhi, o);
+                        }
+                        if (A < nch && sub == 0) {
+                            sOv[A] = mlo;
+                            sOv[NCH + A] = mhi;
+                        }
+                    }
+                    __syncthreads();
+                    // (3) one wave per chunk A (its own first, then the next unclaimed one): lane (a, g) holds
+                    //     segment i = 16 A + a and tests partners 4g..4g+3 of every overlapping chunk B in ascending
+                    //     order; the four lanes of a segment append their hits in partner order (cross-lane prefix),
+                    //     so each skin list ascends. The four partner boxes are read before any is compared, so the
+                    //     twelve reads of a step are in flight together.
+                    {
+                        const int lane = t & 63, wave = t >> 6;
+                        const int a = lane & 15, g = lane >> 4;
+                        for (int A = wave; A < nch;) {
+                            const int i = A * 16 + a;
+                            const uint32_t fi = i < C.nv ? sFl[i] : 0u;
+                            const bool seg = (fi & HAS_NEXT) != 0;
+                            const int g0 = (fi >> 4) & 0x7FF, g1 = (fi >> 15) & 0x7FF;
+                            const bool nextHN = seg && (sFl[i + 1] & HAS_NEXT);
+                            const bool prevHN = seg && (fi & HAS_PREV);
+                            f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = -lo;
+                            if (seg) {
+                                const float2 b0 = sBx[3 * i], b1 = sBx[3 * i + 1], b2 = sBx[3 * i + 2];
+                                lo = mk3(b0.x, b0.y, b1.x);
+                                hi = mk3(b1.y, b2.x, b2.y);
+                            }
+                            int cnt = 0;
+                            // A is the same on every lane, so the mask is too: the loop below runs on the scalar unit
+                            uint64_t mask = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)sOv[NCH + A]) << 32) |
+                                            (uint32_t)__builtin_amdgcn_readfirstlane((int)sOv[A]);
+                            uint16_t* vl = D.vlist + (size_t)C.v0 * VERLET_CAP + i;  // entry e at vl[e * CAP]
+                            while (mask) {
+                                const int B = __builtin_ctzll(mask);
+                                mask &= mask - 1;
+                                float2 e0[4], e1[4], e2[4];
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    const int jc = min(B * 16 + 4 * g + u, C.nv - 1);
+                                    e0[u] = sBx[3 * jc];
+                                    e1[u] = sBx[3 * jc + 1];
+                                    e2[u] = sBx[3 * jc + 2];
+                                }
+                                uint32_t hm = 0;
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    const int j = B * 16 + 4 * g + u;
+                                    const bool apart = (e0[u].x > hi.x) | (e0[u].y > hi.y) | (e1[u].x > hi.z) |
+                                                       (e1[u].y < lo.x) | (e2[u].x < lo.y) | (e2[u].y < lo.z);
+                                    const int dj = j - i;
+                                    const bool self = ((unsigned)(dj + 1) <= 2u) | ((dj == 2) & nextHN) | ((dj == -2) & prevHN);
+                                    if (seg & !apart & !self & (j < C.nv) & (j >= g0) & (j < g1)) hm |= 1u << u;
+                                }
+                                const int c = __builtin_popcount(hm);
+                                const int c0 = __shfl(c, a), c1 = __shfl(c, a + 16), c2 = __shfl(c, a + 32), c3 = __shfl(c, a + 48);
+                                int idx = cnt + (g > 0 ? c0 : 0) + (g > 1 ? c1 : 0) + (g > 2 ? c2 : 0);
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    if (hm & (1u << u)) {
+                                        if (idx < VERLET_CAP) vl[(size_t)idx * CAP] = (uint16_t)(B * 16 + 4 * g + u);
+                                        idx++;
+                                    }
+                                }
+                                cnt += c0 + c1 + c2 + c3;
+                            }
+                            if (g == 0 && i < C.nv) sNcol[i] = cnt <= VERLET_CAP ? cnt : -1;
+                            if (stats && g == 0 && seg) atomicAdd(stats, (unsigned long long)cnt);
+                            int claim = 0;
+                            if (lane == 0) claim = atomicAdd(sWork, 1);
+                            A = __builtin_amdgcn_readfirstlane(claim);
+                        }
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int k = 0; k < VPT; k++)
+                        if (gi[k] >= 0) vcnt[k] = sNcol[t + k * BLOCK];
+                    if (stats && t == 0) atomicAdd(stats + 3, 1ull);
+                }
+                // exact tests over the skin list (or over the whole contact group after a skin-list overflow)
+                float minD[VPT];
+#pragma unroll
+                for (int k = 0; k < VPT; k++) {
+                    const int sl = t + k * BLOCK;
+                    float minDist = INFINITY;
+                    ncol[k] = 0;
+                    pairs[k] = 0;
+                    if (gi[k] >= 0 && (flags_of(prm[k]) & HAS_NEXT)) {
+                        const int i = gi[k];
+                        const f3 a0 = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+                        const f3 x1 = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
+                        const f3 a1 = x1 - a0;
+                        const float ri = fmaxf(sPad[sl], sPad[sl + 1]);
+                        minDist = sPad[sl] - P.detR;
+                        const float mg = 1e-4f + 2e-6f * fmaxf(fabsf(a0.x), fmaxf(fabsf(a0.y), fabsf(a0.z)));
+                        const float rm = ri + mg;
+                        const f3 lo = vmin(a0, x1) - mk3(rm, rm, rm), hi = vmax(a0, x1) + mk3(rm, rm, rm);
+                        const uint16_t* vl = D.vlist + (size_t)C.v0 * VERLET_CAP + sl;
+                        uint16_t* cl = D.clist + (size_t)C.v0 * MAX_COLL + sl;
+                        const bool full = vcnt[k] < 0;
+                        const int g0 = grp[k] & 0xFFFF, g1 = grp[k] >> 16;
+                        int rs0 = 0, rs1 = 0;
+                        if (full) {  // local range of the vertex's own rope
+                            rs0 = D.roff[rope[k]] - (i - sl);
+                            rs1 = D.roff[rope[k] + 1] - (i - sl);
+                        }
+                        const int n = full ? g1 - g0 : vcnt[k];
+                        int cnt = 0, tests = 0;
+                        // skin entries come from global memory in batches of four loads issued together, so a long
+                        // list waits once per batch instead of once per entry
+                        for (int cb = 0; cb < n; cb += 4) {
+                            int jb[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++)
+                                jb[u] = full ? g0 + cb + u : (cb + u < n ? (int)vl[(size_t)(cb + u) * CAP] : 0);
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const int j = jb[u];
+                                if (cb + u >= n) break;
+                                if (full && (!(sFl[j] & HAS_NEXT) || (j >= rs0 && j < rs1 && abs(j - sl) <= 2))) continue;
+                                const f3 c0 = mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]);
+                                const f3 c1 = mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]);
+                                const float rj = fmaxf(sPad[j], sPad[j + 1]);
+                                const f3 blo = vmin(c0, c1), bhi = vmax(c0, c1);
+                                if (blo.x - rj > hi.x || blo.y - rj > hi.y || blo.z - rj > hi.z || bhi.x + rj < lo.x ||
+                                    bhi.y + rj < lo.y || bhi.z + rj < lo.z)
+                                    continue;
+                                tests++;
+                                float d2;
+                                if (pair_hit(a1, c0 - a0, c1 - a0, ri + rj, d2)) {
+                                    if (cnt < MAX_COLL) {
+                                        cl[(size_t)cnt * CAP] = (uint16_t)j;
+                                        minDist = fminf(minDist, 0.45f * sqrtf(d2));
+                                    }
+                                    if (cnt < 2) pairs[k] |= (uint32_t)j << (16 * cnt);
+                                    cnt++;
+                                }
+                            }
+                        }
+                        if (stats) {  // statistics only: [skin entries at builds, exact tests, list entries, builds]
+                            atomicAdd(stats + 1, (unsigned long long)tests);
+                            atomicAdd(stats + 2, (unsigned long long)cnt);
+                        }
+                        if (cnt > MAX_COLL) D.err[0] = ERR_MAX_COLLISIONS;
+                        ncol[k] = min(cnt, MAX_COLL);
+                    }
+                    minD[k] = minDist;
+                }
+                // sLim has a region of its own (last read in the previous substep's first iteration): the first
+                // iteration's barrier publishes the limits, and its solve phase reads the neighbour's
+#pragma unroll
+                for (int k = 0; k < VPT; k++) sLim[t + k * BLOCK] = minD[k];
+            }
+
+            // numItr x (position sweep, orientation sweep). A wave with rope-rope pairs this substep issues ahead
+            // of the others on its SIMD: the barriers wait for it.
+            if (P.ropeContact) {
+                bool pairs = false;
+#pragma unroll
+                for (int k = 0; k < VPT; k++) pairs = pairs || ncol[k] > 0;
+                if (__any(pairs))
+                    __builtin_amdgcn_s_setprio(2);
+                else
+                    __builtin_amdgcn_s_setprio(0);
+            }
+            for (int it = 0; it < P.numItr; it++) {
+                f3 fk[VPT];
+                s3 Hk[VPT];
+#pragma unroll
+                for (int k = 0; k < VPT; k++) {
+                    const int sl = t + k * BLOCK;
+                    f3 f2 = mk3(0.f, 0.f, 0.f);
+                    s3 H2 = szero();
+                    fk[k] = f2;
+                    Hk[k] = H2;
+                    if (gi[k] >= 0) {
+                        const uint32_t fl = flags_of(prm[k]);
+                        const f3 p0 = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+                        const f3 d0 = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
+                        f3 p1 = p0, p1dx = d0;
+                        if (fl & HAS_NEXT) {
+                            p1 = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
+                            p1dx = mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]);
+                        }
+                        vertex_terms<F>(P, D.grids, cell_poses(P, D, rope[k], s), nc[k], cand[k], prm[k].x, prm[k].y,
+                                        prm[k].z, fl, p0, d0, ld3L(sY, sl), qo[k], p1, p1dx, fk[k], Hk[k], f2, H2);
+                        if (P.ropeContact && (fl & HAS_NEXT)) {
+                            const float dmp = Ar<F>::div(P.damping, h);
+                            const uint16_t* lst = D.clist + (size_t)C.v0 * MAX_COLL + sl;
+                            for (int c = 0; c < ncol[k]; c++) {
+                                const int j = c < 2 ? (int)((pairs[k] >> (16 * c)) & 0xFFFFu) : (int)lst[(size_t)c * CAP];
+                                collision_term<F>(P, dmp, p0, p1, d0, p1dx, mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]),
+                                                  mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]),
+                                                  mk3(sD[j], sD[CAP + j], sD[2 * CAP + j]),
+                                                  mk3(sD[j + 1], sD[CAP + j + 1], sD[2 * CAP + j + 1]), fk[k], Hk[k], f2, H2);
+                            }
+                        }
+                    }
+                    sF[sl] = f2.x;
+                    sF[CAP + sl] = f2.y;
+                    sF[2 * CAP + sl] = f2.z;
+#pragma unroll
+                    for (int c = 0; c < 6; c++) sF[(3 + c) * CAP + sl] = H2.m[c];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < VPT; k++) {
+                    const int sl = t + k * BLOCK;
+                    if (gi[k] >= 0) {
+                        const uint32_t fl = flags_of(prm[k]);
+                        if (P.ropeContact && it == 0) {  // step limit of the substep: own and previous segment
+                            limEff[k] = sLim[sl];
+                            if (fl & HAS_PREV) limEff[k] = fminf(limEff[k], sLim[sl - 1]);
+                        }
+                        f3 f = fk[k];
+                        s3 H = Hk[k];
+                        if (fl & HAS_PREV) {
+                            f = f + mk3(sF[sl - 1], sF[CAP + sl - 1], sF[2 * CAP + sl - 1]);
+                            s3 Hp;
+#pragma unroll
+                            for (int c = 0; c < 6; c++) Hp.m[c] = sF[(3 + c) * CAP + sl - 1];
+                            sadd(H, Hp);
+                        }
+                        if (prm[k].x != 0.f) {
+                            f3 d = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
+                            d = solve_update<F>(P, d, H, f, P.ropeContact != 0, limEff[k]);
+                            sD[sl] = d.x;
+                            sD[CAP + sl] = d.y;
+                            sD[2 * CAP + sl] = d.z;
+                        }
+                        // publish the frames of the previous orientation sweep (at it = 0 those of the previous substep
+                        // or of the root program)
+                        sQ[sl] = stq(qo[k]);
                     }
                 }
                 __syncthreads();
+#pragma unroll
+                for (int k = 0; k < VPT; k++) {
+                    const int sl = t + k * BLOCK;
+                    if (gi[k] >= 0) {
+                        const uint32_t fl = flags_of(prm[k]);
+                        if (!(fl & FIX_ORIENT) && (fl & HAS_NEXT)) {
+                            const q4 q0 = ldq(sQ[sl]);
+                            const q4 qp = (fl & HAS_PREV) ? ldq(sQ[sl - 1]) : q0;
+                            const q4 qn = (fl & HAS_NEXT_ORIENT) ? ldq(sQ[sl + 1]) : q0;
+                            qo[k] = orient_update<F>(fl, prm[k].y, prm[k].z, mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
+                                                  mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]),
+                                                  mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]),
+                                                  mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]), qp, q0, qn,
+                                                  ldq(sQr[(fl & HAS_PREV) ? sl - 1 : sl]), ldq(sQr[sl]));
+                        }
+                    }
+                }
+            }
+            __syncthreads();  // the last sweep's neighbour reads are done before the end phase moves the positions
+            if (P.dragQuad > 0.f) {  // the tips' drag tangent: the previous segment's frame after the last sweep
 #pragma unroll
                 for (int k = 0; k < VPT; k++)
-                    if (gi[k] >= 0) vcnt[k] = sNcol[t + k * BLOCK];
-                if (stats && t == 0) atomicAdd(stats + 3, 1ull);
+                    if (gi[k] >= 0) sQ[t + k * BLOCK] = stq(qo[k]);
+                __syncthreads();
             }
-            // exact tests over the skin list (or over the whole contact group after a skin-list overflow)
-            float minD[VPT];
-#pragma unroll
-            for (int k = 0; k < VPT; k++) {
-                const int sl = t + k * BLOCK;
-                float minDist = INFINITY;
-                ncol[k] = 0;
-                pairs[k] = 0;
-                if (gi[k] >= 0 && (flags_of(prm[k]) & HAS_NEXT)) {
-                    const int i = gi[k];
-                    const f3 a0 = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-                    const f3 x1 = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
-                    const f3 a1 = x1 - a0;
-                    const float ri = fmaxf(sPad[sl], sPad[sl + 1]);
-                    minDist = sPad[sl] - P.detR;
-                    const float mg = 1e-4f + 2e-6f * fmaxf(fabsf(a0.x), fmaxf(fabsf(a0.y), fabsf(a0.z)));
-                    const float rm = ri + mg;
-                    const f3 lo = vmin(a0, x1) - mk3(rm, rm, rm), hi = vmax(a0, x1) + mk3(rm, rm, rm);
-                    const uint16_t* vl = D.vlist + (size_t)C.v0 * VERLET_CAP + sl;
-                    uint16_t* cl = D.clist + (size_t)C.v0 * MAX_COLL + sl;
-                    const bool full = vcnt[k] < 0;
-                    const int g0 = grp[k] & 0xFFFF, g1 = grp[k] >> 16;
-                    int rs0 = 0, rs1 = 0;
-                    if (full) {  // local range of the vertex's own rope
-                        rs0 = D.roff[rope[k]] - (i - sl);
-                        rs1 = D.roff[rope[k] + 1] - (i - sl);
-                    }
-                    const int n = full ? g1 - g0 : vcnt[k];
-                    int cnt = 0, tests = 0;
-                    // skin entries come from global memory in batches of four loads issued together, so a long
-                    // list waits once per batch instead of once per entry
-                    for (int cb = 0; cb < n; cb += 4) {
-                        int jb[4];
-#pragma unroll
-                        for (int u = 0; u < 4; u++)
-                            jb[u] = full ? g0 + cb + u : (cb + u < n ? (int)vl[(size_t)(cb + u) * CAP] : 0);
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const int j = jb[u];
-                            if (cb + u >= n) break;
-                            if (full && (!(sFl[j] & HAS_NEXT) || (j >= rs0 && j < rs1 && abs(j - sl) <= 2))) continue;
-                            const f3 c0 = mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]);
-                            const f3 c1 = mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]);
-                            const float rj = fmaxf(sPad[j], sPad[j + 1]);
-                            const f3 blo = vmin(c0, c1), bhi = vmax(c0, c1);
-                            if (blo.x - rj > hi.x || blo.y - rj > hi.y || blo.z - rj > hi.z || bhi.x + rj < lo.x ||
-                                bhi.y + rj < lo.y || bhi.z + rj < lo.z)
-                                continue;
-                            tests++;
-                            float d2;
-                            if (pair_hit(a1, c0 - a0, c1 - a0, ri + rj, d2)) {
-                                if (cnt < MAX_COLL) {
-                                    cl[(size_t)cnt * CAP] = (uint16_t)j;
-                                    minDist = fminf(minDist, 0.45f * sqrtf(d2));
-                                }
-                                if (cnt < 2) pairs[k] |= (uint32_t)j << (16 * cnt);
-                                cnt++;
-                            }
-                        }
-                    }
-                    if (stats) {  // statistics only: [skin entries at builds, exact tests, list entries, builds]
-                        atomicAdd(stats + 1, (unsigned long long)tests);
-                        atomicAdd(stats + 2, (unsigned long long)cnt);
-                    }
-                    if (cnt > MAX_COLL) D.err[0] = ERR_MAX_COLLISIONS;
-                    ncol[k] = min(cnt, MAX_COLL);
-                }
-                minD[k] = minDist;
-            }
-            // sLim has a region of its own (last read before the previous substep's iterations), so one
-            // barrier publishes the limits to the neighbours
-#pragma unroll
-            for (int k = 0; k < VPT; k++) sLim[t + k * BLOCK] = minD[k];
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < VPT; k++) {
-                const int sl = t + k * BLOCK;
-                if (gi[k] >= 0) {
-                    limEff[k] = sLim[sl];
-                    if (flags_of(prm[k]) & HAS_PREV) limEff[k] = fminf(limEff[k], sLim[sl - 1]);
-                }
-            }
-        }
 
-        // numItr x (position sweep, orientation sweep). A wave with rope-rope pairs this substep issues ahead
-        // of the others on its SIMD: the barriers wait for it.
-        if (P.ropeContact) {
-            bool pairs = false;
+            // end of substep: force readout (every lane, for the wave-wide wrench sums), then velocity and commit
+            if (FORCES) {
 #pragma unroll
-            for (int k = 0; k < VPT; k++) pairs = pairs || ncol[k] > 0;
-            if (__any(pairs))
-                __builtin_amdgcn_s_setprio(2);
-            else
-                __builtin_amdgcn_s_setprio(0);
-        }
-        for (int it = 0; it < P.numItr; it++) {
-            f3 fk[VPT];
-            s3 Hk[VPT];
-#pragma unroll
-            for (int k = 0; k < VPT; k++) {
-                const int sl = t + k * BLOCK;
-                f3 f2 = mk3(0.f, 0.f, 0.f);
-                s3 H2 = szero();
-                fk[k] = f2;
-                Hk[k] = H2;
-                if (gi[k] >= 0) {
-                    const uint32_t fl = flags_of(prm[k]);
-                    const f3 p0 = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-                    const f3 d0 = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
-                    f3 p1 = p0, p1dx = d0;
-                    if (fl & HAS_NEXT) {
-                        p1 = mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]);
-                        p1dx = mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]);
-                    }
-                    vertex_terms<F>(P, D.grids, cell_poses(P, D, rope[k], s), nc[k], cand[k], prm[k].x, prm[k].y,
-                                    prm[k].z, fl, p0, d0, ld3L(sY, sl), qo[k], p1, p1dx, fk[k], Hk[k], f2, H2);
-                    if (P.ropeContact && (fl & HAS_NEXT)) {
-                        const float dmp = Ar<F>::div(P.damping, h);
-                        const uint16_t* lst = D.clist + (size_t)C.v0 * MAX_COLL + sl;
-                        for (int c = 0; c < ncol[k]; c++) {
-                            const int j = c < 2 ? (int)((pairs[k] >> (16 * c)) & 0xFFFFu) : (int)lst[(size_t)c * CAP];
-                            collision_term<F>(P, dmp, p0, p1, d0, p1dx, mk3(sX[j], sX[CAP + j], sX[2 * CAP + j]),
-                                              mk3(sX[j + 1], sX[CAP + j + 1], sX[2 * CAP + j + 1]),
-                                              mk3(sD[j], sD[CAP + j], sD[2 * CAP + j]),
-                                              mk3(sD[j + 1], sD[CAP + j + 1], sD[2 * CAP + j + 1]), fk[k], Hk[k], f2, H2);
-                        }
+                for (int k = 0; k < VPT; k++) {
+                    const int sl = t + k * BLOCK;
+                    const bool on = gi[k] >= 0 && prm[k].x != 0.f;
+                    const float kap = Ar<F>::div(P.cKappa, h);
+                    const f3 fv = surface_readout<F>(P, D, cell_poses(P, D, rope[k], s), rope[k], on, nc[k], cand[k],
+                                                     mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
+                                                     mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]),
+                                                     on ? Ar<F>::div(kap * kap, prm[k].x) : 0.f);
+                    if (on) {
+                        float* fo = D.force + 3 * (size_t)gi[k];
+                        fo[0] = fo[0] + fv.x;
+                        fo[1] = fo[1] + fv.y;
+                        fo[2] = fo[2] + fv.z;
                     }
                 }
-                sF[sl] = f2.x;
-                sF[CAP + sl] = f2.y;
-                sF[2 * CAP + sl] = f2.z;
-#pragma unroll
-                for (int c = 0; c < 6; c++) sF[(3 + c) * CAP + sl] = H2.m[c];
             }
-            __syncthreads();
 #pragma unroll
             for (int k = 0; k < VPT; k++) {
-                const int sl = t + k * BLOCK;
                 if (gi[k] >= 0) {
+                    const int sl = t + k * BLOCK;
                     const uint32_t fl = flags_of(prm[k]);
-                    f3 f = fk[k];
-                    s3 H = Hk[k];
-                    if (fl & HAS_PREV) {
-                        f = f + mk3(sF[sl - 1], sF[CAP + sl - 1], sF[2 * CAP + sl - 1]);
-                        s3 Hp;
-#pragma unroll
-                        for (int c = 0; c < 6; c++) Hp.m[c] = sF[(3 + c) * CAP + sl - 1];
-                        sadd(H, Hp);
-                    }
+                    const f3 d = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
+                    const f3 x = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
                     if (prm[k].x != 0.f) {
-                        f3 d = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
-                        d = solve_update<F>(P, d, H, f, P.ropeContact != 0, limEff[k]);
-                        sD[sl] = d.x;
-                        sD[CAP + sl] = d.y;
-                        sD[2 * CAP + sl] = d.z;
+                        f3 tg = mk3(1.f, 0.f, 0.f);
+                        if (P.dragQuad > 0.f) tg = V<F>::qrot_ex((fl & HAS_NEXT) ? qo[k] : ldq(sQ[sl - 1]));
+                        st3L(sVel, sl, end_velocity<F>(P, d, tg));
                     }
-                    // publish the frames of the previous orientation sweep (at it = 0 those of the previous substep
-                    // or of the root program)
-                    sQ[sl] = stq(qo[k]);
+                    const f3 xn = x + d;
+                    sX[sl] = xn.x;
+                    sX[CAP + sl] = xn.y;
+                    sX[2 * CAP + sl] = xn.z;
                 }
             }
-            __syncthreads();
+            // mesh broad phase for the next substep (YB/sim/contact.cuh:121-136). With many meshes, chunks of 16 lanes
+            // on one rope split the meshes over their lanes first, then each vertex runs the exact test on the chunk's
+            // survivors only; with a few, each vertex tests them all.
+            if (P.numSdf > 0 && P.numSdf <= CHUNK_BROADPHASE_MIN_MESHES) {
 #pragma unroll
-            for (int k = 0; k < VPT; k++) {
-                const int sl = t + k * BLOCK;
-                if (gi[k] >= 0) {
-                    const uint32_t fl = flags_of(prm[k]);
-                    if (!(fl & FIX_ORIENT) && (fl & HAS_NEXT)) {
-                        const q4 q0 = ldq(sQ[sl]);
-                        const q4 qp = (fl & HAS_PREV) ? ldq(sQ[sl - 1]) : q0;
-                        const q4 qn = (fl & HAS_NEXT_ORIENT) ? ldq(sQ[sl + 1]) : q0;
-                        qo[k] = orient_update<F>(fl, prm[k].y, prm[k].z, mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
-                                              mk3(sX[sl + 1], sX[CAP + sl + 1], sX[2 * CAP + sl + 1]),
-                                              mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]),
-                                              mk3(sD[sl + 1], sD[CAP + sl + 1], sD[2 * CAP + sl + 1]), qp, q0, qn,
-                                              ldq(sQr[(fl & HAS_PREV) ? sl - 1 : sl]), ldq(sQr[sl]));
+                for (int k = 0; k < VPT; k++) {
+                    const int sl = t + k * BLOCK;
+                    if (gi[k] >= 0)
+                        nc[k] = prm[k].x != 0.f ? build_candidates(P, cell_poses(P, D, rope[k], s),
+                                                                   mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
+                                                                   2.f * len(mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl])) +
+                                                                       0.02f,
+                                                                   cand[k])
+                                                : 0;
+                }
+            } else if (P.numSdf > 0) {
+#pragma unroll
+                for (int k = 0; k < VPT; k++) {
+                    const int sl = t + k * BLOCK;
+                    const bool act = gi[k] >= 0 && prm[k].x != 0.f;
+                    const f3 xn = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
+                    const float padv = 2.f * len(mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl])) + 0.02f;
+                    int r0 = act ? rope[k] : 0x7fffffff, r1 = act ? rope[k] : -1;
+#pragma unroll
+                    for (int o = 1; o < 16; o <<= 1) {
+                        r0 = min(r0, __shfl_xor(r0, o));
+                        r1 = max(r1, __shfl_xor(r1, o));
                     }
+                    const bool oneRope = r0 == r1;  // false also for a chunk without free vertices
+                    const Pose* chunkPoses = cell_poses(P, D, oneRope ? r0 : 0, s);
+                    const uint64_t mask = chunk_mesh_mask(P, chunkPoses, act && oneRope, xn, padv);
+                    if (act)
+                        nc[k] = oneRope ? build_candidates_masked(P, chunkPoses, xn, padv, mask, cand[k])
+                                        : build_candidates(P, cell_poses(P, D, rope[k], s), xn, padv, cand[k]);
+                    else if (gi[k] >= 0)
+                        nc[k] = 0;
                 }
             }
-        }
-        __syncthreads();  // the last sweep's neighbour reads are done before the end phase moves the positions
-        if (P.dragQuad > 0.f) {  // the tips' drag tangent: the previous segment's frame after the last sweep
-#pragma unroll
-            for (int k = 0; k < VPT; k++)
-                if (gi[k] >= 0) sQ[t + k * BLOCK] = stq(qo[k]);
-            __syncthreads();
+            // no barrier here: the end phase touched only each vertex's own slots, and the next substep reads a
+            // neighbour's slot only after its init barrier
         }
 
-        // end of substep: force readout (every lane, for the wave-wide wrench sums), then velocity and commit
-        if (FORCES) {
-#pragma unroll
-            for (int k = 0; k < VPT; k++) {
-                const int sl = t + k * BLOCK;
-                const bool on = gi[k] >= 0 && prm[k].x != 0.f;
-                const float kap = Ar<F>::div(P.cKappa, h);
-                const f3 fv = surface_readout<F>(P, D, cell_poses(P, D, rope[k], s), rope[k], on, nc[k], cand[k],
-                                                 mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
-                                                 mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]),
-                                                 on ? Ar<F>::div(kap * kap, prm[k].x) : 0.f);
-                if (on) {
-                    float* fo = D.force + 3 * (size_t)gi[k];
-                    fo[0] = fo[0] + fv.x;
-                    fo[1] = fo[1] + fv.y;
-                    fo[2] = fo[2] + fv.z;
-                }
-            }
-        }
+        // write the state back, fill the views
 #pragma unroll
         for (int k = 0; k < VPT; k++) {
             if (gi[k] >= 0) {
                 const int sl = t + k * BLOCK;
-                const uint32_t fl = flags_of(prm[k]);
-                const f3 d = mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl]);
+                const int i = gi[k];
                 const f3 x = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-                if (prm[k].x != 0.f) {
-                    f3 tg = mk3(1.f, 0.f, 0.f);
-                    if (P.dragQuad > 0.f) tg = V<F>::qrot_ex((fl & HAS_NEXT) ? qo[k] : ldq(sQ[sl - 1]));
-                    st3L(sVel, sl, end_velocity<F>(P, d, tg));
+                const q4 q = qo[k];
+                D.x[i] = st3(x);
+                const f3 vel = ld3L(sVel, sl);
+                D.v[i] = st3(vel);
+                D.vlast[i] = st3(ld3L(sVl, sl));
+                D.q[i] = stq(q);
+                D.outPos[3 * i] = x.x;
+                D.outPos[3 * i + 1] = x.y;
+                D.outPos[3 * i + 2] = x.z;
+                D.outVel[3 * i] = vel.x;
+                D.outVel[3 * i + 1] = vel.y;
+                D.outVel[3 * i + 2] = vel.z;
+                D.outQuat[4 * i] = q.x;
+                D.outQuat[4 * i + 1] = q.y;
+                D.outQuat[4 * i + 2] = q.z;
+                D.outQuat[4 * i + 3] = q.w;
+                if (P.numSdf > 0) {
+                    D.ncand[i] = nc[k];
+                    for (int c = 0; c < MAX_CAND; c++)
+                        D.cand[(size_t)i * MAX_CAND + c] = (nc[k] > 0 && c < nc[k]) ? cand_get(cand[k], c) : -1;
                 }
-                const f3 xn = x + d;
-                sX[sl] = xn.x;
-                sX[CAP + sl] = xn.y;
-                sX[2 * CAP + sl] = xn.z;
             }
         }
-        // mesh broad phase for the next substep (YB/sim/contact.cuh:121-136). With many meshes, chunks of 16 lanes
-        // on one rope split the meshes over their lanes first, then each vertex runs the exact test on the chunk's
-        // survivors only; with a few, each vertex tests them all.
-        if (P.numSdf > 0 && P.numSdf <= CHUNK_BROADPHASE_MIN_MESHES) {
-#pragma unroll
-            for (int k = 0; k < VPT; k++) {
-                const int sl = t + k * BLOCK;
-                if (gi[k] >= 0)
-                    nc[k] = prm[k].x != 0.f ? build_candidates(P, cell_poses(P, D, rope[k], s),
-                                                               mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]),
-                                                               2.f * len(mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl])) +
-                                                                   0.02f,
-                                                               cand[k])
-                                            : 0;
-            }
-        } else if (P.numSdf > 0) {
-#pragma unroll
-            for (int k = 0; k < VPT; k++) {
-                const int sl = t + k * BLOCK;
-                const bool act = gi[k] >= 0 && prm[k].x != 0.f;
-                const f3 xn = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-                const float padv = 2.f * len(mk3(sD[sl], sD[CAP + sl], sD[2 * CAP + sl])) + 0.02f;
-                int r0 = act ? rope[k] : 0x7fffffff, r1 = act ? rope[k] : -1;
-#pragma unroll
-                for (int o = 1; o < 16; o <<= 1) {
-                    r0 = min(r0, __shfl_xor(r0, o));
-                    r1 = max(r1, __shfl_xor(r1, o));
-                }
-                const bool oneRope = r0 == r1;  // false also for a chunk without free vertices
-                const Pose* chunkPoses = cell_poses(P, D, oneRope ? r0 : 0, s);
-                const uint64_t mask = chunk_mesh_mask(P, chunkPoses, act && oneRope, xn, padv);
-                if (act)
-                    nc[k] = oneRope ? build_candidates_masked(P, chunkPoses, xn, padv, mask, cand[k])
-                                    : build_candidates(P, cell_poses(P, D, rope[k], s), xn, padv, cand[k]);
-                else if (gi[k] >= 0)
-                    nc[k] = 0;
-            }
-        }
-        // no barrier here: the end phase touched only each vertex's own slots, and the next substep reads a
-        // neighbour's slot only after its init barrier
-    }
-
-    // write the state back, fill the views
-#pragma unroll
-    for (int k = 0; k < VPT; k++) {
-        if (gi[k] >= 0) {
-            const int sl = t + k * BLOCK;
-            const int i = gi[k];
-            const f3 x = mk3(sX[sl], sX[CAP + sl], sX[2 * CAP + sl]);
-            const q4 q = qo[k];
-            D.x[i] = st3(x);
-            const f3 vel = ld3L(sVel, sl);
-            D.v[i] = st3(vel);
-            D.vlast[i] = st3(ld3L(sVl, sl));
-            D.q[i] = stq(q);
-            D.outPos[3 * i] = x.x;
-            D.outPos[3 * i + 1] = x.y;
-            D.outPos[3 * i + 2] = x.z;
-            D.outVel[3 * i] = vel.x;
-            D.outVel[3 * i + 1] = vel.y;
-            D.outVel[3 * i + 2] = vel.z;
-            D.outQuat[4 * i] = q.x;
-            D.outQuat[4 * i + 1] = q.y;
-            D.outQuat[4 * i + 2] = q.z;
-            D.outQuat[4 * i + 3] = q.w;
-            if (P.numSdf > 0) {
-                D.ncand[i] = nc[k];
-                for (int c = 0; c < MAX_CAND; c++)
-                    D.cand[(size_t)i * MAX_CAND + c] = (nc[k] > 0 && c < nc[k]) ? cand_get(cand[k], c) : -1;
-            }
-        }
-    }
-    // the cell's duration orders the next frame's launch (k_rank_cells)
-    if (D.cellTime) {
+        // the cell's duration orders the next frame's launch (k_rank_cells); the barrier also keeps the next
+        // claim from overwriting LDS that this cell still reads
         __syncthreads();
-        if (t == 0) D.cellTime[cellIdx] = (unsigned)(__builtin_amdgcn_s_memrealtime() - startTick);
+        if (D.cellTime && t == 0) D.cellTime[cellIdx] = (unsigned)(__builtin_amdgcn_s_memrealtime() - startTick);
     }
 }
 
 // Launch order of the cells for the next frame, longest first by the durations of this frame: cell c goes
-// to position #{c' : time[c'] > time[c], or time[c'] == time[c] and c' < c}, a permutation. Workgroups start
-// in launch order, so the slowest cells start first and the short ones fill in behind them (longest
+// to position #{c' : time[c'] > time[c], or time[c'] == time[c] and c' < c}, a permutation. Workgroups claim
+// cells in this order, so the slowest cells start first and the short ones fill in behind them (longest
 // processing time first). The order decides only where and when a cell runs, never its results.
 __global__ __launch_bounds__(256) void k_rank_cells(const unsigned* time, int n, int* order) {
     __shared__ unsigned tile[256];
@@ -1187,13 +1200,31 @@ static hipError_t launch_cell_f(const Params& P, const Dev& D, const Drive& Dr, 
                                 int ncells, int S, unsigned long long* stats, hipStream_t st) {
     const size_t lds = cell_lds_bytes(B, V);
     static bool attrSet = false;
+    static int resident = 0;  // workgroups of this kernel the device holds at once
     if (!attrSet) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cell<B, V, FORCES, F>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
+        int dev = 0, cus = 0, perCu = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(&k_cell<B, V, FORCES, F>),
+                                                             B, lds);
+        if (e != hipSuccess) return e;
+        resident = cus * perCu;
         attrSet = true;
     }
-    hipLaunchKernelGGL((k_cell<B, V, FORCES, F>), dim3(ncells), dim3(B), lds, st, P, D, Dr, cells, order, S, stats);
+    int grid = ncells;
+    if (D.cellQueue) {
+        // the queue starts at the head of the launch order; a workgroup beyond the resident count would only
+        // find it drained
+        const hipError_t e = launch_fill_i32(D.cellQueue, 0, 1, st);
+        if (e != hipSuccess) return e;
+        if (resident > 0) grid = std::min(ncells, resident);
+    }
+    hipLaunchKernelGGL((k_cell<B, V, FORCES, F>), dim3(grid), dim3(B), lds, st, P, D, Dr, cells, order, ncells, S,
+                       stats);
     return hipGetLastError();
 }
 // The per-frame inputs of the persistent kernels: the root program and the moving-mesh poses at every substep.

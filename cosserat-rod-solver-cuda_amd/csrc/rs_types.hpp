@@ -104,6 +104,7 @@ struct Dev {
     const Pose* posesS;      // [S][nropes][numSdf] mesh poses while meshes move
     const int* cellMap;      // [cells * cell capacity] vertex of each cell slot (-1 past the cell's end)
     unsigned* cellTime;      // [cells] each cell's duration in its last frame (100 MHz ticks), or null
+    int* cellQueue;          // [1] next unclaimed position of the cell launch order (cleared before each frame)
 };
 
 // What drives the roots during one frame.

@@ -222,8 +222,8 @@ int rs_check_broadphase(rs_batch* b, const float* pos, const float* dx, int* cou
 int rs_set_kernel_timing(rs_batch* b, int on);
 int rs_kernel_time(rs_batch* b, double* total_ms, int* launches);
 /* Rope-rope broad-phase statistics of the persistent kernel (no counterpart in the reference): returns
- * and clears [chunk-box overlaps, exact segment-pair tests, list entries, segment-substeps] counted since the
- * last call, and turns counting on or off for the next frames. */
+ * and clears [skin-list entries written at list builds, exact segment-pair tests, candidate-list entries, list
+ * builds per cell] counted since the last call, and turns counting on or off for the next frames. */
 int rs_set_chunk_stats(rs_batch* b, int on, long long* out4);
 
 /* ---- Differentiation form. The reference has no adjoint, so these entry points have no counterpart in it.
