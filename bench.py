@@ -380,9 +380,9 @@ def differentiation_bench(n_ropes, frames):
     units = s.num_vertices * s.substeps * frames
     all_envs = np.arange(s.num_ropes, dtype=np.int32)
 
-    def forward_frames(mode):
+    def forward_frames(mode, variant=-1):
         b.reset_envs(all_envs)
-        b.set_dispatch(mode, -1)
+        b.set_dispatch(mode, variant)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for f in range(frames):
@@ -399,8 +399,12 @@ def differentiation_bench(n_ropes, frames):
     b.set_dispatch(1, -1)
     b.autotune(s.dt, s.substeps, True)
     variant = b.dispatch
-    forward_frames(1)
-    t_persist = forward_frames(1)
+    forward_frames(1, variant[1])  # the tuned variant, launched directly
+    t_persist = forward_frames(1, variant[1])
+    b.set_graph(True)  # the batching form: the tuned persistent kernel with the frame captured as a hipGraph
+    forward_frames(1, variant[1])
+    t_persist_graph = forward_frames(1, variant[1])
+    b.set_graph(False)
     b.set_dispatch(*variant)
 
     w = np.asarray(b.inv_masses(), np.float64).reshape(-1)
@@ -442,7 +446,9 @@ def differentiation_bench(n_ropes, frames):
             "plain_kernels_ms_per_frame": 1e3 * t_plain / frames,
             "plain_kernels_graph_ms_per_frame": 1e3 * t_plain_graph / frames,
             "persistent_kernel_ms_per_frame": 1e3 * t_persist / frames,
+            "persistent_kernel_graph_ms_per_frame": 1e3 * t_persist_graph / frames,
             "persistent_speedup_vs_plain": t_plain / t_persist,
+            "persistent_graph_speedup_vs_plain": t_plain / t_persist_graph,
             "persistent_dispatch": list(variant),
             "rollout_forward_ms_per_frame": 1e3 * fwd / frames,
             "rollout_forward_vertex_substeps_per_s": units / fwd,
@@ -568,6 +574,11 @@ def timed_window(scene, b, dev, W, K, kernel_timing=False):
     return marks[-1] - marks[0], ms, kern
 
 
+def progress(msg):
+    """One line on stderr as each leg of the run ends (the JSON line goes to stdout at the very end)."""
+    print(f"# {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
+
+
 def series(ms):
     return {"first_ms": ms[0], "last_ms": ms[-1], "min_ms": min(ms), "max_ms": max(ms),
             "ms": [round(v, 3) for v in ms]}
@@ -655,6 +666,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     wall, frame_ms, (kern_ms, launches) = timed_window(s, b, dev, W, K, kernel_timing=True)
+    progress(f"headline: {1e3 * wall / K:.2f} ms per frame over {K} frames")
 
     readout = None
     if RK > 0:  # the same batch continues with the contact-force readout on: float sums, then fixed-point sums
@@ -676,6 +688,7 @@ def main():
         for name in ("float", "fixed_point"):
             readout[name + "_overhead"] = readout[name + "_ms_per_frame"] / readout["off_ms_per_frame"] - 1.0
         readout["resting"] = resting_readout_bench(args.ropes)
+        progress("readout leg done")
     dispatch, captures = list(b.dispatch), b.graph_captures
     del b
 
@@ -733,23 +746,29 @@ def main():
                                             "operations); fast is fused multiply-add with the hardware reciprocal, "
                                             "square root and log"}
         del b2
+        progress(f"other arithmetic ({other}): {1e3 * wall2 / K:.2f} ms per frame")
     if readout is not None:
         line["readout"] = readout
     if not args.no_plain:
         line["plain"] = plain_legs()
+        progress("plain-method legs done")
     if not args.no_bake:
         line["bake_mesh"] = bake_bench()
+        progress("mesh bake leg done")
     if args.bvh_sizes:
         line["rope_bvh"] = rope_bvh_bench([int(v) for v in args.bvh_sizes.split(",")])
+        progress("rope hierarchy leg done")
     if args.diff_frames > 0:
         d = differentiation_bench(args.diff_ropes, args.diff_frames)
         line["differentiation"] = d
         line["adjoint_forward_ratio"] = d["adjoint_forward_ratio"]
+        progress(f"differentiation leg done: adjoint/forward {d['adjoint_forward_ratio']:.2f}")
     if args.scale_frames > 0:
         try:
             line["scale"] = scale_bench(args.scale_frames, args.scale_adjoint_frames)
         except Exception as e:  # the headline line must still print
             line["scale"] = {"error": f"{type(e).__name__}: {e}"}
+        progress("scale leg done")
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_ropes, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())))
     print(json.dumps(line), flush=True)
