@@ -6,13 +6,17 @@ per-vertex EA / EI / mass, linear drag and the root program. Branches are taken 
 values, so autograd differentiates the same frozen active set the adjoint uses.
 
 It restates the same method as rope_oracle.hpp (same reference lines) for the parts the adjoint
-covers. No rope-rope contact. Meshes are static within a frame.
+covers. Meshes are static within a frame. Rope-rope contact follows the oracle's detection: the candidate
+lists are formed from detached values each substep (a discrete decision), while the step limit (the
+distance of each listed pair and the detection padding) and the barrier and friction terms carry gradients.
 - root program: csrc/batch.cu:122-159
 - initItr: YB/sim/iteration.cu:34-69
 - cosseratItr without rope-rope terms: YB/sim/cosserat.cu:41-88,198-215
 - quaternionLambdaItr: YB/sim/cosserat.cu:227-273
 - endItr with linear and quadratic drag: YB/sim/iteration.cu:77-119
 - plane and SDF contact: YB/sim/contact.cuh:27-163
+- rope-rope detection and step limit: csrc/batch.cu:19-49, YB/sim/collision.cu:103-136
+- rope-rope barrier and friction: YB/sim/cosserat.cu:90-158
 - per-vertex mesh broad phase: YB/sim/contact.cuh:121-136
 (YB = third_party/yarnball/KittenEngine/YarnBall)
 
@@ -91,6 +95,25 @@ def _ssolve(Hp, f):
                         (c02 * fx + c12 * fy + c22 * fz) * inv], -1)
 
 
+def seg_closest(a0, a1, b0, b1):
+    """Closest-point parameters (u on a0-a1, v on b0-b1) by the reference's three steps (KittenEngine
+    segmentClosestPoints as restated in rope_oracle.hpp): the infinite-line v, clamped; u from it, clamped;
+    v again from u, clamped. A singular 2x2 system takes v = 0.5."""
+    A0, A1 = a1 - a0, b1 - b0
+    a, b, c = _dot(A0, A0), _dot(A0, A1), _dot(A1, A1)
+    diff = b0 - a0
+    num = b * _dot(A0, diff) - a * _dot(A1, diff)
+    den = a * c - b * b
+    ok = den != 0
+    y = num / torch.where(ok, den, torch.ones_like(den))
+    ok = ok & torch.isfinite(y)
+    y = torch.where(ok, y, torch.full_like(y, 0.5))
+    y = torch.clamp(y, 0.0, 1.0)
+    x = torch.clamp(_dot((b0 + y * A1) - a0, A0) / a, 0.0, 1.0)
+    y = torch.clamp(_dot((a0 + x * A0) - b0, A1) / c, 0.0, 1.0)
+    return x, y
+
+
 class TorchRope:
     """Rollouts of a configured float64 OracleBatch in torch float64 (autograd-capable).
 
@@ -99,7 +122,7 @@ class TorchRope:
     """
 
     def __init__(self, ob, gravity, planes=(), contact=(0.0, 0.5, 0.5, 1e-3, 2.0), mesh_specs=(), drag_quad=0.0,
-                 drag_quad_t=0.0, damping=0.0, accel=1.0, dtype=torch.float64):
+                 drag_quad_t=0.0, damping=0.0, accel=1.0, dtype=torch.float64, rope_contact=None):
         """dtype float64 is the reference; float32 measures what float32 arithmetic alone does to the same
         method (the construction state is still taken from the float64 oracle, rounded once)."""
         assert ob.prec == 64
@@ -132,6 +155,20 @@ class TorchRope:
         self.planes = [(t(p[:3]) / float(np.linalg.norm(p[:3])), float(p[3])) for p in planes]
         self.cR, self.cMuS, self.cMuK, self.cVSlip, self.cKappa = [float(c) for c in contact]
         self.dragQuad, self.dragQuadT, self.damping, self.accel = drag_quad, drag_quad_t, damping, accel
+        self.ropeContact = rope_contact is not None
+        self.stats = {}  # rope-rope activity of the last rollouts: active barrier terms, friction terms, clamped steps
+        if self.ropeContact:  # set_rope_contact keywords; uploadMeta (YB/YarnBall.cu:261-272)
+            rc = rope_contact
+            self.rcRadius, self.rcBarrier = float(rc["radius"]), float(rc["barrier"])
+            self.kCollision, self.fricMu = float(rc["k_collision"]), float(rc["mu"])
+            self.kFriction = float(rc["k_friction"])
+            self.detR = self.rcRadius + 0.5 * self.rcBarrier
+            self.scaledDetR = self.detR * float(rc.get("detection_scaler", 1.2))
+            G = int(rc.get("group_ropes", 1))
+            grp = np.arange(self.nropes) // G
+            self.vgroup = grp[np.repeat(np.arange(self.nropes), np.diff(self.off))]
+            self.ropeOfNp = np.repeat(np.arange(self.nropes), np.diff(self.off))
+            self.hasNextNp = (fl & HAS_NEXT) != 0
         self.meshes = []
         if mesh_specs:
             poses = ob.poses()  # [nropes, 64, 19]: pos, q, vel, omega, lo, hi
@@ -230,6 +267,98 @@ class TorchRope:
         hm = torch.where(over, torch.ones_like(hm), hm)
         return hm & self.free  # pinned vertices keep no candidates
 
+    # ------------------------------------------------------------------ rope-rope contact
+    def _detect(self, X, y):
+        """bruteDetectK (csrc/batch.cu:19-49) with the step limit (YB/sim/collision.cu:103-136). The lists are
+        formed from detached float64 values in the oracle's order (ascending j per segment i); the step limit
+        min(pad_i - detR, 0.45 |d_ij|), with pad = scaledDetR + |y|, keeps its gradient. Returns the listed
+        pairs (i, j) and each vertex's limit min(limit_i, limit_{i-1})."""
+        pad = self.scaledDetR + torch.sqrt(_dot(y, y))
+        Xd, pd = X.detach(), pad.detach()
+        hn = self.hasNextNp
+        I, J = [], []
+        for g in np.unique(self.vgroup):
+            vs = np.nonzero((self.vgroup == g) & hn)[0]
+            if len(vs) < 2:
+                continue
+            ii, jj = np.meshgrid(vs, vs, indexing="ij")
+            ii, jj = ii.ravel(), jj.ravel()
+            keep = ~((self.ropeOfNp[ii] == self.ropeOfNp[jj]) & (np.abs(jj - ii) <= 2))
+            ii, jj = ii[keep], jj[keep]
+            ti, tj = torch.as_tensor(ii), torch.as_tensor(jj)
+            a0 = Xd[ti]
+            a1, b0, b1 = Xd[ti + 1] - a0, Xd[tj] - a0, Xd[tj + 1] - a0
+            with torch.no_grad():
+                u, v = seg_closest(torch.zeros_like(a1), a1, b0, b1)
+                dv = u * a1 - (b0 + v * (b1 - b0))
+                r = torch.maximum(pd[ti], pd[ti + 1]) + torch.maximum(pd[tj], pd[tj + 1])
+                hit = (_dot(dv, dv) < r * r).squeeze(-1).numpy()
+            I.append(ii[hit])
+            J.append(jj[hit])
+        I = np.concatenate(I) if I else np.zeros(0, np.int64)
+        J = np.concatenate(J) if J else np.zeros(0, np.int64)
+        order = np.lexsort((J, I))
+        I, J = I[order], J[order]
+        if len(I) and np.bincount(I).max() > 128:
+            raise RuntimeError("MAX_COLLISIONS_PER_SEGMENT exceeded")
+        inf = torch.full((self.nv, 1), float("inf"), dtype=self.dtype)
+        ms = torch.where(self.hasNext, pad - self.detR, inf)
+        if len(I):
+            ti, tj = torch.as_tensor(I), torch.as_tensor(J)
+            a0 = X[ti]
+            a1, b0, b1 = X[ti + 1] - a0, X[tj] - a0, X[tj + 1] - a0
+            u, v = seg_closest(torch.zeros_like(a1), a1, b0, b1)
+            dv = u * a1 - (b0 + v * (b1 - b0))
+            dist = 0.45 * torch.sqrt(_dot(dv, dv))
+            ms = ms.scatter_reduce(0, ti.unsqueeze(-1), dist, reduce="amin", include_self=True)
+        lim = torch.where(self.hasPrev, torch.minimum(ms, ms[self.prv]), ms)
+        return (torch.as_tensor(I), torch.as_tensor(J)), lim
+
+    def _pair_terms(self, X, d, pairs, dmp):
+        """The barrier and friction terms of every listed pair (YB/sim/cosserat.cu:99-158): forces and packed
+        Hessians on vertex i (f, H) and on i + 1 (f2, H2)."""
+        I, J = pairs
+        p0, p1, d0, p1dx = X[I], X[I + 1], d[I], d[I + 1]
+        b0, b1, db0, db1 = X[J], X[J + 1], d[J], d[J + 1]
+        u, v = seg_closest(torch.zeros_like(p0), (p1 - p0) + (p1dx - d0), (b0 - p0) + (db0 - d0), (b1 - p0) + (db1 - d0))
+        dpos = (p0 + u * (p1 - p0)) - (b0 + v * (b1 - b0))
+        ddpos = (d0 + u * (p1dx - d0)) - (db0 + v * (db1 - db0))
+        normal = dpos + ddpos
+        n2 = _dot(normal, normal)
+        on = n2 > 0
+        dd = _safe_len(normal, on)
+        n = normal / dd
+        wy, wx = u, 1.0 - u
+        invb = 1.0 / self.rcBarrier
+        kCol = self.kCollision * invb
+        g = (dd - 2.0 * self.rcRadius) * invb
+        on = on & (g <= 1.0)
+        g = torch.clamp(g, min=1e-3)
+        invd = 1.0 / g
+        logd = torch.log(g)
+        dH = ((-3.0 + (2.0 + invd) * invd) - 2.0 * logd) * kCol * invb
+        ff = -(1.0 - g) * ((g - 1.0) + 2.0 * g * logd) * invd * kCol
+        f = (ff * wx - dmp * dH * wx * wx * _dot(n, d0)) * n
+        f2 = (ff * wy - dmp * dH * wy * wy * _dot(n, p1dx)) * n
+        dH = dH * (1.0 + dmp)
+        op = _outer_packed(n)
+        H = op * (dH * wx * wx)
+        H2 = op * (dH * wy * wy)
+        U = ddpos - _dot(n, ddpos) * n
+        fr = on & (_dot(U, U) > 0)
+        ul = _safe_len(U, fr)
+        f1 = torch.minimum(torch.full_like(ul, self.kFriction), self.fricMu * ff / ul)
+        eye = torch.tensor([1.0, 1.0, 1.0, 0.0, 0.0, 0.0], dtype=self.dtype)
+        z3, z6 = torch.zeros_like(f), torch.zeros_like(H)
+        f = f - torch.where(fr, f1 * wx * U, z3)
+        H = H - torch.where(fr, (op - eye) * (wx * wx * f1), z6)
+        f2 = f2 - torch.where(fr, f1 * wy * U, z3)
+        H2 = H2 - torch.where(fr, (op - eye) * (wy * wy * f1), z6)
+        st = self.stats
+        st["barrier"] = st.get("barrier", 0) + int(on.sum())
+        st["friction"] = st.get("friction", 0) + int(fr.sum())
+        return (torch.where(on, f, z3), torch.where(on, H, z6), torch.where(on, f2, z3), torch.where(on, H2, z6))
+
     # ------------------------------------------------------------------ the substep
     def substep(self, st, prm, s, S, dt, lastH, h, prog, cand):
         x, v, vl, q = st
@@ -258,6 +387,8 @@ class TorchRope:
         vl_new = v
         X = x
         dmp = self.damping / h
+        if self.ropeContact:
+            pairs, lim = self._detect(X, y)
         invw = torch.where(free, w, torch.ones_like(w))
         ex = torch.tensor([1.0, 0.0, 0.0], dtype=self.dtype)
         kap = self.cKappa / h
@@ -281,10 +412,20 @@ class TorchRope:
             diag = torch.cat([dd.expand(-1, 3), torch.zeros(self.nv, 3, dtype=self.dtype)], -1)
             H = torch.where(hn, H + diag, H)
             H2 = torch.where(hn, diag, torch.zeros_like(H))
+            if self.ropeContact and pairs[0].numel() > 0:
+                df, dH, df2, dH2 = self._pair_terms(X, d, pairs, dmp)
+                I = pairs[0]
+                f, H = f.index_add(0, I, df), H.index_add(0, I, dH)
+                f2, H2 = f2.index_add(0, I, df2), H2.index_add(0, I, dH2)
             F = torch.where(self.hasPrev, f + f2[self.prv], f)
             Ht = torch.where(self.hasPrev, H + H2[self.prv], H)
             Hs = torch.where(free, Ht, torch.tensor([1.0, 1.0, 1.0, 0.0, 0.0, 0.0], dtype=self.dtype).expand_as(Ht))
             dn = torch.where(free, d + self.accel * _ssolve(Hs, F), d)
+            if self.ropeContact:  # step clamp (YB/sim/cosserat.cu:206-215)
+                ln = torch.sqrt(_dot(dn, dn))
+                cl = free & (ln > lim) & (ln > 0)
+                self.stats["clamped"] = self.stats.get("clamped", 0) + int(cl.sum())
+                dn = torch.where(cl, dn * (lim / torch.where(cl, ln, torch.ones_like(ln))), dn)
             # orientation sweep (quaternionLambdaItr)
             upd = (~self.fix) & self.hasNext
             e = ((X[self.nxt] - X) + (dn[self.nxt] - dn)) / self.l[:, None]

@@ -33,6 +33,32 @@ def contact_scene(n_ropes=8, V=33, frames=6, substeps=20, seed=5, with_contact=T
     return s
 
 
+def rope_contact_scene(n_groups=2, V=25, frames=6, substeps=20, seed=9):
+    """Rope-rope contact in groups of two: in each environment rope B starts 1.2 cm above rope A and crosses
+    it at right angles, both leaning down from roots that follow the root program, so the ropes press on and
+    slide along each other within the first frame. No surfaces (the barrier, friction and step limit carry
+    the contact)."""
+    from ropesim import scenes
+    rng = np.random.default_rng(seed)
+    seg = 0.02
+    ropes = []
+    for g in range(n_groups):
+        c = np.array([0.5 * g, 0.0, 0.3])
+        for k in range(2):
+            ang = rng.uniform(-0.2, 0.2) + (0.0 if k == 0 else np.pi / 2)
+            d = np.array([np.cos(ang), np.sin(ang), -0.15 - 0.1 * k])
+            d /= np.linalg.norm(d)
+            L = seg * (V - 1)
+            root = c - 0.5 * L * np.array([np.cos(ang), np.sin(ang), 0.0]) + np.array([0.0, 0.0, 0.012 * k])
+            ropes.append(root + seg * np.arange(V)[:, None] * d)
+    pos = np.concatenate(ropes).astype(np.float32)
+    off = (np.arange(len(ropes) + 1) * V).astype(np.int32)
+    rc = dict(radius=0.004, barrier=0.001, k_collision=0.025, mu=0.4, k_friction=1e7, detection_scaler=1.2,
+              group_ropes=2)
+    return scenes.Scene("rope_contact", pos, off, 0.1, 500.0, 1e-4, 1.0 / 60.0 / substeps, 8, 1.0 / 60.0, substeps,
+                        frames, drag=0.2, rope_contact=rc, seed=seed)
+
+
 def program(s):
     fis = s.frame_inputs(s.frames)
     return [np.stack([f[k] for f in fis]).astype(np.float64) for k in ("p0", "v0", "p1")]
@@ -70,7 +96,7 @@ def torch_grads(oracle, s, mat, drag, prog, tips, tgt, dtype=None):
                    contact=[s.contact[k] for k in ("radius", "mu_s", "mu_k", "v_slip", "kappa")] if s.contact else
                    (0.0, 0.5, 0.5, 1e-3, 2.0),
                    mesh_specs=[dict(values=m.values, origin=m.origin, spacing=m.spacing, mu_s=m.mu_s, mu_k=m.mu_k)
-                               for m in s.meshes], dtype=dtype)
+                               for m in s.meshes], dtype=dtype, rope_contact=s.rope_contact)
     t = lambda a: torch.tensor(np.asarray(a, np.float64), dtype=dtype, requires_grad=True)
     EA, EI, M = (t(a) for a in mat)
     P0, V0, P1 = (t(a) for a in prog)
