@@ -156,6 +156,338 @@ __device__ __forceinline__ float surfaces_adj(const Params& P, const Grid* grids
     return kcb;
 }
 
+// ------------------------------------------------------------------ rope-rope contact adjoints
+//
+// Reverse of the closest-point parameters (u, v) = seg_closest(0, A, B0, B1) by its three steps (infinite-line
+// v, clamped; u from it, clamped; v again from u, clamped): the adjoints of A, B0, B1 given those of u and v. A
+// clamped step passes no gradient unless its value lies in [0, 1]; the singular branch (v = 0.5) passes none.
+__device__ void seg_closest_adj(f3 A, f3 B0, f3 B1, float ub, float vb, f3& Ab, f3& B0b, f3& B1b) {
+    const f3 A1 = B1 - B0;
+    const float a = dot(A, A), b = dot(A, A1), c = dot(A1, A1);
+    const float pa = dot(A, B0), pb = dot(A1, B0);
+    const float den = a * c - b * b;
+    float y0 = (b * pa - a * pb) / den;
+    const bool fin = isfinite(y0);
+    if (!fin) y0 = 0.5f;
+    const float y0c = clampf(y0, 0.f, 1.f);
+    const f3 m1 = B0 + y0c * A1;
+    const float x0 = dot(m1, A) / a;
+    const float xc = clampf(x0, 0.f, 1.f);
+    const f3 m2 = xc * A - B0;
+    const float y1 = dot(m2, A1) / c;
+    f3 A0b = mk3(0.f, 0.f, 0.f), A1b = A0b, Db = A0b;
+    float ab = 0.f, bb = 0.f, cb = 0.f;
+    const float y1b = (y1 >= 0.f && y1 <= 1.f) ? vb : 0.f;
+    const float n2b = y1b / c;
+    cb -= y1b * y1 / c;
+    const f3 m2b = n2b * A1;
+    A1b = A1b + n2b * m2;
+    const float xcb = ub + dot(m2b, A);
+    A0b = A0b + xc * m2b;
+    Db = Db - m2b;
+    const float x0b = (x0 >= 0.f && x0 <= 1.f) ? xcb : 0.f;
+    const float n1b = x0b / a;
+    ab -= x0b * x0 / a;
+    const f3 m1b = n1b * A;
+    A0b = A0b + n1b * m1;
+    Db = Db + m1b;
+    A1b = A1b + y0c * m1b;
+    const float y0b = (fin && y0 >= 0.f && y0 <= 1.f) ? dot(m1b, A1) : 0.f;
+    if (y0b != 0.f) {
+        const float numb = y0b / den, denb = -y0b * y0 / den;
+        ab += denb * c;
+        cb += denb * a;
+        bb -= 2.f * b * denb;
+        bb += numb * pa;
+        ab -= numb * pb;
+        const float pab = numb * b, pbb = -numb * a;
+        A0b = A0b + pab * B0;
+        Db = Db + pab * A;
+        A1b = A1b + pbb * B0;
+        Db = Db + pbb * A1;
+    }
+    A0b = A0b + (2.f * ab) * A + bb * A1;
+    A1b = A1b + bb * A + (2.f * cb) * A1;
+    Ab = A0b;
+    B1b = A1b;
+    B0b = Db - A1b;
+}
+
+// Adjoint of one rope-rope barrier + friction term (collision_eval, YB/sim/cosserat.cu:99-158) at the forward's
+// branches. Seeds: vertex i's Newton adjoint u1 and direction l1 (its F adjoint is u1, its H adjoint -u1 l1^T),
+// vertex i+1's u2, l2. Returns the adjoints of the term's inputs in g[0..7]: p0, p1, d0, p1dx (segment i, i+1)
+// and b0, b1, db0, db1 (segment j, j+1). The forward part repeats collision_eval's arithmetic, so every branch
+// (contact, the 1e-3 floor of the gap, friction, the friction cap) is the one the forward took.
+template <bool F>
+__device__ __noinline__ void pair_term_adj(const Params& P, float dmp, f3 p0, f3 p1, f3 d0, f3 p1dx, f3 b0, f3 b1,
+                                           f3 db0, f3 db1, f3 u1, f3 l1, f3 u2, f3 l2, f3* g) {
+    using Af = Ar<F>;
+    using M = V<F>;
+    for (int k = 0; k < 8; k++) g[k] = mk3(0.f, 0.f, 0.f);
+    const float invb = Af::rcp(P.barrier);
+    const float rad2 = 2.f * P.radius;
+    const float kCol = P.kCollision * invb;
+    const f3 A = (p1 - p0) + (p1dx - d0), B0 = (b0 - p0) + (db0 - d0), B1 = (b1 - p0) + (db1 - d0);
+    float u, v;
+    M::seg_closest(mk3(0.f, 0.f, 0.f), A, B0, B1, u, v);
+    const f3 dpos = M::mix(p0, p1, u) - M::mix(b0, b1, v);
+    const f3 ddpos = M::mix(d0, p1dx, u) - M::mix(db0, db1, v);
+    const f3 N = dpos + ddpos;
+    const float dd = M::len(N);
+    if (!(dd > 0.f)) return;
+    const f3 n = N * Af::rcp(dd);
+    const float wy = u, wx = 1.f - u;
+    const float gr = (dd - rad2) * invb;
+    if (gr > 1.f) return;
+    const float gc = fmaxf(gr, 1e-3f);
+    const float invd = Af::rcp(gc), logd = Af::log(gc);
+    const float dH0 = Af::fma(-2.f, logd, Af::fma(2.f + invd, invd, -3.f)) * kCol * invb;
+    const float Bf = Af::fma(2.f * gc, logd, gc - 1.f);
+    const float ff = -(1.f - gc) * Bf * invd * kCol;
+    const float nd0 = dot(n, d0), nd1 = dot(n, p1dx);
+    const float s1 = ff * wx - dmp * dH0 * wx * wx * nd0, s2 = ff * wy - dmp * dH0 * wy * wy * nd1;
+    const float dH = dH0 * (1.f + dmp);
+    const float h1 = dH * wx * wx, h2 = dH * wy * wy;
+    const f3 U = M::axpy(-M::dot(n, ddpos), n, ddpos);
+    const float ul = M::len(U);
+    const bool fric = ul > 0.f;
+    float f1 = 0.f, q = 0.f;
+    if (fric) {
+        q = Af::div(P.fricMu * ff, ul);
+        f1 = fminf(P.kFriction, q);
+    }
+    const float a1 = -(f1 * wx), a2 = -(f1 * wy), e1 = -(wx * wx * f1), e2 = -(wy * wy * f1);
+    // reverse
+    const float un1 = dot(u1, n), dn1 = dot(l1, n), un2 = dot(u2, n), dn2 = dot(l2, n);
+    const float s1b = un1, s2b = un2, h1b = -un1 * dn1, h2b = -un2 * dn2;
+    f3 nb = s1 * u1 + s2 * u2 - (h1 + e1) * (dn1 * u1 + un1 * l1) - (h2 + e2) * (dn2 * u2 + un2 * l2);
+    f3 Ub = a1 * u1 + a2 * u2;
+    float wxb = 0.f, wyb = 0.f, ffb = 0.f, dHb = 0.f;
+    if (fric) {
+        const float a1b = dot(u1, U), a2b = dot(u2, U);
+        const float e1b = -un1 * dn1 + dot(u1, l1), e2b = -un2 * dn2 + dot(u2, l2);
+        const float f1b = -(a1b * wx + e1b * wx * wx + a2b * wy + e2b * wy * wy);
+        wxb += -f1 * a1b - 2.f * wx * f1 * e1b;
+        wyb += -f1 * a2b - 2.f * wy * f1 * e2b;
+        if (q < P.kFriction) {
+            ffb += f1b * P.fricMu / ul;
+            const float ulb = -f1b * q / ul;
+            Ub = Ub + (ulb / ul) * U;
+        }
+    }
+    const float nddp = dot(n, ddpos), nUb = dot(n, Ub);
+    f3 ddposb = Ub - nUb * n;
+    nb = nb - (nddp * Ub + nUb * ddpos);
+    dHb += (h1b * wx * wx + h2b * wy * wy) * (1.f + dmp);
+    wxb += h1b * dH * 2.f * wx;
+    wyb += h2b * dH * 2.f * wy;
+    ffb += s1b * wx + s2b * wy;
+    wxb += s1b * (ff - 2.f * dmp * dH0 * wx * nd0);
+    wyb += s2b * (ff - 2.f * dmp * dH0 * wy * nd1);
+    dHb -= dmp * (s1b * wx * wx * nd0 + s2b * wy * wy * nd1);
+    nb = nb - (s1b * dmp * dH0 * wx * wx) * d0 - (s2b * dmp * dH0 * wy * wy) * p1dx;
+    f3 d0b = -(s1b * dmp * dH0 * wx * wx) * n, p1dxb = -(s2b * dmp * dH0 * wy * wy) * n;
+    float grb = 0.f;
+    if (gr >= 1e-3f) {
+        const float A_ = 1.f - gc;
+        const float dff = -kCol * ((A_ * (2.f * logd + 3.f) - Bf) * invd - A_ * Bf * invd * invd);
+        const float ddH = kCol * invb * (-2.f * invd) * (1.f + invd + invd * invd);
+        grb = ffb * dff + dHb * ddH;
+    }
+    const f3 Nb = (nb - dot(n, nb) * n) / dd + (grb * invb) * n;
+    ddposb = ddposb + Nb;
+    const f3 dposb = Nb;
+    float ub = wyb - wxb + dot(dposb, p1 - p0) + dot(ddposb, p1dx - d0);
+    const float vb = -dot(dposb, b1 - b0) - dot(ddposb, db1 - db0);
+    f3 p0b = (1.f - u) * dposb, p1b = u * dposb, b0b = -(1.f - v) * dposb, b1b = -v * dposb;
+    d0b = d0b + (1.f - u) * ddposb;
+    p1dxb = p1dxb + u * ddposb;
+    f3 db0b = -(1.f - v) * ddposb, db1b = -v * ddposb;
+    f3 Ab, B0b, B1b;
+    seg_closest_adj(A, B0, B1, ub, vb, Ab, B0b, B1b);
+    const f3 base = Ab + B0b + B1b;
+    g[0] = p0b - base;
+    g[1] = p1b + Ab;
+    g[2] = d0b - base;
+    g[3] = p1dxb + Ab;
+    g[4] = b0b + B0b;
+    g[5] = b1b + B1b;
+    g[6] = db0b + B0b;
+    g[7] = db1b + B1b;
+}
+
+// Adjoint of a pair's step-limit distance 0.45 |u a1 - mix(b0, b1, v)| with a1, b0, b1 relative to xa0 and
+// (u, v) = seg_closest(0, a1, b0, b1) (YB/sim/collision.cu:103-136, the detection's arithmetic), scaled by gb:
+// the adjoints of the four endpoint positions.
+__device__ __noinline__ void limit_dist_adj(f3 xa0, f3 xa1, f3 xb0, f3 xb1, float gb, f3* g) {
+    const f3 a1 = xa1 - xa0, b0 = xb0 - xa0, b1 = xb1 - xa0;
+    float u, v;
+    seg_closest(mk3(0.f, 0.f, 0.f), a1, b0, b1, u, v);
+    const f3 d = u * a1 - mix(b0, b1, v);
+    const float l = len(d);
+    for (int k = 0; k < 4; k++) g[k] = mk3(0.f, 0.f, 0.f);
+    if (!(l > 0.f)) return;
+    const f3 Db = (0.45f * gb / l) * d;
+    f3 a1b = u * Db, b0b = -(1.f - v) * Db, b1b = -v * Db;
+    const float ub = dot(Db, a1), vb = -dot(Db, b1 - b0);
+    f3 Ab, B0b, B1b;
+    seg_closest_adj(a1, b0, b1, ub, vb, Ab, B0b, B1b);
+    a1b = a1b + Ab;
+    b0b = b0b + B0b;
+    b1b = b1b + B1b;
+    g[0] = -(a1b + b0b + b1b);
+    g[1] = a1b;
+    g[2] = b0b;
+    g[3] = b1b;
+}
+
+// Rope-rope detection of a workgroup's contact groups at one substep, from the positions and paddings in LDS:
+// the reference's per-segment test in ascending order (k_detect, csrc/batch.cu:19-49) with its step limit
+// (YB/sim/collision.cu:103-136), reproduced operation for operation so the lists and limits are the forward's.
+// Candidates are culled first by chunk boxes (16 slots) and then by segment boxes, each padded by the
+// segment's detection radius and a rounding margin, which keeps every pair the exact test can accept.
+// Lists hold local slots, ascending; with REV the segment also lists, ascending, the segments whose own lists
+// hold it. Returns the count (fwd), the limit and the slot whose distance set it (-1: the padding).
+struct RcLds {
+    float *x, *pad, *lim, *sb, *cb;  // positions [3][B], paddings [B], limits [B], segment boxes [6][B], chunk boxes [6][B/16]
+    float *dk, *mb;                  // (reverse pass) iterate d^k [3][B], step-limit adjoints [B]
+    int *fl, *rope, *arg, *nrev;     // flags [B], rope [B], limit argument [B], partner-list lengths [B]
+};
+
+// The detection arrays in dynamic LDS at `base` (after the mesh pose slots); x: positions already in LDS, or
+// null for a region of their own.
+template <int B>
+__device__ __forceinline__ RcLds rc_lds(char* base, float* x) {
+    RcLds L;
+    float* f = reinterpret_cast<float*>(base);
+    L.pad = f;
+    f += B;
+    L.lim = f;
+    f += B;
+    L.sb = f;
+    f += 6 * B;
+    L.cb = f;
+    f += 6 * (B / 16);
+    L.dk = f;
+    f += 3 * B;
+    L.mb = f;
+    f += B;
+    L.x = x;
+    if (!x) {
+        L.x = f;
+        f += 3 * B;
+    }
+    int* q = reinterpret_cast<int*>(f);
+    L.fl = q;
+    L.rope = q + B;
+    L.arg = q + 2 * B;
+    L.nrev = q + 3 * B;
+    return L;
+}
+
+__device__ __forceinline__ void seg_box(const RcLds& L, int B, int t, uint32_t fl, f3& lo, f3& hi) {
+    if (!(fl & HAS_NEXT)) {
+        lo = mk3(INFINITY, INFINITY, INFINITY);
+        hi = mk3(-INFINITY, -INFINITY, -INFINITY);
+        return;
+    }
+    const f3 a = ld3s(L.x, B, t), b = ld3s(L.x, B, t + 1);
+    const float r = fmaxf(L.pad[t], L.pad[t + 1]);
+    const float big = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(b.x))),
+                            fmaxf(fabsf(b.y), fabsf(b.z)));
+    const float m = r + 1e-6f + 1e-5f * big;
+    lo = mk3(fminf(a.x, b.x) - m, fminf(a.y, b.y) - m, fminf(a.z, b.z) - m);
+    hi = mk3(fmaxf(a.x, b.x) + m, fmaxf(a.y, b.y) + m, fmaxf(a.z, b.z) + m);
+}
+
+__device__ __forceinline__ bool box_meets(const float* bx, int stride, int k, f3 lo, f3 hi) {
+    return !(bx[k] > hi.x || bx[stride + k] > hi.y || bx[2 * stride + k] > hi.z || bx[3 * stride + k] < lo.x ||
+             bx[4 * stride + k] < lo.y || bx[5 * stride + k] < lo.z);
+}
+
+// Stage 1 (all lanes, then a barrier by the caller): segment boxes and chunk boxes.
+template <int B>
+__device__ __forceinline__ void rc_boxes(const RcLds& L, int t, bool act, uint32_t fl) {
+    f3 lo, hi;
+    seg_box(L, B, t, act ? fl : 0u, lo, hi);
+    L.sb[t] = lo.x;
+    L.sb[B + t] = lo.y;
+    L.sb[2 * B + t] = lo.z;
+    L.sb[3 * B + t] = hi.x;
+    L.sb[4 * B + t] = hi.y;
+    L.sb[5 * B + t] = hi.z;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        lo.x = fminf(lo.x, __shfl_xor(lo.x, o));
+        lo.y = fminf(lo.y, __shfl_xor(lo.y, o));
+        lo.z = fminf(lo.z, __shfl_xor(lo.z, o));
+        hi.x = fmaxf(hi.x, __shfl_xor(hi.x, o));
+        hi.y = fmaxf(hi.y, __shfl_xor(hi.y, o));
+        hi.z = fmaxf(hi.z, __shfl_xor(hi.z, o));
+    }
+    if ((t & 15) == 0) {
+        constexpr int NCH = B / 16;
+        const int c = t >> 4;
+        L.cb[c] = lo.x;
+        L.cb[NCH + c] = lo.y;
+        L.cb[2 * NCH + c] = lo.z;
+        L.cb[3 * NCH + c] = hi.x;
+        L.cb[4 * NCH + c] = hi.y;
+        L.cb[5 * NCH + c] = hi.z;
+    }
+}
+
+// Stage 2: the lists of segment t over its group's local slots [g0, g1). err: set on a list overflow.
+template <int B, bool REV>
+__device__ __forceinline__ int rc_lists(const Params& P, const RcLds& L, int t, uint32_t fl, int g0, int g1,
+                                        uint16_t* fwd, uint16_t* rev, int& nrev, float& lim, int& arg, int* err) {
+    constexpr int NCH = B / 16;
+    int cnt = 0, rc = 0;
+    lim = INFINITY;
+    arg = -1;
+    nrev = 0;
+    if (!(fl & HAS_NEXT)) return 0;
+    lim = L.pad[t] - P.detR;
+    f3 lo = mk3(L.sb[t], L.sb[B + t], L.sb[2 * B + t]), hi = mk3(L.sb[3 * B + t], L.sb[4 * B + t], L.sb[5 * B + t]);
+    const f3 a0 = ld3s(L.x, B, t);
+    const f3 a1 = ld3s(L.x, B, t + 1) - a0;
+    const float ri = fmaxf(L.pad[t], L.pad[t + 1]);
+    const int myRope = L.rope[t];
+    for (int ch = g0 >> 4; ch <= (g1 - 1) >> 4; ch++) {
+        if (!box_meets(L.cb, NCH, ch, lo, hi)) continue;
+        const int j1 = min(ch * 16 + 16, g1);
+        for (int j = max(ch * 16, g0); j < j1; j++) {
+            if (!(L.fl[j] & HAS_NEXT)) continue;
+            if (L.rope[j] == myRope && abs(j - t) <= 2) continue;
+            if (!box_meets(L.sb, B, j, lo, hi)) continue;
+            const float rj = fmaxf(L.pad[j], L.pad[j + 1]);
+            float d2;
+            if (pair_hit(a1, ld3s(L.x, B, j) - a0, ld3s(L.x, B, j + 1) - a0, ri + rj, d2)) {
+                if (cnt < MAX_COLL) {
+                    fwd[cnt] = (uint16_t)j;
+                    const float dj = 0.45f * sqrtf(d2);
+                    if (dj < lim) {
+                        lim = dj;
+                        arg = j;
+                    }
+                }
+                cnt++;
+            }
+            if (REV) {
+                const f3 c0 = ld3s(L.x, B, j);
+                float e2;
+                if (pair_hit(ld3s(L.x, B, j + 1) - c0, a0 - c0, ld3s(L.x, B, t + 1) - c0, rj + ri, e2)) {
+                    if (rc < MAX_COLL) rev[rc] = (uint16_t)j;
+                    rc++;
+                }
+            }
+        }
+    }
+    if (cnt > MAX_COLL || rc > MAX_COLL) *err = ERR_MAX_COLLISIONS;
+    nrev = min(rc, MAX_COLL);
+    return min(cnt, MAX_COLL);
+}
+
 // History of one frame of the ropes of a launch. Local vertex li = i - A.hv0 of A.hn.
 __device__ __forceinline__ size_t hs(const AdjDev& A, int s, int li) { return (size_t)s * A.hn + li; }
 __device__ __forceinline__ size_t hk(const AdjDev& A, int I, int s, int k, int li) {
@@ -189,11 +521,12 @@ __device__ __forceinline__ Vc load_vc(const Dev& D, int i, bool act) {
 // the frame kernel (so the values, and every branch, are those of the recorded forward). Every
 // substep's inputs and every iteration's d^k, q^k, total H and Newton direction δ = H^-1 F go to the
 // history. The frame-end state is compared bit for bit with the next checkpoint.
-template <int B, bool F>
+template <int B, bool F, bool RC>
 __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDev A, Drive Dr, int S) {
     __shared__ float4 sQ[2][B];
     __shared__ float sX[3 * B], sD[2][3 * B], sF2[9 * B];
-    extern __shared__ Pose sPoseAll[];  // the poses of the workgroup's ropes, sized at launch
+    extern __shared__ Pose sPoseAll[];  // the poses of the workgroup's ropes, sized at launch; then, with rope
+                                        // contact, the detection arrays (rc_lds_bytes)
     const int I = P.numItr;
     const int rb0 = A.blk[blockIdx.x], rb1 = A.blk[blockIdx.x + 1], t = threadIdx.x;
     const int v0 = D.roff[rb0], n = D.roff[rb1] - v0;
@@ -207,6 +540,19 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
     const Vc c = load_vc(D, i, act);
     if (P.numSdf > 0 && !P.sdfMoving)
         for (int m = t; m < nPose; m += B) sPoseAll[m] = A.fposes[(size_t)rb0 * P.numSdf + m];
+    RcLds L{};
+    int g0 = 0, g1 = 0, ncol = 0;
+    float lim = INFINITY;
+    uint16_t* row = nullptr;
+    if (RC) {
+        L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), sX);
+        L.fl[t] = act ? (int)c.fl : 0;
+        L.rope[t] = r;
+        const int G = P.groupRopes, gg = r / G;
+        g0 = D.roff[min(gg * G, P.nropes)] - v0;
+        g1 = D.roff[min((gg + 1) * G, P.nropes)] - v0;
+        row = A.lists + (size_t)(act ? li : 0) * 2 * MAX_COLL;
+    }
 
     f3 x = mk3(0.f, 0.f, 0.f), v = x, vl = x;
     q4 q{0.f, 0.f, 0.f, 1.f};
@@ -253,8 +599,20 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
             st3s(sX, B, t, x);
             st3s(sD[0], B, t, d);
             sQ[0][t] = stq(q);
+            if (RC) L.pad[t] = padding;
         }
         __syncthreads();
+        if (RC) {  // rope-rope lists and step limits of the substep (the forward's detection)
+            rc_boxes<B>(L, t, act, c.fl);
+            __syncthreads();
+            float ms = INFINITY;
+            int arg, nrev;
+            if (act) ncol = rc_lists<B, false>(P, L, t, c.fl, g0, g1, row, nullptr, nrev, ms, arg, D.err);
+            L.lim[t] = ms;
+            __syncthreads();
+            lim = ms;
+            if (act && (c.fl & HAS_PREV)) lim = fminf(lim, L.lim[t - 1]);
+        }
         uint64_t hitBits = 0;
         for (int it = 0; it < I; it++) {
             const float* dk = sD[it & 1];
@@ -272,6 +630,14 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
                 vertex_terms<F, true>(P, D.grids, sPose, nc, cand, c.w, c.l, c.kS, c.fl, p0, d0, y,
                                       ldq(sQ[it & 1][t]), p1, p1dx, f, H, f2, H2, &hit);
                 if (it < 8) hitBits |= (uint64_t)(hit & 0xFFu) << (8 * it);
+                if (RC && ncol > 0) {  // rope-rope terms in list order, as the forward adds them
+                    const float dmp = Ar<F>::div(P.damping, h);
+                    for (int e = 0; e < ncol; e++) {
+                        const int j = row[e];
+                        collision_term<F>(P, dmp, p0, p1, d0, p1dx, ld3s(sX, B, j), ld3s(sX, B, j + 1), ld3s(dk, B, j),
+                                          ld3s(dk, B, j + 1), f, H, f2, H2);
+                    }
+                }
                 st3s(sF2, B, t, f2);
 #pragma unroll
                 for (int m = 0; m < 6; m++) sF2[(3 + m) * B + t] = H2.m[m];
@@ -288,7 +654,7 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
                 f3 d = ld3s(dk, B, t), dl = mk3(0.f, 0.f, 0.f);
                 if (c.w != 0.f) {
                     dl = V<F>::ssolve(H, f);
-                    d = solve_update<F>(P, d, H, f, false, INFINITY);
+                    d = solve_update<F>(P, d, H, f, RC, lim);
                 }
                 st3s(dn, B, t, d);
                 A.hD[hk(A, I, s, it + 1, li)] = st3(d);
@@ -361,10 +727,11 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
 // Pass 2: backwards through the frame of one rope from the history (no recompute of the iterations).
 // Per iteration three phases: orientation adjoint (pushes to the neighbours), Newton adjoint u = a H^-1 d̄,
 // segment-term adjoint (pushes to the next vertex).
-template <int B, bool F>
+template <int B, bool F, bool RC>
 __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A, Drive Dr, int S) {
     __shared__ float ex[EX_N * B];
-    extern __shared__ Pose sPoseAll[];  // the poses of the workgroup's ropes, sized at launch
+    extern __shared__ Pose sPoseAll[];  // the poses of the workgroup's ropes, sized at launch; then, with rope
+                                        // contact, the detection arrays (rc_lds_bytes)
     const int I = P.numItr;
     const int rb0 = A.blk[blockIdx.x], rb1 = A.blk[blockIdx.x + 1], t = threadIdx.x;
     const int v0 = D.roff[rb0], n = D.roff[rb1] - v0;
@@ -400,6 +767,21 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
     const bool upd = !(c.fl & FIX_ORIENT) && (c.fl & HAS_NEXT);
     const bool hn = (c.fl & HAS_NEXT) != 0, hp = (c.fl & HAS_PREV) != 0;
     const q4 EX{1.f, 0.f, 0.f, 0.f};
+    RcLds L{};
+    int g0 = 0, g1 = 0, ncol = 0;
+    uint16_t *row = nullptr, *rrow = nullptr;
+    if (RC) {
+        L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), nullptr);
+        L.fl[t] = act ? (int)c.fl : 0;
+        L.rope[t] = r;
+        const int G = P.groupRopes, gg = r / G;
+        g0 = D.roff[min(gg * G, P.nropes)] - v0;
+        g1 = D.roff[min((gg + 1) * G, P.nropes)] - v0;
+        row = A.lists + (size_t)(act ? li : 0) * 2 * MAX_COLL;
+        rrow = row + MAX_COLL;
+    }
+    // row of list entry e of slot k: the workgroup's vertices are consecutive local vertices
+    auto rev_row = [&](int k) { return A.lists + (size_t)(li - t + k) * 2 * MAX_COLL + MAX_COLL; };
 
     for (int s = S - 1; s >= 0; s--) {
         __syncthreads();  // sPose and the exchange are free
@@ -423,7 +805,13 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             hitMask = (uint64_t)mm.x | ((uint64_t)mm.y << 32);
         }
         // end update: x' = X + d, v' = (d / h)(1 - drag h) on free vertices
-        f3 aD = ax, aX = ax, aY = mk3(0.f, 0.f, 0.f);
+        f3 aD = ax, aX = ax, aY = mk3(0.f, 0.f, 0.f), aYp = aY;  // aYp: y through the detection padding
+        float lim = INFINITY, mOwn = 0.f, mPrev = 0.f;
+        bool limPrev = false;
+        if (RC && act) {
+            st3s(L.x, B, t, X0);
+            L.pad[t] = P.scaledDetR + M::len(y);  // initItr's padding of this substep
+        }
         q4 aQ = aq;
         if (act && c.w != 0.f) {
             const f3 d = ld3(A.hD[hk(A, I, s, I, li)]);
@@ -432,6 +820,22 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             gdrag -= (double)h * (double)dot(d * invH, av);
         }
         __syncthreads();  // sPose of a moving frame is in place
+        if (RC) {  // the substep's rope-rope lists and step limits, with the lists of the segments listing each one
+            rc_boxes<B>(L, t, act, c.fl);
+            __syncthreads();
+            float ms = INFINITY;
+            int arg = -1, nrev = 0;
+            if (act) ncol = rc_lists<B, true>(P, L, t, c.fl, g0, g1, row, rrow, nrev, ms, arg, D.err);
+            L.lim[t] = ms;
+            L.arg[t] = arg;
+            L.nrev[t] = nrev;
+            __syncthreads();
+            lim = ms;
+            if (act && hp && L.lim[t - 1] < ms) {
+                lim = L.lim[t - 1];
+                limPrev = true;
+            }
+        }
 
         // History of iteration k: d^{k+1} of vertex t and t+1 (the previous iteration's d^k, already loaded),
         // d^k, the frames q^k of t-1..t+1, the Hessian and the Newton direction. Iteration k-1's loads are
@@ -559,9 +963,23 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                     grho[2] += gr.z;
                     grho[3] += gr.w;
                 }
+                if (RC && c.w != 0.f) {  // the step clamp d' = z lim / |z| of the forward, z = d^k + a H^-1 F
+                    const f3 z = M::axpy(P.accel, dl, d0);
+                    const float lz = M::len(z);
+                    if (lz > lim && lz > 0.f) {
+                        const f3 zh = z * (1.f / lz);
+                        const float limb = dot(zh, aD);
+                        aD = (lim / lz) * (aD - limb * zh);
+                        if (limPrev)
+                            mPrev += limb;
+                        else
+                            mOwn += limb;
+                    }
+                }
                 if (c.w != 0.f) u = P.accel * M::ssolve(H, aD);
                 st3s(ex + EX_U * B, B, t, u);
                 st3s(ex + (EX_U + 3) * B, B, t, dl);
+                if (RC) st3s(L.dk, B, t, d0);
             }
             __syncthreads();
             // -- adjoint of the segment terms of vertex t (inertia, surfaces, stretch of segment (t, t+1))
@@ -601,6 +1019,37 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                     aQ = qsub(aQ, rot_ex_adj(q0, cb));
                     gk += ((double)kkb + (double)ddb * (double)invl) * (double)invl;
                 }
+                if (RC) {  // rope-rope terms: this segment's own list, then the segments that list (t) or (t - 1)
+                    f3 gq[8];
+                    const f3 zero = mk3(0.f, 0.f, 0.f);
+                    const f3 u2 = hn ? ld3s(ex + EX_U * B, B, t + 1) : zero;
+                    const f3 dl2 = hn ? ld3s(ex + (EX_U + 3) * B, B, t + 1) : zero;
+                    for (int e = 0; e < ncol; e++) {
+                        const int j = row[e];
+                        pair_term_adj<F>(P, dmp, X0, X1, d0, d1, ld3s(L.x, B, j), ld3s(L.x, B, j + 1), ld3s(L.dk, B, j),
+                                         ld3s(L.dk, B, j + 1), u, dl, u2, dl2, gq);
+                        aX = aX + gq[0];
+                        aD = aD + gq[2];
+                        pX = pX + gq[1];
+                        pD = pD + gq[3];
+                    }
+                    for (int side = 0; side < 2; side++) {
+                        const int jt = t - side;  // the partner segment (jt, jt + 1) holds this vertex at `side`
+                        if (side == 1 && !hp) break;
+                        const int nr = L.nrev[jt];
+                        const uint16_t* rr = rev_row(jt);
+                        for (int e = 0; e < nr; e++) {
+                            const int k = rr[e];
+                            pair_term_adj<F>(P, dmp, ld3s(L.x, B, k), ld3s(L.x, B, k + 1), ld3s(L.dk, B, k),
+                                             ld3s(L.dk, B, k + 1), ld3s(L.x, B, jt), ld3s(L.x, B, jt + 1),
+                                             ld3s(L.dk, B, jt), ld3s(L.dk, B, jt + 1), ld3s(ex + EX_U * B, B, k),
+                                             ld3s(ex + (EX_U + 3) * B, B, k), ld3s(ex + EX_U * B, B, k + 1),
+                                             ld3s(ex + (EX_U + 3) * B, B, k + 1), gq);
+                            aX = aX + gq[4 + side];
+                            aD = aD + gq[6 + side];
+                        }
+                    }
+                }
                 st3s(ex + EX_P * B, B, t, pD);
                 st3s(ex + (EX_P + 3) * B, B, t, pX);
             }
@@ -611,10 +1060,53 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             aD = aD + ld3s(ex + EX_P * B, B, t - 1);
             aX = aX + ld3s(ex + (EX_P + 3) * B, B, t - 1);
         }
+        if (RC) {  // step-limit adjoints: limit_t = min(pad_t - detR, 0.45 |d_tj| over the listed j)
+            L.mb[t] = mPrev;  // adjoint of the previous segment's limit
+            __syncthreads();
+            const float Mt = mOwn + ((act && hn) ? L.mb[t + 1] : 0.f);
+            __syncthreads();
+            L.mb[t] = Mt;
+            __syncthreads();
+            if (act) {
+                f3 g4[4];
+                if (Mt != 0.f) {
+                    const int a = L.arg[t];
+                    if (a < 0) {
+                        aYp = aYp + (Mt / len(y)) * y;
+                    } else {
+                        limit_dist_adj(X0, X1, ld3s(L.x, B, a), ld3s(L.x, B, a + 1), Mt, g4);
+                        aX = aX + g4[0];
+                    }
+                }
+                if (hp) {
+                    const float Mp = L.mb[t - 1];
+                    const int a = L.arg[t - 1];
+                    if (Mp != 0.f && a >= 0) {
+                        limit_dist_adj(ld3s(L.x, B, t - 1), X0, ld3s(L.x, B, a), ld3s(L.x, B, a + 1), Mp, g4);
+                        aX = aX + g4[1];
+                    }
+                }
+                for (int side = 0; side < 2; side++) {
+                    const int jt = t - side;
+                    if (side == 1 && !hp) break;
+                    const int nr = L.nrev[jt];
+                    const uint16_t* rr = rev_row(jt);
+                    for (int e = 0; e < nr; e++) {
+                        const int k = rr[e];
+                        const float Mk = L.mb[k];
+                        if (Mk == 0.f || L.arg[k] != jt) continue;
+                        limit_dist_adj(ld3s(L.x, B, k), ld3s(L.x, B, k + 1), ld3s(L.x, B, jt), ld3s(L.x, B, jt + 1), Mk,
+                                       g4);
+                        aX = aX + g4[2 + side];
+                    }
+                }
+            }
+        }
 
         // init: d^0 = h v (+ h^2 s g), y = h v + h^2 g, last velocity <- v
         if (act) {
             f3 vb = h * aD + avl, vlb = mk3(0.f, 0.f, 0.f);
+            if (RC) vb = vb + h * aYp;
             if (c.w != 0.f) {
                 vb = vb + h * aY;
                 const f3 g = mk3(P.gx, P.gy, P.gz);
@@ -714,13 +1206,18 @@ __global__ void k_adjoint_finish(Params P, Dev D, const float4* rhoUnit, const d
 
 }  // namespace
 
+size_t adjoint_rc_lds_bytes(int block) {
+    return sizeof(float) * ((size_t)15 * block + 6 * (block / 16)) + sizeof(int) * 4 * (size_t)block;
+}
+
 size_t adjoint_history_bytes(int numItr) {
     return 5 * sizeof(float4) + 2 * sizeof(uint2) + (size_t)(numItr + 1) * 2 * sizeof(float4) +
            (size_t)numItr * 9 * sizeof(float);
 }
 
 bool adjoint_block_supported(int block, bool reverse) {
-    return block == 64 || block == 128 || block == 192 || block == 256 || (reverse && block == 512);
+    (void)reverse;
+    return block == 64 || block == 128 || block == 192 || block == 256 || block == 384 || block == 512;
 }
 
 namespace {
@@ -731,65 +1228,74 @@ int waves_of(const void* fn) {
     const int regs = (a.numRegs + 7) / 8 * 8;
     return std::max(1, std::min(8, 512 / regs));
 }
-template <int B>
+template <int B, bool RC>
 const void* pass_fn(bool reverse, bool fast) {
     if (reverse)
-        return fast ? reinterpret_cast<const void*>(&k_adjoint_reverse<B, true>)
-                    : reinterpret_cast<const void*>(&k_adjoint_reverse<B, false>);
-    return fast ? reinterpret_cast<const void*>(&k_adjoint_forward<B, true>)
-                : reinterpret_cast<const void*>(&k_adjoint_forward<B, false>);
+        return fast ? reinterpret_cast<const void*>(&k_adjoint_reverse<B, true, RC>)
+                    : reinterpret_cast<const void*>(&k_adjoint_reverse<B, false, RC>);
+    return fast ? reinterpret_cast<const void*>(&k_adjoint_forward<B, true, RC>)
+                : reinterpret_cast<const void*>(&k_adjoint_forward<B, false, RC>);
+}
+template <int B>
+const void* pass_fn(bool reverse, bool fast, bool rc) {
+    return rc ? pass_fn<B, true>(reverse, fast) : pass_fn<B, false>(reverse, fast);
 }
 }  // namespace
 
-int adjoint_waves_per_simd(int block, bool reverse, bool fast) {
+int adjoint_waves_per_simd(int block, bool reverse, bool fast, bool ropeContact) {
     switch (block) {
-        case 64: return waves_of(pass_fn<64>(reverse, fast));
-        case 128: return waves_of(pass_fn<128>(reverse, fast));
-        case 192: return waves_of(pass_fn<192>(reverse, fast));
-        case 256: return waves_of(pass_fn<256>(reverse, fast));
-        case 512:
-            if (!reverse) return 1;
-            return waves_of(fast ? reinterpret_cast<const void*>(&k_adjoint_reverse<512, true>)
-                                 : reinterpret_cast<const void*>(&k_adjoint_reverse<512, false>));
+        case 64: return waves_of(pass_fn<64>(reverse, fast, ropeContact));
+        case 128: return waves_of(pass_fn<128>(reverse, fast, ropeContact));
+        case 192: return waves_of(pass_fn<192>(reverse, fast, ropeContact));
+        case 256: return waves_of(pass_fn<256>(reverse, fast, ropeContact));
+        case 384: return waves_of(pass_fn<384>(reverse, fast, ropeContact));
+        case 512: return waves_of(pass_fn<512>(reverse, fast, ropeContact));
         default: return 1;
     }
 }
 
-template <int B, bool F>
+template <int B, bool F, bool RC>
 static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
-                                int nblocks, size_t poseBytes, hipStream_t s) {
+                                int nblocks, size_t lds, hipStream_t s) {
+    // dynamic LDS past the default limit needs the kernel's attribute raised once
+    static bool attr[2] = {false, false};
+    const void* fn = pass_fn<B, RC>(reverse, F);
+    if (!attr[reverse]) {
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        if (e != hipSuccess) return e;
+        attr[reverse] = true;
+    }
     if (reverse)
-        hipLaunchKernelGGL((k_adjoint_reverse<B, F>), dim3(nblocks), dim3(B), poseBytes, s, P, D, A, Dr, S);
+        hipLaunchKernelGGL((k_adjoint_reverse<B, F, RC>), dim3(nblocks), dim3(B), lds, s, P, D, A, Dr, S);
     else
-        hipLaunchKernelGGL((k_adjoint_forward<B, F>), dim3(nblocks), dim3(B), poseBytes, s, P, D, A, Dr, S);
+        hipLaunchKernelGGL((k_adjoint_forward<B, F, RC>), dim3(nblocks), dim3(B), lds, s, P, D, A, Dr, S);
     return hipGetLastError();
 }
 template <int B>
 static hipError_t launch_pass_t(bool reverse, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
-                                int nblocks, size_t poseBytes, hipStream_t s) {
-    if (P.fastMath) return launch_pass_f<B, true>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
-    return launch_pass_f<B, false>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
+                                int nblocks, size_t lds, hipStream_t s) {
+    const bool rc = P.ropeContact != 0;
+    if (P.fastMath)
+        return rc ? launch_pass_f<B, true, true>(reverse, P, D, A, Dr, S, nblocks, lds, s)
+                  : launch_pass_f<B, true, false>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+    return rc ? launch_pass_f<B, false, true>(reverse, P, D, A, Dr, S, nblocks, lds, s)
+              : launch_pass_f<B, false, false>(reverse, P, D, A, Dr, S, nblocks, lds, s);
 }
 
 hipError_t launch_adjoint_pass(bool reverse, int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr,
                                int S, int nblocks, int ropesPerBlock, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (!adjoint_block_supported(block, reverse) || ropesPerBlock < 1 ||
-        (size_t)ropesPerBlock * P.numSdf > (size_t)adjoint_pose_slots(block))
+        (size_t)ropesPerBlock * P.numSdf > (size_t)adjoint_pose_slots(block) || (P.ropeContact && !A.lists))
         return hipErrorInvalidValue;
-    const size_t poseBytes = (size_t)ropesPerBlock * P.numSdf * sizeof(Pose);
+    const size_t lds = (size_t)ropesPerBlock * P.numSdf * sizeof(Pose) + (P.ropeContact ? adjoint_rc_lds_bytes(block) : 0);
     switch (block) {
-        case 64: return launch_pass_t<64>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
-        case 128: return launch_pass_t<128>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
-        case 192: return launch_pass_t<192>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
-        case 256: return launch_pass_t<256>(reverse, P, D, A, Dr, S, nblocks, poseBytes, s);
-        case 512:
-            if (P.fastMath)
-                hipLaunchKernelGGL((k_adjoint_reverse<512, true>), dim3(nblocks), dim3(512), poseBytes, s, P, D, A, Dr, S);
-            else
-                hipLaunchKernelGGL((k_adjoint_reverse<512, false>), dim3(nblocks), dim3(512), poseBytes, s, P, D, A, Dr,
-                                   S);
-            return hipGetLastError();
+        case 64: return launch_pass_t<64>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+        case 128: return launch_pass_t<128>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+        case 192: return launch_pass_t<192>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+        case 256: return launch_pass_t<256>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+        case 384: return launch_pass_t<384>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+        case 512: return launch_pass_t<512>(reverse, P, D, A, Dr, S, nblocks, lds, s);
         default: return hipErrorInvalidValue;
     }
 }

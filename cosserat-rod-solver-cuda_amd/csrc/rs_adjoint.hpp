@@ -44,16 +44,22 @@ struct AdjDev {
     int* mismatch;  // count of vertices whose recomputed end state differs from the next checkpoint
     const Pose* fposes;     // frame-start poses [N][numSdf]
     const Motion* fmotion;  // frame motion [N][numSdf] while meshes move, else null
+    // with rope-rope contact, per local vertex: the substep's partner list (workgroup-local slots, ascending) and
+    // the list of the segments whose lists hold it, MAX_COLL entries each ([hn][2][MAX_COLL]); a pass rebuilds
+    // them at every substep
+    uint16_t* lists;
 };
 
-// Longest rope the backward kernels support; pose slots a workgroup keeps in LDS (its ropes times the meshes);
-// history bytes per vertex-substep.
-constexpr int ADJ_MAX_BLOCK = 256;
+// Longest rope (or rope contact group) the backward kernels support; pose slots a workgroup keeps in LDS (its
+// ropes times the meshes); history bytes per vertex-substep.
+constexpr int ADJ_MAX_BLOCK = 512;
 constexpr int adjoint_pose_slots(int block) { return block > MAX_SDF ? block : MAX_SDF; }
 size_t adjoint_history_bytes(int numItr);
-bool adjoint_block_supported(int block, bool reverse);  // 64, 128, 192, 256; 512 for the reverse pass
+bool adjoint_block_supported(int block, bool reverse);  // 64, 128, 192, 256, 384, 512
 // Waves per SIMD the pass's register allocation allows at this block size (the device's occupancy limit).
-int adjoint_waves_per_simd(int block, bool reverse, bool fast);
+int adjoint_waves_per_simd(int block, bool reverse, bool fast, bool ropeContact);
+// Dynamic LDS of the rope-rope detection arrays of one workgroup (beside its pose slots).
+size_t adjoint_rc_lds_bytes(int block);
 // One pass (forward recompute, or reverse) over `nblocks` workgroups of `block` threads, workgroup b on the
 // ropes [A.blk[b], A.blk[b + 1]), at most ropesPerBlock of them (their mesh poses take ropesPerBlock * numSdf
 // LDS slots, at most adjoint_pose_slots(block)). The two passes of a frame may use different packings: they

@@ -163,6 +163,7 @@ struct rs_batch {
     struct Tape {
         bool on = false, closed = false;
         int maxFrames = 0, frames = 0, numSdf = 0, maxS = 0;
+        int unitRopes = 1;         // ropes per adjoint unit (adjoint_unit_ropes at the first recorded frame)
         float4* ck = nullptr;      // [(maxFrames + 1)][4][nv]: x, v, v_last, q at each frame start (+ the end)
         float* prog = nullptr;     // [maxFrames][17 N] staged root programs
         Pose* poses = nullptr;     // [maxFrames][N][numSdf]
@@ -202,6 +203,8 @@ struct rs_batch {
     int* d_mismatch = nullptr;
     int* d_blk = nullptr;  // the adjoint's workgroup -> rope ranges
     size_t blkCap = 0;
+    uint16_t* d_adjLists = nullptr;  // the adjoint's rope-rope lists ([chunk vertices][2][MAX_COLL])
+    size_t adjListsCap = 0;
 
     ~rs_batch() {
         if (stream) hipStreamSynchronize(stream);
@@ -215,7 +218,7 @@ struct rs_batch {
                         D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_cellTime, d_cellQueue, d_hits,
                         d_grids, d_cur, d_tgt, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
                         tape.poses, tape.motion, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
-                        d_gdrag, d_mismatch, d_blk};
+                        d_gdrag, d_mismatch, d_blk, d_adjLists};
         for (void* p : ptrs)
             if (p) hipFree(p);
         for (float* g : gridMem) hipFree(g);
@@ -845,13 +848,19 @@ struct rs_batch {
     void gather_nolock() { hcheck(launch_gather(params(), D, stream), "gather"); }
 
     // ---------------------------------------------------------------- differentiation
+    // Ropes the adjoint keeps in one workgroup: one rope, or a whole contact group while rope-rope contact is on.
+    int adjoint_unit_ropes() const { return ropeContact ? groupRopes : 1; }
     void tape_check() const {
-        if (ropeContact) throw Invalid("the adjoint does not cover rope-rope contact; disable it while recording");
         if (dragQuad > 0.f) throw Invalid("the adjoint does not cover quadratic drag; set drag_quad = 0 while recording");
         if ((int)grids.size() != tape.numSdf) throw Invalid("meshes were added or cleared while recording");
+        if (tape.frames > 0 && adjoint_unit_ropes() != tape.unitRopes)
+            throw Invalid("rope-rope contact or its group size changed while recording");
+        const int G = adjoint_unit_ropes();
         int maxV = 0;
-        for (int r = 0; r < N; r++) maxV = std::max(maxV, roff[r + 1] - roff[r]);
-        if (maxV > ADJ_MAX_BLOCK) throw Invalid("the adjoint supports ropes of at most 256 vertices");
+        for (int r = 0; r < N; r += G) maxV = std::max(maxV, roff[std::min(r + G, N)] - roff[r]);
+        if (maxV > ADJ_MAX_BLOCK)
+            throw Invalid(ropeContact ? "the adjoint supports rope contact groups of at most 512 vertices"
+                                      : "the adjoint supports ropes of at most 512 vertices");
     }
     void copy_state(float4* dst) {
         const size_t n = nv;
@@ -864,6 +873,7 @@ struct rs_batch {
         tape_check();
         if (tape.frames >= tape.maxFrames) throw Invalid("the tape is full (max_frames of rs_tape_begin)");
         const int f = tape.frames;
+        if (f == 0) tape.unitRopes = adjoint_unit_ropes();
         copy_state(tape.ck + (size_t)f * 4 * nv);
         hcheck(hipMemcpyAsync(tape.prog + (size_t)f * 17 * N, d_prog, 17 * (size_t)N * 4, hipMemcpyDeviceToDevice,
                               stream),
@@ -1745,9 +1755,15 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
         if (!T.closed || T.frames < 1) throw Invalid("no recorded rollout: call rs_tape_begin, then step");
         const int F = T.frames, N = b->N;
         const size_t n = b->nv;
+        // A unit is what a workgroup never splits: one rope, or a whole contact group of U consecutive ropes when
+        // the rollout has rope-rope contact (the group's lists and pair terms stay inside the workgroup).
+        const int U = T.unitRopes;
+        bool anyRC = false;
+        for (const Params& P : T.P) anyRC = anyRC || P.ropeContact != 0;
+        auto unit_end = [&](int r) { return std::min(r + U, N); };
         int maxV = 0;
-        for (int r = 0; r < N; r++) maxV = std::max(maxV, b->roff[r + 1] - b->roff[r]);
-        // Workgroups of each pass: consecutive ropes packed up to B vertices (and the LDS pose slots). The passes
+        for (int r = 0; r < N; r = unit_end(r)) maxV = std::max(maxV, b->roff[unit_end(r)] - b->roff[r]);
+        // Workgroups of each pass: consecutive units packed up to B vertices (and the LDS pose slots). The passes
         // share only the per-vertex history, so each takes its own packing.
         const int M = T.numSdf;
         const bool fastAdj = T.P.empty() ? b->fastMath : T.P[0].fastMath != 0;  // the recorded arithmetic
@@ -1757,9 +1773,11 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
             while (r < r1) {
                 const int start = r;
                 int nvb = 0;
-                while (r < r1 && r - start < ropesPerBlock && nvb + (b->roff[r + 1] - b->roff[r]) <= B) {
-                    nvb += b->roff[r + 1] - b->roff[r];
-                    r++;
+                while (r < r1) {
+                    const int e = std::min(unit_end(r), r1);
+                    if (r > start && (e - start > ropesPerBlock || nvb + (b->roff[e] - b->roff[r]) > B)) break;
+                    nvb += b->roff[e] - b->roff[r];
+                    r = e;
                 }
                 if (out) out->push_back(start);
                 waves += (nvb + 63) / 64;
@@ -1778,9 +1796,11 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
         auto choose = [&](bool reverse) {
             Packing best;
             double bestCost = 0.0;
-            for (int B : {64, 128, 192, 256, 512}) {
+            for (int B : {64, 128, 192, 256, 384, 512}) {
                 if (B < maxV || !adjoint_block_supported(B, reverse)) continue;
-                const int perCU = 4 * adjoint_waves_per_simd(B, reverse, fastAdj) / (B / 64);
+                // LDS: the pose slots and, with rope contact, the detection arrays
+                if (M > 0 && (size_t)U * M > (size_t)adjoint_pose_slots(B)) continue;
+                const int perCU = 4 * adjoint_waves_per_simd(B, reverse, fastAdj, anyRC) / (B / 64);
                 if (perCU < 1) continue;
                 if (best.blk < 0) {
                     std::vector<int> t;
@@ -1789,7 +1809,7 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
                     bestCost = (double)(t.size() - 1) / perCU;
                 }
                 const int maxRopes = M > 0 ? adjoint_pose_slots(B) / M : B;
-                if (maxRopes < 2) continue;
+                if (maxRopes < 2 * U) continue;
                 std::vector<int> t;
                 pack(B, maxRopes, 0, N, &t);
                 const double c = (double)(t.size() - 1) / perCU;
@@ -1799,7 +1819,7 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
                     bestCost = c;
                 }
             }
-            if (best.blk < 0) throw Invalid("the adjoint supports ropes of at most 256 vertices");
+            if (best.blk < 0) throw Invalid("the adjoint supports ropes (rope contact groups) of at most 512 vertices");
             return best;
         };
         const Packing pk[2] = {choose(false), choose(true)};
@@ -1817,10 +1837,12 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
         std::vector<Chunk> chunks;
         for (int r = 0; r < N;) {
             Chunk c{r, 0, b->roff[r], 0};
-            while (r < N && (size_t)(c.nv + b->roff[r + 1] - b->roff[r]) * perV <= budget) {
-                c.nv += b->roff[r + 1] - b->roff[r];
-                c.nropes++;
-                r++;
+            while (r < N) {
+                const int e = unit_end(r);
+                if (c.nropes > 0 && (size_t)(c.nv + b->roff[e] - b->roff[r]) * perV > budget) break;
+                c.nv += b->roff[e] - b->roff[r];
+                c.nropes += e - r;
+                r = e;
             }
             chunks.push_back(c);
         }
@@ -1845,8 +1867,16 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
         hcheck(hipMemcpyAsync(b->d_blk, blkTable.data(), blkTable.size() * sizeof(int), hipMemcpyHostToDevice,
                               b->stream),
                "adjoint workgroups");
-        size_t need = 0;
-        for (const Chunk& c : chunks) need = std::max(need, (size_t)c.nv * perV);
+        size_t need = 0, maxChunkV = 0;
+        for (const Chunk& c : chunks) {
+            need = std::max(need, (size_t)c.nv * perV);
+            maxChunkV = std::max(maxChunkV, (size_t)c.nv);
+        }
+        if (anyRC && maxChunkV * 2 * MAX_COLL > b->adjListsCap) {
+            if (b->d_adjLists) hipFree(b->d_adjLists);
+            b->d_adjLists = dalloc<uint16_t>(maxChunkV * 2 * MAX_COLL, "adjoint rope-rope lists");
+            b->adjListsCap = maxChunkV * 2 * MAX_COLL;
+        }
         if (need > b->histBytes) {
             if (b->d_hist) hipFree(b->d_hist);
             b->d_hist = dalloc<char>(need, "adjoint history");
@@ -1919,6 +1949,7 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
                 A.mismatch = b->d_mismatch;
                 A.fposes = M > 0 ? T.poses + (size_t)f * N * M : nullptr;
                 A.fmotion = (M > 0 && T.P[f].sdfMoving) ? T.motion + (size_t)f * N * M : nullptr;
+                A.lists = anyRC ? b->d_adjLists : nullptr;
                 for (int pass = 0; pass < 2; pass++) {
                     A.blk = b->d_blk + chunkBlk0[pass][ci];
                     hcheck(launch_adjoint_pass(pass == 1, pk[pass].blk, T.P[f], b->D, A, dr, S, chunkBlks[pass][ci],
@@ -1934,7 +1965,14 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
         int mis = 0;
         hcheck(hipMemcpyAsync(gd.data(), b->d_gdrag, N * 8, hipMemcpyDeviceToHost, st), "read drag adjoint");
         hcheck(hipMemcpyAsync(&mis, b->d_mismatch, 4, hipMemcpyDeviceToHost, st), "read mismatch");
+        int err[2] = {0, 0};
+        if (anyRC) hcheck(hipMemcpyAsync(err, b->D.err, sizeof(err), hipMemcpyDeviceToHost, st), "read error word");
         hcheck(hipStreamSynchronize(st), "adjoint");
+        if (err[0] != 0 || err[1] != 0) {
+            hcheck(hipMemsetAsync(b->D.err, 0, sizeof(err), st), "reset error word");
+            hcheck(hipStreamSynchronize(st), "reset error word");
+            throw SolverErr("MAX_COLLISIONS_PER_SEGMENT exceeded in the adjoint's rope-rope lists");
+        }
         double sum = 0.0;
         for (double v : gd) sum += v;
         if (g_drag) *g_drag = sum;

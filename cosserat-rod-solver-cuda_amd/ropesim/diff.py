@@ -13,8 +13,9 @@ each frame. Gradients of any loss on those positions flow back to:
 The forward records one checkpoint per frame. The backward is the hand-written adjoint of the substep
 (RopeBatch.adjoint). It recomputes each frame's substeps from its checkpoint, so memory grows with
 the number of frames, not substeps. The active set of the forward is frozen: contact on/off, friction
-branch, orientation signs, broad-phase lists. Not covered: rope-rope contact and quadratic drag; recording
-refuses both. The reference has no adjoint; this module has no counterpart in it.
+branch, orientation signs, broad-phase lists, rope-rope candidate lists (the barrier, its friction and the
+step limit are differentiated with the lists fixed). Not covered: quadratic drag; recording refuses it, and
+rope contact groups above 512 vertices. The reference has no adjoint; this module has no counterpart in it.
 """
 from __future__ import annotations
 

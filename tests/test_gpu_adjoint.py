@@ -21,7 +21,8 @@ stays at 1e-3 or twice the float32 gap.
 import numpy as np
 import pytest
 
-from adjoint_cases import contact_scene, make_oracle, nominal_material, oracle_loss, program, rel, targets, torch_grads
+from adjoint_cases import (contact_scene, make_oracle, nominal_material, oracle_loss, program, rel, rope_contact_scene,
+                           targets, torch_grads)
 
 pytestmark = pytest.mark.gpu
 
@@ -92,7 +93,7 @@ def test_rollout_forward_matches_plain_step(rs, case):
         assert torch.equal(xf, case["x"][f]), f"frame {f}"
 
 
-def _floor_check(rs, oracle, s, b, g_hip, mat, prog, tips, tgt, label, g32s=None):
+def _floor_check(rs, oracle, s, b, g_hip, mat, prog, tips, tgt, label, g32s=None, factor=2.0):
     """Every gradient family within 1e-3 of float64 autograd. Where float32 arithmetic itself moves the
     gradient further, the bar is twice that gap (the largest gap of the float32 autograd realisations of
     the same transcription against float64), as in the positions tests."""
@@ -106,7 +107,7 @@ def _floor_check(rs, oracle, s, b, g_hip, mat, prog, tips, tgt, label, g32s=None
     print(f"\n  {label}: adjoint vs float64 autograd (largest float32 autograd gap):",
           ", ".join(f"{k} {e:.1e} ({gap:.1e})" for k, e, gap in rows))
     for k, e, gap in rows:
-        assert e <= max(TOL, 2 * gap), (k, e, gap)
+        assert e <= max(TOL, factor * gap), (k, e, gap)
 
 
 def test_adjoint_matches_float64_autograd(rs, oracle, case, g32s):
@@ -140,6 +141,10 @@ def test_adjoint_matches_central_differences(oracle, case, g32s, which, eps):
     reasons as tests/test_torch_transcription.py. Bar: 1e-3, or twice the largest gap of the float32
     autograd realisations' projections to the central difference where float32 arithmetic alone moves it
     further."""
+    _fd_check(oracle, case, g32s, which, eps)
+
+
+def _fd_check(oracle, case, g32s, which, eps, factor=2.0):
     s = case["s"]
     mat, prog, tips, tgt, g = case["mat"], case["prog"], case["tips"], case["tgt"], case["g"]
     if which in ("ea", "ei"):
@@ -169,10 +174,11 @@ def test_adjoint_matches_central_differences(oracle, case, g32s, which, eps):
     gap = max(abs(float(np.sum(g32[which] * v)) - fd) / abs(fd) for g32 in g32s)
     print(f"\n  {which} fast {case['fast']}: adjoint {ad:.6e}  central difference {fd:.6e}  "
           f"(largest float32 autograd gap {gap:.1e})")
-    assert abs(ad - fd) <= max(TOL, 2 * gap) * abs(fd), (ad, fd, gap)
+    assert abs(ad - fd) <= max(TOL, factor * gap) * abs(fd), (ad, fd, gap)
 
 
 def test_recording_refuses_what_the_adjoint_does_not_cover(rs):
+    """Quadratic drag is not covered; a rope contact group must fit one workgroup (512 vertices)."""
     s = contact_scene(n_ropes=2, frames=1)
     b = s.build(rs)
     b.drag_quad = 0.5
@@ -180,8 +186,147 @@ def test_recording_refuses_what_the_adjoint_does_not_cover(rs):
         b.tape_begin(2)
     b.drag_quad = 0.0
     b.set_rope_contact(True, 0.001, 0.001, 0.01, 0.1, 1.0)
-    with pytest.raises(ValueError, match="rope-rope"):
-        b.tape_begin(2)
+    b.tape_begin(2)  # two 33-vertex ropes in one group: covered
+    b.tape_end()
+    pos = np.zeros((2, 300, 3), np.float32)
+    pos[:, :, 0] = 0.01 * np.arange(300)
+    pos[1, :, 1] = 0.1
+    big = rs.RopeBatch.from_positions(pos, 0.1, 500.0, 1e-4, 1 / 3000, 8)
+    big.set_rope_contact(True, 0.001, 0.001, 0.01, 0.1, 1.0, 1.2, 2)
+    with pytest.raises(ValueError, match="512 vertices"):
+        big.tape_begin(2)
+
+
+# ---------------------------------------------------------------- rope-rope contact
+# Three frames of the crossing-rope scene: 88 active barrier terms (each with friction) and 167 clamped steps.
+# There float32 autograd of the transcription sits a stable 6e-5..2e-3 from float64 (the same under every
+# rounding-level EA perturbation: a bias of float32 positions, not a branch), so the bar separates a wrong
+# term from rounding. The IEEE mode is held to twice that floor. The fast mode (1-ulp hardware reciprocal,
+# square root and log in the forward) measured 1.5..3.2 times the floor (v0 2.0e-3 against 6.3e-4, p0 3.2e-3
+# against 2.1e-3) where the IEEE mode lands on it (6.3e-4, 2.0e-3): the adjoint's code is the same in both
+# modes, so the fast mode is held to four times the floor.
+RC_FACTOR = {True: 4.0, False: 2.0}
+
+
+@pytest.fixture(scope="module", params=[True, False], ids=["fast", "ieee"])
+def rc_case(rs, oracle, request):
+    s = rope_contact_scene(frames=3)
+    b = s.build(rs)
+    b.fast_math = request.param
+    mat = nominal_material(s, b)
+    prog = program(s)
+    tips, tgt = targets(s)
+    g_hip, l_hip, x = hip_grads(rs, s, b, mat, s.drag, prog, tips, tgt)
+    return dict(s=s, b=b, mat=mat, prog=prog, tips=tips, tgt=tgt, g=g_hip, loss=l_hip, x=x, fast=request.param)
+
+
+@pytest.fixture(scope="module")
+def rc_g32s(oracle):
+    s = rope_contact_scene(frames=3)
+    import ropesim
+    mat = nominal_material(s, s.build(ropesim))
+    return fp32_realisations(oracle, s, mat, program(s), *targets(s))
+
+
+def test_rope_contact_rollout_forward_matches_plain_step(rs, rc_case):
+    """The recorded rollout with rope-rope contact is the solver's ordinary stepping, bit for bit (the
+    backward's recompute is then checked against these checkpoints: a mismatch raises)."""
+    import torch
+    s, mat, prog = rc_case["s"], rc_case["mat"], rc_case["prog"]
+    b = s.build(rs)
+    b.fast_math = rc_case["fast"]
+    b.set_material(*(a.astype(np.float32) for a in mat))
+    b.drag = s.drag
+    for f in range(s.frames):
+        b.step(*(torch.tensor(a[f], dtype=torch.float32, device="cuda") for a in prog[:2]), s.dt, s.substeps,
+               root_pos_end=torch.tensor(prog[2][f], dtype=torch.float32, device="cuda"))
+        xf = torch.as_tensor(b.positions(), device="cuda").reshape(-1, 3)
+        assert torch.equal(xf, rc_case["x"][f]), f"frame {f}"
+
+
+def test_rope_contact_adjoint_matches_float64_autograd(rs, oracle, rc_case, rc_g32s):
+    """Crossing ropes pressing and sliding on each other (the barrier, its friction and the step limit all
+    act: tests/test_torch_transcription.py counts them). The adjoint differentiates the lists as frozen and
+    the step limit's distances and paddings as the transcription does."""
+    s = rc_case["s"]
+    _, l_ref = torch_grads(oracle, s, rc_case["mat"], s.drag, rc_case["prog"], rc_case["tips"], rc_case["tgt"])
+    assert abs(rc_case["loss"] - l_ref) <= 1e-4 * abs(l_ref)
+    _floor_check(rs, oracle, s, rc_case["b"], rc_case["g"], rc_case["mat"], rc_case["prog"], rc_case["tips"],
+                 rc_case["tgt"], f"rope-rope contact, fast {rc_case['fast']}", rc_g32s, RC_FACTOR[rc_case["fast"]])
+
+
+@pytest.mark.parametrize("which,eps", [("ea", 1e-6), ("ei", 1e-5), ("drag", 1e-6), ("p1", 1e-8), ("v0", 1e-6)])
+def test_rope_contact_adjoint_matches_central_differences(oracle, rc_case, rc_g32s, which, eps):
+    """As test_adjoint_matches_central_differences, on the rope-rope contact scene (steps as in
+    tests/test_torch_transcription.py for it)."""
+    _fd_check(oracle, rc_case, rc_g32s, which, eps, RC_FACTOR[rc_case["fast"]])
+
+
+def test_rope_contact_adjoint_six_frames(rs, oracle):
+    """The same scene over six frames (630 barrier terms, 590 clamped steps): the contacts make the discrete map
+    sensitive enough that float32 autograd of the transcription lands 2e-2..3e-2 from float64 under every
+    rounding-level perturbation. The IEEE device mode is held to twice that floor (measured 1.8e-2..3.4e-2 against
+    floors of 1.7e-2..3.3e-2)."""
+    s = rope_contact_scene(frames=6)
+    b = s.build(rs)
+    b.fast_math = False
+    mat = nominal_material(s, b)
+    prog = program(s)
+    tips, tgt = targets(s)
+    g, _, _ = hip_grads(rs, s, b, mat, s.drag, prog, tips, tgt)
+    _floor_check(rs, oracle, s, b, g, mat, prog, tips, tgt, "rope-rope contact, 6 frames, ieee")
+
+
+def _rc_groups_batch(rs, n_groups, F=2, V=24):
+    """n_groups environments of two crossing ropes each, geometry, program and loss weights derived from the
+    group index only, so the first group of a large batch is exactly the group of a one-group batch. Returns
+    the gradients of a weighted-position loss."""
+    import torch
+    g = np.arange(n_groups)
+    base = np.column_stack([0.5 * (g % 64), 0.5 * (g // 64), np.full(n_groups, 0.3)])
+    ropes = []
+    for k in range(2):
+        ang = 0.1 * np.sin(g * 0.7 + k) + k * np.pi / 2
+        d = np.column_stack([np.cos(ang), np.sin(ang), np.full(n_groups, -0.15 - 0.1 * k)])
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        root = base - 0.01 * (V - 1) * np.column_stack([np.cos(ang), np.sin(ang), np.zeros(n_groups)]) + [0, 0, 0.012 * k]
+        ropes.append(root[:, None, :] + 0.02 * np.arange(V)[None, :, None] * d[:, None, :])
+    pos = np.stack(ropes, 1).reshape(2 * n_groups, V, 3).astype(np.float32)
+    b = rs.RopeBatch.from_positions(pos, 0.1, 500.0, 1e-4, 1 / 1200, 8)
+    b.set_gravity(np.array([0, 0, -9.81], np.float32))
+    b.damping = 0.0
+    b.set_rope_contact(True, 0.004, 0.001, 0.025, 0.4, 1e7, 1.2, 2)
+    n, nv = 2 * n_groups, 2 * n_groups * V
+    t = lambda a: torch.tensor(np.asarray(a, np.float32), device="cuda", requires_grad=True)
+    jit = 1.0 + 0.1 * np.sin(np.arange(nv) * 0.37)
+    EA, EI = t(500.0 * jit), t(1e-4 * (2.0 - jit))
+    M = t(np.tile(np.r_[np.full(V - 1, 0.002), 0.001], n))
+    roots = pos[:, 0].astype(np.float64)
+    idx = np.arange(n)
+    p0 = np.repeat(roots[None], F, 0)
+    v0 = 0.1 * np.stack([np.column_stack([np.cos(idx + f), np.sin(idx + f), np.zeros(n)]) for f in range(F)])
+    P0, V0, P1 = t(p0), t(v0), t(p0 + 0.01 * v0)
+    D = torch.tensor(0.2, device="cuda", requires_grad=True)
+    x = rs.diff.rollout(b, EA, EI, M, D, P0, V0, P1, 1 / 60, 20)
+    w = torch.tensor(np.cos(np.arange(nv * 3) * 0.11).reshape(nv, 3), device="cuda", dtype=torch.float32)
+    ((x * w).sum(0) ** 2).sum().backward()
+    return {k: v.grad.detach().cpu().numpy() for k, v in dict(ea=EA, ei=EI, m=M, p0=P0, v0=V0, p1=P1).items()}
+
+
+def test_rope_contact_gradients_do_not_depend_on_packing(rs):
+    """8192 two-rope groups of 48 vertices pack several groups per workgroup; one group gets a workgroup of its
+    own. The adjoint sums every rope-rope contribution in a fixed order (no atomics), so the first group's
+    gradients agree bit for bit, and a second run repeats them."""
+    V = 24
+    big = _rc_groups_batch(rs, 8192, V=V)
+    small = _rc_groups_batch(rs, 1, V=V)
+    again = _rc_groups_batch(rs, 1, V=V)
+    for name in ("ea", "ei", "m"):
+        assert np.array_equal(big[name][:2 * V].view(np.uint32), small[name].view(np.uint32)), name
+        assert np.array_equal(again[name].view(np.uint32), small[name].view(np.uint32)), name
+    for name in ("p0", "v0", "p1"):
+        assert np.array_equal(big[name][:, :2].view(np.uint32), small[name].view(np.uint32)), name
+    assert np.abs(small["ea"]).max() > 0
 
 
 @pytest.mark.parametrize("fast", [True, False], ids=["fast", "ieee"])
