@@ -160,7 +160,9 @@ __device__ __forceinline__ float surfaces_adj(const Params& P, const Grid* grids
 //
 // Reverse of the closest-point parameters (u, v) = seg_closest(0, A, B0, B1) by its three steps (infinite-line
 // v, clamped; u from it, clamped; v again from u, clamped): the adjoints of A, B0, B1 given those of u and v. A
-// clamped step passes no gradient unless its value lies in [0, 1]; the singular branch (v = 0.5) passes none.
+// clamped step passes no gradient unless its value lies in [0, 1]; the singular branch (v = 0.5) passes none,
+// and neither does the infinite-line step of nearly parallel segments (a c - b^2 <= 1e-4 a c, within 0.6
+// degrees), whose v float32 cannot resolve and whose derivative grows as 1 / sin^2 of their angle.
 __device__ void seg_closest_adj(f3 A, f3 B0, f3 B1, float ub, float vb, f3& Ab, f3& B0b, f3& B1b) {
     const f3 A1 = B1 - B0;
     const float a = dot(A, A), b = dot(A, A1), c = dot(A1, A1);
@@ -192,7 +194,8 @@ __device__ void seg_closest_adj(f3 A, f3 B0, f3 B1, float ub, float vb, f3& Ab, 
     A0b = A0b + n1b * m1;
     Db = Db + m1b;
     A1b = A1b + y0c * m1b;
-    const float y0b = (fin && y0 >= 0.f && y0 <= 1.f) ? dot(m1b, A1) : 0.f;
+    const bool ill = !(den > 1e-4f * (a * c));
+    const float y0b = (fin && !ill && y0 >= 0.f && y0 <= 1.f) ? dot(m1b, A1) : 0.f;
     if (y0b != 0.f) {
         const float numb = y0b / den, denb = -y0b * y0 / den;
         ab += denb * c;
@@ -218,9 +221,18 @@ __device__ void seg_closest_adj(f3 A, f3 B0, f3 B1, float ub, float vb, f3& Ab, 
 // vertex i+1's u2, l2. Returns the adjoints of the term's inputs in g[0..7]: p0, p1, d0, p1dx (segment i, i+1)
 // and b0, b1, db0, db1 (segment j, j+1). The forward part repeats collision_eval's arithmetic, so every branch
 // (contact, the 1e-3 floor of the gap, friction, the friction cap) is the one the forward took.
+// The rope-rope material a pair term reads, passed by value: a reference to the kernel's Params would make the
+// compiler keep a copy of the whole struct in per-lane scratch memory for the call.
+struct PairMat {
+    float barrier, radius, kCollision, fricMu, kFriction;
+};
+__device__ __forceinline__ PairMat pair_mat(const Params& P) {
+    return PairMat{P.barrier, P.radius, P.kCollision, P.fricMu, P.kFriction};
+}
+
 template <bool F>
-__device__ __noinline__ void pair_term_adj(const Params& P, float dmp, f3 p0, f3 p1, f3 d0, f3 p1dx, f3 b0, f3 b1,
-                                           f3 db0, f3 db1, f3 u1, f3 l1, f3 u2, f3 l2, f3* g) {
+__device__ __noinline__ void pair_term_adj(PairMat P, float dmp, f3 p0, f3 p1, f3 d0, f3 p1dx, f3 b0, f3 b1, f3 db0,
+                                           f3 db1, f3 u1, f3 l1, f3 u2, f3 l2, f3* g) {
     using Af = Ar<F>;
     using M = V<F>;
     for (int k = 0; k < 8; k++) g[k] = mk3(0.f, 0.f, 0.f);
@@ -318,7 +330,9 @@ __device__ __noinline__ void pair_term_adj(const Params& P, float dmp, f3 p0, f3
 
 // Adjoint of a pair's step-limit distance 0.45 |u a1 - mix(b0, b1, v)| with a1, b0, b1 relative to xa0 and
 // (u, v) = seg_closest(0, a1, b0, b1) (YB/sim/collision.cu:103-136, the detection's arithmetic), scaled by gb:
-// the adjoints of the four endpoint positions.
+// the adjoints of the four endpoint positions. The distance between two segments is a minimum over (u, v),
+// so its derivative has no term through them (at an interior minimum the distance is stationary in u and v,
+// at a clamped one they do not move); (u, v) are held fixed.
 __device__ __noinline__ void limit_dist_adj(f3 xa0, f3 xa1, f3 xb0, f3 xb1, float gb, f3* g) {
     const f3 a1 = xa1 - xa0, b0 = xb0 - xa0, b1 = xb1 - xa0;
     float u, v;
@@ -328,13 +342,7 @@ __device__ __noinline__ void limit_dist_adj(f3 xa0, f3 xa1, f3 xb0, f3 xb1, floa
     for (int k = 0; k < 4; k++) g[k] = mk3(0.f, 0.f, 0.f);
     if (!(l > 0.f)) return;
     const f3 Db = (0.45f * gb / l) * d;
-    f3 a1b = u * Db, b0b = -(1.f - v) * Db, b1b = -v * Db;
-    const float ub = dot(Db, a1), vb = -dot(Db, b1 - b0);
-    f3 Ab, B0b, B1b;
-    seg_closest_adj(a1, b0, b1, ub, vb, Ab, B0b, B1b);
-    a1b = a1b + Ab;
-    b0b = b0b + B0b;
-    b1b = b1b + B1b;
+    const f3 a1b = u * Db, b0b = -(1.f - v) * Db, b1b = -v * Db;
     g[0] = -(a1b + b0b + b1b);
     g[1] = a1b;
     g[2] = b0b;
@@ -344,14 +352,19 @@ __device__ __noinline__ void limit_dist_adj(f3 xa0, f3 xa1, f3 xb0, f3 xb1, floa
 // Rope-rope detection of a workgroup's contact groups at one substep, from the positions and paddings in LDS:
 // the reference's per-segment test in ascending order (k_detect, csrc/batch.cu:19-49) with its step limit
 // (YB/sim/collision.cu:103-136), reproduced operation for operation so the lists and limits are the forward's.
-// Candidates are culled first by chunk boxes (16 slots) and then by segment boxes, each padded by the
-// segment's detection radius and a rounding margin, which keeps every pair the exact test can accept.
-// Lists hold local slots, ascending; with REV the segment also lists, ascending, the segments whose own lists
-// hold it. Returns the count (fwd), the limit and the slot whose distance set it (-1: the padding).
+// The candidates come from skin lists: at a build each segment records, ascending, the partners whose box
+// (padded by the build radius R = max(pad) + slack, the skin and a rounding margin) meets its own. Until some
+// vertex has moved more than the skin from its build position or some padding has outgrown R, every pair the
+// exact test can accept is on those lists (two segments approach by at most twice the skin). A list longer than
+// SKIN_CAP falls back to a scan of the whole contact group. Lists hold workgroup-local slots.
+constexpr int SKIN_CAP = 64;
+constexpr int LIST_STRIDE = 2 * MAX_COLL + SKIN_CAP;  // per local vertex: partner list, reverse list, skin list
+
 struct RcLds {
-    float *x, *pad, *lim, *sb, *cb;  // positions [3][B], paddings [B], limits [B], segment boxes [6][B], chunk boxes [6][B/16]
-    float *dk, *mb;                  // (reverse pass) iterate d^k [3][B], step-limit adjoints [B]
-    int *fl, *rope, *arg, *nrev;     // flags [B], rope [B], limit argument [B], partner-list lengths [B]
+    float *x, *pad, *lim, *x0, *R, *sb, *cb;  // positions [3][B], paddings [B], limits [B], build positions [3][B],
+                                              // build radii [B], build boxes [6][B], chunk boxes [6][B/16]
+    float *dk, *mb, *mown, *mprev;            // (reverse pass) iterate d^k [3][B], step-limit adjoints [B] x 3
+    int *fl, *rope, *arg, *nrev, *ncol, *vc;  // flags, rope, limit argument, list lengths, skin-list length (-1: scan)
 };
 
 // The detection arrays in dynamic LDS at `base` (after the mesh pose slots); x: positions already in LDS, or
@@ -360,44 +373,30 @@ template <int B>
 __device__ __forceinline__ RcLds rc_lds(char* base, float* x) {
     RcLds L;
     float* f = reinterpret_cast<float*>(base);
-    L.pad = f;
-    f += B;
-    L.lim = f;
-    f += B;
-    L.sb = f;
-    f += 6 * B;
-    L.cb = f;
-    f += 6 * (B / 16);
-    L.dk = f;
-    f += 3 * B;
-    L.mb = f;
-    f += B;
-    L.x = x;
-    if (!x) {
-        L.x = f;
-        f += 3 * B;
-    }
+    auto take = [&](int n) {
+        float* p = f;
+        f += n;
+        return p;
+    };
+    L.pad = take(B);
+    L.lim = take(B);
+    L.x0 = take(3 * B);
+    L.R = take(B);
+    L.sb = take(6 * B);
+    L.cb = take(6 * (B / 16));
+    L.dk = take(3 * B);
+    L.mb = take(B);
+    L.mown = take(B);
+    L.mprev = take(B);
+    L.x = x ? x : take(3 * B);
     int* q = reinterpret_cast<int*>(f);
     L.fl = q;
     L.rope = q + B;
     L.arg = q + 2 * B;
     L.nrev = q + 3 * B;
+    L.ncol = q + 4 * B;
+    L.vc = q + 5 * B;
     return L;
-}
-
-__device__ __forceinline__ void seg_box(const RcLds& L, int B, int t, uint32_t fl, f3& lo, f3& hi) {
-    if (!(fl & HAS_NEXT)) {
-        lo = mk3(INFINITY, INFINITY, INFINITY);
-        hi = mk3(-INFINITY, -INFINITY, -INFINITY);
-        return;
-    }
-    const f3 a = ld3s(L.x, B, t), b = ld3s(L.x, B, t + 1);
-    const float r = fmaxf(L.pad[t], L.pad[t + 1]);
-    const float big = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(b.x))),
-                            fmaxf(fabsf(b.y), fabsf(b.z)));
-    const float m = r + 1e-6f + 1e-5f * big;
-    lo = mk3(fminf(a.x, b.x) - m, fminf(a.y, b.y) - m, fminf(a.z, b.z) - m);
-    hi = mk3(fmaxf(a.x, b.x) + m, fmaxf(a.y, b.y) + m, fmaxf(a.z, b.z) + m);
 }
 
 __device__ __forceinline__ bool box_meets(const float* bx, int stride, int k, f3 lo, f3 hi) {
@@ -405,11 +404,33 @@ __device__ __forceinline__ bool box_meets(const float* bx, int stride, int k, f3
              bx[4 * stride + k] < lo.y || bx[5 * stride + k] < lo.z);
 }
 
-// Stage 1 (all lanes, then a barrier by the caller): segment boxes and chunk boxes.
+// The rebuild vote of slot t (every substep; the caller ORs it over the workgroup): its vertex moved past the
+// skin since the build, or its padding outgrew the build radius of a segment it belongs to.
+__device__ __forceinline__ bool rc_skin_stale(const Params& P, const RcLds& L, int B, int t, bool act, uint32_t fl) {
+    if (!act) return false;
+    const float skin = 2.f * P.scaledDetR;
+    const f3 dx = ld3s(L.x, B, t) - ld3s(L.x0, B, t);
+    const float pad = L.pad[t];
+    return len2(dx) > skin * skin || ((fl & HAS_NEXT) && pad > L.R[t]) || ((fl & HAS_PREV) && pad > L.R[t - 1]);
+}
+
+// Skin-list build, stage 1 (all lanes; a barrier follows): build positions and radii, boxes, chunk boxes.
 template <int B>
-__device__ __forceinline__ void rc_boxes(const RcLds& L, int t, bool act, uint32_t fl) {
-    f3 lo, hi;
-    seg_box(L, B, t, act ? fl : 0u, lo, hi);
+__device__ __forceinline__ void rc_skin_boxes(const Params& P, const RcLds& L, int t, bool act, uint32_t fl) {
+    const float skin = 2.f * P.scaledDetR, slackR = 0.25f * P.scaledDetR;
+    f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = mk3(-INFINITY, -INFINITY, -INFINITY);
+    if (act) {
+        const f3 a = ld3s(L.x, B, t);
+        st3s(L.x0, B, t, a);
+        if (fl & HAS_NEXT) {
+            const f3 b = ld3s(L.x, B, t + 1);
+            const float R = fmaxf(L.pad[t], L.pad[t + 1]) + slackR;
+            L.R[t] = R;
+            const float rb = R + skin + 1e-4f + 2e-6f * fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z)));
+            lo = mk3(fminf(a.x, b.x) - rb, fminf(a.y, b.y) - rb, fminf(a.z, b.z) - rb);
+            hi = mk3(fmaxf(a.x, b.x) + rb, fmaxf(a.y, b.y) + rb, fmaxf(a.z, b.z) + rb);
+        }
+    }
     L.sb[t] = lo.x;
     L.sb[B + t] = lo.y;
     L.sb[2 * B + t] = lo.z;
@@ -437,55 +458,138 @@ __device__ __forceinline__ void rc_boxes(const RcLds& L, int t, bool act, uint32
     }
 }
 
-// Stage 2: the lists of segment t over its group's local slots [g0, g1). err: set on a list overflow.
-template <int B, bool REV>
-__device__ __forceinline__ int rc_lists(const Params& P, const RcLds& L, int t, uint32_t fl, int g0, int g1,
-                                        uint16_t* fwd, uint16_t* rev, int& nrev, float& lim, int& arg, int* err) {
+// Skin-list build, stage 2: the partners of segment t over its group's local slots [g0, g1), ascending.
+template <int B>
+__device__ __forceinline__ void rc_skin_list(const RcLds& L, int t, uint32_t fl, int g0, int g1, uint16_t* skinRow) {
     constexpr int NCH = B / 16;
-    int cnt = 0, rc = 0;
-    lim = INFINITY;
-    arg = -1;
-    nrev = 0;
-    if (!(fl & HAS_NEXT)) return 0;
-    lim = L.pad[t] - P.detR;
-    f3 lo = mk3(L.sb[t], L.sb[B + t], L.sb[2 * B + t]), hi = mk3(L.sb[3 * B + t], L.sb[4 * B + t], L.sb[5 * B + t]);
-    const f3 a0 = ld3s(L.x, B, t);
-    const f3 a1 = ld3s(L.x, B, t + 1) - a0;
-    const float ri = fmaxf(L.pad[t], L.pad[t + 1]);
-    const int myRope = L.rope[t];
-    for (int ch = g0 >> 4; ch <= (g1 - 1) >> 4; ch++) {
-        if (!box_meets(L.cb, NCH, ch, lo, hi)) continue;
-        const int j1 = min(ch * 16 + 16, g1);
-        for (int j = max(ch * 16, g0); j < j1; j++) {
-            if (!(L.fl[j] & HAS_NEXT)) continue;
-            if (L.rope[j] == myRope && abs(j - t) <= 2) continue;
-            if (!box_meets(L.sb, B, j, lo, hi)) continue;
-            const float rj = fmaxf(L.pad[j], L.pad[j + 1]);
-            float d2;
-            if (pair_hit(a1, ld3s(L.x, B, j) - a0, ld3s(L.x, B, j + 1) - a0, ri + rj, d2)) {
-                if (cnt < MAX_COLL) {
-                    fwd[cnt] = (uint16_t)j;
-                    const float dj = 0.45f * sqrtf(d2);
-                    if (dj < lim) {
-                        lim = dj;
-                        arg = j;
-                    }
-                }
+    int cnt = 0;
+    if (fl & HAS_NEXT) {
+        const f3 lo = mk3(L.sb[t], L.sb[B + t], L.sb[2 * B + t]);
+        const f3 hi = mk3(L.sb[3 * B + t], L.sb[4 * B + t], L.sb[5 * B + t]);
+        const int myRope = L.rope[t];
+        for (int ch = g0 >> 4; ch <= (g1 - 1) >> 4; ch++) {
+            if (!box_meets(L.cb, NCH, ch, lo, hi)) continue;
+            const int j1 = min(ch * 16 + 16, g1);
+            for (int j = max(ch * 16, g0); j < j1; j++) {
+                if (!(L.fl[j] & HAS_NEXT)) continue;
+                if (L.rope[j] == myRope && abs(j - t) <= 2) continue;
+                if (!box_meets(L.sb, B, j, lo, hi)) continue;
+                if (cnt < SKIN_CAP) skinRow[cnt] = (uint16_t)j;
                 cnt++;
-            }
-            if (REV) {
-                const f3 c0 = ld3s(L.x, B, j);
-                float e2;
-                if (pair_hit(ld3s(L.x, B, j + 1) - c0, a0 - c0, ld3s(L.x, B, t + 1) - c0, rj + ri, e2)) {
-                    if (rc < MAX_COLL) rev[rc] = (uint16_t)j;
-                    rc++;
-                }
             }
         }
     }
+    L.vc[t] = cnt <= SKIN_CAP ? cnt : -1;
+}
+
+// The exact tests of segment t against candidate j (the forward's: does t list j), and with REV also from j's
+// side (does j list t). rt, rj: the segments' detection radii.
+__device__ __forceinline__ void rc_pair_tests(const RcLds& L, int B, int t, int j, float rt, float rj, f3 a0, f3 a1,
+                                              int& cnt, uint16_t* fwd, float& lim, int& arg, bool rev, int& rc,
+                                              uint16_t* revRow) {
+    float d2;
+    if (pair_hit(a1, ld3s(L.x, B, j) - a0, ld3s(L.x, B, j + 1) - a0, rt + rj, d2)) {
+        if (cnt < MAX_COLL) {
+            fwd[cnt] = (uint16_t)j;
+            const float dj = 0.45f * sqrtf(d2);
+            if (dj < lim) {
+                lim = dj;
+                arg = j;
+            }
+        }
+        cnt++;
+    }
+    if (rev) {
+        const f3 c0 = ld3s(L.x, B, j);
+        float e2;
+        if (pair_hit(ld3s(L.x, B, j + 1) - c0, a0 - c0, ld3s(L.x, B, t + 1) - c0, rj + rt, e2)) {
+            if (rc < MAX_COLL) revRow[rc] = (uint16_t)j;
+            rc++;
+        }
+    }
+}
+
+// The substep's lists of segment t: exact tests over its skin list (or over its whole group after a skin-list
+// overflow), candidates whose current boxes (padded by the detection radii) miss its own skipped first. Writes
+// the partner count, reverse count, limit and the limit's slot (-1: the padding) to LDS.
+template <int B, bool REV>
+__device__ __forceinline__ void rc_lists(const Params& P, const RcLds& L, int t, uint32_t fl, int g0, int g1,
+                                         uint16_t* row, int* err) {
+    uint16_t* fwd = row;
+    uint16_t* rev = row + MAX_COLL;
+    const uint16_t* skin = row + 2 * MAX_COLL;
+    int cnt = 0, rc = 0, arg = -1;
+    float lim = INFINITY;
+    if (fl & HAS_NEXT) {
+        lim = L.pad[t] - P.detR;
+        const f3 a0 = ld3s(L.x, B, t), x1 = ld3s(L.x, B, t + 1);
+        const f3 a1 = x1 - a0;
+        const float rt = fmaxf(L.pad[t], L.pad[t + 1]);
+        const float mg = 1e-4f + 2e-6f * fmaxf(fabsf(a0.x), fmaxf(fabsf(a0.y), fabsf(a0.z)));
+        const float rm = rt + mg;
+        const f3 lo = mk3(fminf(a0.x, x1.x) - rm, fminf(a0.y, x1.y) - rm, fminf(a0.z, x1.z) - rm);
+        const f3 hi = mk3(fmaxf(a0.x, x1.x) + rm, fmaxf(a0.y, x1.y) + rm, fmaxf(a0.z, x1.z) + rm);
+        const int vc = L.vc[t];
+        const int myRope = L.rope[t];
+        const int n = vc < 0 ? g1 - g0 : vc;
+        for (int e = 0; e < n; e++) {
+            const int j = vc < 0 ? g0 + e : (int)skin[e];
+            if (vc < 0 && (!(L.fl[j] & HAS_NEXT) || (L.rope[j] == myRope && abs(j - t) <= 2))) continue;
+            const f3 c0 = ld3s(L.x, B, j), c1 = ld3s(L.x, B, j + 1);
+            const float rj = fmaxf(L.pad[j], L.pad[j + 1]);
+            if (fminf(c0.x, c1.x) - rj > hi.x || fminf(c0.y, c1.y) - rj > hi.y || fminf(c0.z, c1.z) - rj > hi.z ||
+                fmaxf(c0.x, c1.x) + rj < lo.x || fmaxf(c0.y, c1.y) + rj < lo.y || fmaxf(c0.z, c1.z) + rj < lo.z)
+                continue;
+            rc_pair_tests(L, B, t, j, rt, rj, a0, a1, cnt, fwd, lim, arg, REV, rc, rev);
+        }
+    }
     if (cnt > MAX_COLL || rc > MAX_COLL) *err = ERR_MAX_COLLISIONS;
-    nrev = min(rc, MAX_COLL);
-    return min(cnt, MAX_COLL);
+    L.ncol[t] = min(cnt, MAX_COLL);
+    L.nrev[t] = min(rc, MAX_COLL);
+    L.lim[t] = lim;
+    L.arg[t] = arg;
+}
+
+// One substep's detection by every lane of the workgroup: the rebuild vote (a barrier; it also publishes what
+// the caller wrote before), a skin-list build when any slot voted or `first`, then the exact tests; ends with a
+// barrier after which every slot's lists, counts and limits are in place.
+template <int B, bool REV>
+__device__ __forceinline__ void rc_detect(const Params& P, const RcLds& L, int t, bool act, uint32_t fl, int g0, int g1,
+                                          uint16_t* row, int* err, bool first) {
+    const bool stale = first || rc_skin_stale(P, L, B, t, act, fl);
+    if (__syncthreads_or(stale)) {
+        rc_skin_boxes<B>(P, L, t, act, fl);
+        __syncthreads();
+        if (act) rc_skin_list<B>(L, t, fl, g0, g1, row + 2 * MAX_COLL);
+    }
+    if (act) rc_lists<B, REV>(P, L, t, fl, g0, g1, row, err);
+    else {
+        L.ncol[t] = 0;
+        L.nrev[t] = 0;
+        L.lim[t] = INFINITY;
+        L.arg[t] = -1;
+    }
+    __syncthreads();
+}
+
+// Rope-rope contact modes of the adjoint kernels. With rope contact a frame's contact groups split in two: those
+// in which some segment had a rope-rope pair at some substep (RC_PAIRS: detection, pair terms, the step limit's
+// pair distances) and the others (RC_CLAMP: their lists are empty all frame, so the step limit of a segment is
+// its padding less the detection radius and only the step clamp remains). Both kernels run over the same
+// workgroups; each workgroup keeps to the one its groups' flags name.
+enum { RC_NONE = 0, RC_CLAMP = 1, RC_PAIRS = 2 };
+__device__ __forceinline__ bool rc_wrong_class(const Params& P, const AdjDev& A, int RCM, int rb0, int rb1) {
+    if (RCM == RC_NONE) return false;
+    bool pairs = A.pairFlag == nullptr;
+    for (int g = rb0 / P.groupRopes; !pairs && g <= (rb1 - 1) / P.groupRopes; g++) pairs = A.pairFlag[g] != 0;
+    return pairs != (RCM == RC_PAIRS);
+}
+
+// Local slots [g0, g1) of the contact group of rope r in a workgroup whose first vertex is v0.
+__device__ __forceinline__ void group_range(const Params& P, const Dev& D, int r, int v0, int& g0, int& g1) {
+    const int G = P.groupRopes, g = r / G;
+    g0 = D.roff[min(g * G, P.nropes)] - v0;
+    g1 = D.roff[min((g + 1) * G, P.nropes)] - v0;
 }
 
 // History of one frame of the ropes of a launch. Local vertex li = i - A.hv0 of A.hn.
@@ -521,14 +625,16 @@ __device__ __forceinline__ Vc load_vc(const Dev& D, int i, bool act) {
 // the frame kernel (so the values, and every branch, are those of the recorded forward). Every
 // substep's inputs and every iteration's d^k, q^k, total H and Newton direction δ = H^-1 F go to the
 // history. The frame-end state is compared bit for bit with the next checkpoint.
-template <int B, bool F, bool RC>
+template <int B, bool F, int RCM>
 __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDev A, Drive Dr, int S) {
+    constexpr bool RC = RCM == RC_PAIRS;
     __shared__ float4 sQ[2][B];
     __shared__ float sX[3 * B], sD[2][3 * B], sF2[9 * B];
     extern __shared__ Pose sPoseAll[];  // the poses of the workgroup's ropes, sized at launch; then, with rope
                                         // contact, the detection arrays (rc_lds_bytes)
     const int I = P.numItr;
     const int rb0 = A.blk[blockIdx.x], rb1 = A.blk[blockIdx.x + 1], t = threadIdx.x;
+    if (rc_wrong_class(P, A, RCM, rb0, rb1)) return;
     const int v0 = D.roff[rb0], n = D.roff[rb1] - v0;
     const bool act = t < n;
     const int i = v0 + t, li = i - A.hv0;
@@ -541,17 +647,10 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
     if (P.numSdf > 0 && !P.sdfMoving)
         for (int m = t; m < nPose; m += B) sPoseAll[m] = A.fposes[(size_t)rb0 * P.numSdf + m];
     RcLds L{};
-    int g0 = 0, g1 = 0, ncol = 0;
-    float lim = INFINITY;
-    uint16_t* row = nullptr;
+    if (RCM != RC_NONE) L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), sX);
     if (RC) {
-        L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), sX);
         L.fl[t] = act ? (int)c.fl : 0;
         L.rope[t] = r;
-        const int G = P.groupRopes, gg = r / G;
-        g0 = D.roff[min(gg * G, P.nropes)] - v0;
-        g1 = D.roff[min((gg + 1) * G, P.nropes)] - v0;
-        row = A.lists + (size_t)(act ? li : 0) * 2 * MAX_COLL;
     }
 
     f3 x = mk3(0.f, 0.f, 0.f), v = x, vl = x;
@@ -599,19 +698,14 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
             st3s(sX, B, t, x);
             st3s(sD[0], B, t, d);
             sQ[0][t] = stq(q);
-            if (RC) L.pad[t] = padding;
+            if (RCM != RC_NONE) L.pad[t] = padding;
         }
-        __syncthreads();
-        if (RC) {  // rope-rope lists and step limits of the substep (the forward's detection)
-            rc_boxes<B>(L, t, act, c.fl);
+        if (RC) {  // rope-rope lists and step limits of the substep (the forward's detection); its vote is the barrier
+            int g0, g1;
+            group_range(P, D, r, v0, g0, g1);
+            rc_detect<B, false>(P, L, t, act, c.fl, g0, g1, A.lists + (size_t)(act ? li : 0) * LIST_STRIDE, D.err, s == 0);
+        } else {
             __syncthreads();
-            float ms = INFINITY;
-            int arg, nrev;
-            if (act) ncol = rc_lists<B, false>(P, L, t, c.fl, g0, g1, row, nullptr, nrev, ms, arg, D.err);
-            L.lim[t] = ms;
-            __syncthreads();
-            lim = ms;
-            if (act && (c.fl & HAS_PREV)) lim = fminf(lim, L.lim[t - 1]);
         }
         uint64_t hitBits = 0;
         for (int it = 0; it < I; it++) {
@@ -630,8 +724,10 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
                 vertex_terms<F, true>(P, D.grids, sPose, nc, cand, c.w, c.l, c.kS, c.fl, p0, d0, y,
                                       ldq(sQ[it & 1][t]), p1, p1dx, f, H, f2, H2, &hit);
                 if (it < 8) hitBits |= (uint64_t)(hit & 0xFFu) << (8 * it);
-                if (RC && ncol > 0) {  // rope-rope terms in list order, as the forward adds them
+                const int ncol = RC ? L.ncol[t] : 0;
+                if (ncol > 0) {  // rope-rope terms in list order, as the forward adds them
                     const float dmp = Ar<F>::div(P.damping, h);
+                    const uint16_t* row = A.lists + (size_t)li * LIST_STRIDE;
                     for (int e = 0; e < ncol; e++) {
                         const int j = row[e];
                         collision_term<F>(P, dmp, p0, p1, d0, p1dx, ld3s(sX, B, j), ld3s(sX, B, j + 1), ld3s(dk, B, j),
@@ -653,8 +749,16 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
                 }
                 f3 d = ld3s(dk, B, t), dl = mk3(0.f, 0.f, 0.f);
                 if (c.w != 0.f) {
+                    float lim = INFINITY;
+                    if (RC) {  // the step limit of the forward: min(limit_t, limit_t-1)
+                        lim = L.lim[t];
+                        if (c.fl & HAS_PREV) lim = fminf(lim, L.lim[t - 1]);
+                    } else if (RCM == RC_CLAMP) {  // no pairs: a segment's limit is its padding less detR
+                        if (c.fl & HAS_NEXT) lim = L.pad[t] - P.detR;
+                        if (c.fl & HAS_PREV) lim = fminf(lim, L.pad[t - 1] - P.detR);
+                    }
                     dl = V<F>::ssolve(H, f);
-                    d = solve_update<F>(P, d, H, f, RC, lim);
+                    d = solve_update<F>(P, d, H, f, RCM != RC_NONE, lim);
                 }
                 st3s(dn, B, t, d);
                 A.hD[hk(A, I, s, it + 1, li)] = st3(d);
@@ -727,13 +831,15 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
 // Pass 2: backwards through the frame of one rope from the history (no recompute of the iterations).
 // Per iteration three phases: orientation adjoint (pushes to the neighbours), Newton adjoint u = a H^-1 d̄,
 // segment-term adjoint (pushes to the next vertex).
-template <int B, bool F, bool RC>
+template <int B, bool F, int RCM>
 __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A, Drive Dr, int S) {
+    constexpr bool RC = RCM == RC_PAIRS;
     __shared__ float ex[EX_N * B];
     extern __shared__ Pose sPoseAll[];  // the poses of the workgroup's ropes, sized at launch; then, with rope
                                         // contact, the detection arrays (rc_lds_bytes)
     const int I = P.numItr;
     const int rb0 = A.blk[blockIdx.x], rb1 = A.blk[blockIdx.x + 1], t = threadIdx.x;
+    if (rc_wrong_class(P, A, RCM, rb0, rb1)) return;
     const int v0 = D.roff[rb0], n = D.roff[rb1] - v0;
     const bool act = t < n;
     const int i = v0 + t, li = i - A.hv0;
@@ -768,20 +874,13 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
     const bool hn = (c.fl & HAS_NEXT) != 0, hp = (c.fl & HAS_PREV) != 0;
     const q4 EX{1.f, 0.f, 0.f, 0.f};
     RcLds L{};
-    int g0 = 0, g1 = 0, ncol = 0;
-    uint16_t *row = nullptr, *rrow = nullptr;
+    if (RCM != RC_NONE) L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), nullptr);
     if (RC) {
-        L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), nullptr);
         L.fl[t] = act ? (int)c.fl : 0;
         L.rope[t] = r;
-        const int G = P.groupRopes, gg = r / G;
-        g0 = D.roff[min(gg * G, P.nropes)] - v0;
-        g1 = D.roff[min((gg + 1) * G, P.nropes)] - v0;
-        row = A.lists + (size_t)(act ? li : 0) * 2 * MAX_COLL;
-        rrow = row + MAX_COLL;
     }
-    // row of list entry e of slot k: the workgroup's vertices are consecutive local vertices
-    auto rev_row = [&](int k) { return A.lists + (size_t)(li - t + k) * 2 * MAX_COLL + MAX_COLL; };
+    // list rows of slot k (the workgroup's vertices are consecutive local vertices): partners, then reverse
+    auto list_row = [&](int k) { return A.lists + (size_t)(li - t + k) * LIST_STRIDE; };
 
     for (int s = S - 1; s >= 0; s--) {
         __syncthreads();  // sPose and the exchange are free
@@ -805,12 +904,14 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             hitMask = (uint64_t)mm.x | ((uint64_t)mm.y << 32);
         }
         // end update: x' = X + d, v' = (d / h)(1 - drag h) on free vertices
-        f3 aD = ax, aX = ax, aY = mk3(0.f, 0.f, 0.f), aYp = aY;  // aYp: y through the detection padding
-        float lim = INFINITY, mOwn = 0.f, mPrev = 0.f;
-        bool limPrev = false;
-        if (RC && act) {
-            st3s(L.x, B, t, X0);
-            L.pad[t] = P.scaledDetR + M::len(y);  // initItr's padding of this substep
+        f3 aD = ax, aX = ax, aY = mk3(0.f, 0.f, 0.f);
+        if (RCM != RC_NONE) {
+            L.mown[t] = 0.f;  // adjoints of this substep's step limits (own segment's, previous segment's)
+            L.mprev[t] = 0.f;
+            if (act) {
+                if (RC) st3s(L.x, B, t, X0);
+                L.pad[t] = P.scaledDetR + M::len(y);  // initItr's padding of this substep
+            }
         }
         q4 aQ = aq;
         if (act && c.w != 0.f) {
@@ -819,22 +920,12 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             aD = aD + (invH * (1.f - P.drag * h)) * av;
             gdrag -= (double)h * (double)dot(d * invH, av);
         }
-        __syncthreads();  // sPose of a moving frame is in place
         if (RC) {  // the substep's rope-rope lists and step limits, with the lists of the segments listing each one
-            rc_boxes<B>(L, t, act, c.fl);
-            __syncthreads();
-            float ms = INFINITY;
-            int arg = -1, nrev = 0;
-            if (act) ncol = rc_lists<B, true>(P, L, t, c.fl, g0, g1, row, rrow, nrev, ms, arg, D.err);
-            L.lim[t] = ms;
-            L.arg[t] = arg;
-            L.nrev[t] = nrev;
-            __syncthreads();
-            lim = ms;
-            if (act && hp && L.lim[t - 1] < ms) {
-                lim = L.lim[t - 1];
-                limPrev = true;
-            }
+            int g0, g1;
+            group_range(P, D, r, v0, g0, g1);
+            rc_detect<B, true>(P, L, t, act, c.fl, g0, g1, list_row(act ? t : 0), D.err, s == S - 1);
+        } else {
+            __syncthreads();  // sPose of a moving frame is in place
         }
 
         // History of iteration k: d^{k+1} of vertex t and t+1 (the previous iteration's d^k, already loaded),
@@ -877,8 +968,10 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             cur = load_iter(I - 1, dI0, dI1);
         }
         for (int k = I - 1; k >= 0; k--) {
+            // without rope contact, iteration k-1's history is loaded here and lands while k computes; with it, the
+            // registers the early loads would hold go to the rope-rope terms, and each iteration loads its own
             IterHist nxt = cur;
-            if (k > 0) nxt = load_iter(k - 1, cur.d0, cur.d1);
+            if (!RC && k > 0) nxt = load_iter(k - 1, cur.d0, cur.d1);
             const f3 dn0 = cur.dn0, dn1 = cur.dn1, d0 = cur.d0, d1 = cur.d1;
             const q4 q0 = cur.q0, qp = cur.qp, qn = cur.qn;
             const s3 H = cur.H;
@@ -963,17 +1056,20 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                     grho[2] += gr.z;
                     grho[3] += gr.w;
                 }
-                if (RC && c.w != 0.f) {  // the step clamp d' = z lim / |z| of the forward, z = d^k + a H^-1 F
+                if (RCM != RC_NONE && c.w != 0.f) {  // the step clamp d' = z lim / |z| of the forward, z = d^k + a H^-1 F
+                    const float lo = RC ? L.lim[t] : (hn ? L.pad[t] - P.detR : INFINITY);
+                    const float lp = !hp ? INFINITY : RC ? L.lim[t - 1] : L.pad[t - 1] - P.detR;
+                    const float lim = lp < lo ? lp : lo;  // fminf(limit_t, limit_t-1), and which one it is
                     const f3 z = M::axpy(P.accel, dl, d0);
                     const float lz = M::len(z);
                     if (lz > lim && lz > 0.f) {
                         const f3 zh = z * (1.f / lz);
                         const float limb = dot(zh, aD);
                         aD = (lim / lz) * (aD - limb * zh);
-                        if (limPrev)
-                            mPrev += limb;
+                        if (lp < lo)
+                            L.mprev[t] += limb;
                         else
-                            mOwn += limb;
+                            L.mown[t] += limb;
                     }
                 }
                 if (c.w != 0.f) u = P.accel * M::ssolve(H, aD);
@@ -1024,9 +1120,11 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                     const f3 zero = mk3(0.f, 0.f, 0.f);
                     const f3 u2 = hn ? ld3s(ex + EX_U * B, B, t + 1) : zero;
                     const f3 dl2 = hn ? ld3s(ex + (EX_U + 3) * B, B, t + 1) : zero;
+                    const int ncol = L.ncol[t];
+                    const uint16_t* row = list_row(t);
                     for (int e = 0; e < ncol; e++) {
                         const int j = row[e];
-                        pair_term_adj<F>(P, dmp, X0, X1, d0, d1, ld3s(L.x, B, j), ld3s(L.x, B, j + 1), ld3s(L.dk, B, j),
+                        pair_term_adj<F>(pair_mat(P), dmp, X0, X1, d0, d1, ld3s(L.x, B, j), ld3s(L.x, B, j + 1), ld3s(L.dk, B, j),
                                          ld3s(L.dk, B, j + 1), u, dl, u2, dl2, gq);
                         aX = aX + gq[0];
                         aD = aD + gq[2];
@@ -1037,10 +1135,10 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                         const int jt = t - side;  // the partner segment (jt, jt + 1) holds this vertex at `side`
                         if (side == 1 && !hp) break;
                         const int nr = L.nrev[jt];
-                        const uint16_t* rr = rev_row(jt);
+                        const uint16_t* rr = list_row(jt) + MAX_COLL;
                         for (int e = 0; e < nr; e++) {
                             const int k = rr[e];
-                            pair_term_adj<F>(P, dmp, ld3s(L.x, B, k), ld3s(L.x, B, k + 1), ld3s(L.dk, B, k),
+                            pair_term_adj<F>(pair_mat(P), dmp, ld3s(L.x, B, k), ld3s(L.x, B, k + 1), ld3s(L.dk, B, k),
                                              ld3s(L.dk, B, k + 1), ld3s(L.x, B, jt), ld3s(L.x, B, jt + 1),
                                              ld3s(L.dk, B, jt), ld3s(L.dk, B, jt + 1), ld3s(ex + EX_U * B, B, k),
                                              ld3s(ex + (EX_U + 3) * B, B, k), ld3s(ex + EX_U * B, B, k + 1),
@@ -1054,31 +1152,34 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                 st3s(ex + (EX_P + 3) * B, B, t, pX);
             }
             __syncthreads();
-            cur = nxt;
+            if (RC) {
+                if (k > 0) cur = load_iter(k - 1, d0, d1);
+            } else {
+                cur = nxt;
+            }
         }
         if (act && hp) {  // segment-term pushes of iteration 0
             aD = aD + ld3s(ex + EX_P * B, B, t - 1);
             aX = aX + ld3s(ex + (EX_P + 3) * B, B, t - 1);
         }
-        if (RC) {  // step-limit adjoints: limit_t = min(pad_t - detR, 0.45 |d_tj| over the listed j)
-            L.mb[t] = mPrev;  // adjoint of the previous segment's limit
-            __syncthreads();
-            const float Mt = mOwn + ((act && hn) ? L.mb[t + 1] : 0.f);
-            __syncthreads();
+        f3 aYp = mk3(0.f, 0.f, 0.f);  // the adjoint of y through the detection padding
+        if (RCM != RC_NONE) {  // step-limit adjoints: limit_t = min(pad_t - detR, 0.45 |d_tj| over the listed j)
+            __syncthreads();  // every clamp adjoint of the substep is in L.mown / L.mprev
+            const float Mt = L.mown[t] + ((act && hn) ? L.mprev[t + 1] : 0.f);
             L.mb[t] = Mt;
             __syncthreads();
             if (act) {
                 f3 g4[4];
                 if (Mt != 0.f) {
-                    const int a = L.arg[t];
+                    const int a = RC ? L.arg[t] : -1;
                     if (a < 0) {
-                        aYp = aYp + (Mt / len(y)) * y;
+                        aYp = (Mt / len(y)) * y;
                     } else {
                         limit_dist_adj(X0, X1, ld3s(L.x, B, a), ld3s(L.x, B, a + 1), Mt, g4);
                         aX = aX + g4[0];
                     }
                 }
-                if (hp) {
+                if (RC && hp) {
                     const float Mp = L.mb[t - 1];
                     const int a = L.arg[t - 1];
                     if (Mp != 0.f && a >= 0) {
@@ -1086,11 +1187,11 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                         aX = aX + g4[1];
                     }
                 }
-                for (int side = 0; side < 2; side++) {
+                for (int side = 0; RC && side < 2; side++) {
                     const int jt = t - side;
                     if (side == 1 && !hp) break;
                     const int nr = L.nrev[jt];
-                    const uint16_t* rr = rev_row(jt);
+                    const uint16_t* rr = list_row(jt) + MAX_COLL;
                     for (int e = 0; e < nr; e++) {
                         const int k = rr[e];
                         const float Mk = L.mb[k];
@@ -1106,7 +1207,7 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
         // init: d^0 = h v (+ h^2 s g), y = h v + h^2 g, last velocity <- v
         if (act) {
             f3 vb = h * aD + avl, vlb = mk3(0.f, 0.f, 0.f);
-            if (RC) vb = vb + h * aYp;
+            if (RCM != RC_NONE) vb = vb + h * aYp;
             if (c.w != 0.f) {
                 vb = vb + h * aY;
                 const f3 g = mk3(P.gx, P.gy, P.gz);
@@ -1207,8 +1308,10 @@ __global__ void k_adjoint_finish(Params P, Dev D, const float4* rhoUnit, const d
 }  // namespace
 
 size_t adjoint_rc_lds_bytes(int block) {
-    return sizeof(float) * ((size_t)15 * block + 6 * (block / 16)) + sizeof(int) * 4 * (size_t)block;
+    return sizeof(float) * ((size_t)22 * block + 6 * (block / 16)) + sizeof(int) * 6 * (size_t)block;
 }
+
+size_t adjoint_list_entries_per_vertex() { return LIST_STRIDE; }
 
 size_t adjoint_history_bytes(int numItr) {
     return 5 * sizeof(float4) + 2 * sizeof(uint2) + (size_t)(numItr + 1) * 2 * sizeof(float4) +
@@ -1228,7 +1331,7 @@ int waves_of(const void* fn) {
     const int regs = (a.numRegs + 7) / 8 * 8;
     return std::max(1, std::min(8, 512 / regs));
 }
-template <int B, bool RC>
+template <int B, int RC>
 const void* pass_fn(bool reverse, bool fast) {
     if (reverse)
         return fast ? reinterpret_cast<const void*>(&k_adjoint_reverse<B, true, RC>)
@@ -1238,7 +1341,7 @@ const void* pass_fn(bool reverse, bool fast) {
 }
 template <int B>
 const void* pass_fn(bool reverse, bool fast, bool rc) {
-    return rc ? pass_fn<B, true>(reverse, fast) : pass_fn<B, false>(reverse, fast);
+    return rc ? pass_fn<B, RC_PAIRS>(reverse, fast) : pass_fn<B, RC_NONE>(reverse, fast);
 }
 }  // namespace
 
@@ -1254,14 +1357,14 @@ int adjoint_waves_per_simd(int block, bool reverse, bool fast, bool ropeContact)
     }
 }
 
-template <int B, bool F, bool RC>
+template <int B, bool F, int RC>
 static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
                                 int nblocks, size_t lds, hipStream_t s) {
     // dynamic LDS past the default limit needs the kernel's attribute raised once
     static bool attr[2] = {false, false};
     const void* fn = pass_fn<B, RC>(reverse, F);
     if (!attr[reverse]) {
-        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 104 * 1024);
         if (e != hipSuccess) return e;
         attr[reverse] = true;
     }
@@ -1273,29 +1376,35 @@ static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, con
 }
 template <int B>
 static hipError_t launch_pass_t(bool reverse, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
-                                int nblocks, size_t lds, hipStream_t s) {
-    const bool rc = P.ropeContact != 0;
-    if (P.fastMath)
-        return rc ? launch_pass_f<B, true, true>(reverse, P, D, A, Dr, S, nblocks, lds, s)
-                  : launch_pass_f<B, true, false>(reverse, P, D, A, Dr, S, nblocks, lds, s);
-    return rc ? launch_pass_f<B, false, true>(reverse, P, D, A, Dr, S, nblocks, lds, s)
-              : launch_pass_f<B, false, false>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+                                int nblocks, size_t lds, hipStream_t s, int rcClass) {
+    if (!P.ropeContact) {
+        if (P.fastMath) return launch_pass_f<B, true, RC_NONE>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+        return launch_pass_f<B, false, RC_NONE>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+    }
+    // the clamp-only kernel and the full one over the same workgroups; each workgroup runs in one of them
+    if (rcClass != RC_PAIRS) {
+        const hipError_t e = P.fastMath ? launch_pass_f<B, true, RC_CLAMP>(reverse, P, D, A, Dr, S, nblocks, lds, s)
+                                        : launch_pass_f<B, false, RC_CLAMP>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+        if (e != hipSuccess || rcClass == RC_CLAMP) return e;
+    }
+    return P.fastMath ? launch_pass_f<B, true, RC_PAIRS>(reverse, P, D, A, Dr, S, nblocks, lds, s)
+                      : launch_pass_f<B, false, RC_PAIRS>(reverse, P, D, A, Dr, S, nblocks, lds, s);
 }
 
 hipError_t launch_adjoint_pass(bool reverse, int block, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr,
-                               int S, int nblocks, int ropesPerBlock, hipStream_t s) {
+                               int S, int nblocks, int ropesPerBlock, hipStream_t s, int rcClass) {
     if (nblocks <= 0) return hipSuccess;
     if (!adjoint_block_supported(block, reverse) || ropesPerBlock < 1 ||
         (size_t)ropesPerBlock * P.numSdf > (size_t)adjoint_pose_slots(block) || (P.ropeContact && !A.lists))
         return hipErrorInvalidValue;
     const size_t lds = (size_t)ropesPerBlock * P.numSdf * sizeof(Pose) + (P.ropeContact ? adjoint_rc_lds_bytes(block) : 0);
     switch (block) {
-        case 64: return launch_pass_t<64>(reverse, P, D, A, Dr, S, nblocks, lds, s);
-        case 128: return launch_pass_t<128>(reverse, P, D, A, Dr, S, nblocks, lds, s);
-        case 192: return launch_pass_t<192>(reverse, P, D, A, Dr, S, nblocks, lds, s);
-        case 256: return launch_pass_t<256>(reverse, P, D, A, Dr, S, nblocks, lds, s);
-        case 384: return launch_pass_t<384>(reverse, P, D, A, Dr, S, nblocks, lds, s);
-        case 512: return launch_pass_t<512>(reverse, P, D, A, Dr, S, nblocks, lds, s);
+        case 64: return launch_pass_t<64>(reverse, P, D, A, Dr, S, nblocks, lds, s, rcClass);
+        case 128: return launch_pass_t<128>(reverse, P, D, A, Dr, S, nblocks, lds, s, rcClass);
+        case 192: return launch_pass_t<192>(reverse, P, D, A, Dr, S, nblocks, lds, s, rcClass);
+        case 256: return launch_pass_t<256>(reverse, P, D, A, Dr, S, nblocks, lds, s, rcClass);
+        case 384: return launch_pass_t<384>(reverse, P, D, A, Dr, S, nblocks, lds, s, rcClass);
+        case 512: return launch_pass_t<512>(reverse, P, D, A, Dr, S, nblocks, lds, s, rcClass);
         default: return hipErrorInvalidValue;
     }
 }

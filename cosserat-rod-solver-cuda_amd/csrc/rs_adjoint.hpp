@@ -44,10 +44,13 @@ struct AdjDev {
     int* mismatch;  // count of vertices whose recomputed end state differs from the next checkpoint
     const Pose* fposes;     // frame-start poses [N][numSdf]
     const Motion* fmotion;  // frame motion [N][numSdf] while meshes move, else null
-    // with rope-rope contact, per local vertex: the substep's partner list (workgroup-local slots, ascending) and
-    // the list of the segments whose lists hold it, MAX_COLL entries each ([hn][2][MAX_COLL]); a pass rebuilds
-    // them at every substep
+    // with rope-rope contact, per local vertex (adjoint_list_entries_per_vertex entries): the substep's partner
+    // list (workgroup-local slots, ascending), the list of the segments whose lists hold it (MAX_COLL each) and
+    // the skin list its candidates come from; each pass rebuilds them
     uint16_t* lists;
+    // with rope-rope contact, per contact group of this frame: nonzero when some segment of the group had a
+    // rope-rope pair at some substep of the recorded forward (null: treat every group so)
+    const int* pairFlag;
 };
 
 // Longest rope (or rope contact group) the backward kernels support; pose slots a workgroup keeps in LDS (its
@@ -60,12 +63,16 @@ bool adjoint_block_supported(int block, bool reverse);  // 64, 128, 192, 256, 38
 int adjoint_waves_per_simd(int block, bool reverse, bool fast, bool ropeContact);
 // Dynamic LDS of the rope-rope detection arrays of one workgroup (beside its pose slots).
 size_t adjoint_rc_lds_bytes(int block);
+// Entries (uint16) of AdjDev::lists per local vertex: partner list, reverse list, skin list.
+size_t adjoint_list_entries_per_vertex();
 // One pass (forward recompute, or reverse) over `nblocks` workgroups of `block` threads, workgroup b on the
 // ropes [A.blk[b], A.blk[b + 1]), at most ropesPerBlock of them (their mesh poses take ropesPerBlock * numSdf
 // LDS slots, at most adjoint_pose_slots(block)). The two passes of a frame may use different packings: they
 // share only the per-vertex history.
+// With rope contact, rcClass picks the workgroups' class: 1 pair-free groups (the clamp-only kernel), 2 groups
+// with rope-rope pairs (the full kernel), 0 both one after the other. Without rope contact it is ignored.
 hipError_t launch_adjoint_pass(bool reverse, int block, const Params& P, const Dev& D, const AdjDev& A,
-                               const Drive& Dr, int S, int nblocks, int ropesPerBlock, hipStream_t s);
+                               const Drive& Dr, int S, int nblocks, int ropesPerBlock, hipStream_t s, int rcClass = 0);
 hipError_t launch_adjoint_finish(const Params& P, const Dev& D, const float4* rhoUnit, const double* gk,
                                  const double* grho, const double* gm, float* gea, float* gei, float* gmass,
                                  hipStream_t s);

@@ -164,6 +164,8 @@ struct rs_batch {
         bool on = false, closed = false;
         int maxFrames = 0, frames = 0, numSdf = 0, maxS = 0;
         int unitRopes = 1;         // ropes per adjoint unit (adjoint_unit_ropes at the first recorded frame)
+        int* pairFlags = nullptr;  // [maxFrames][groups] each frame's per-group rope-rope pair flags (rope contact)
+        int groups = 0;
         float4* ck = nullptr;      // [(maxFrames + 1)][4][nv]: x, v, v_last, q at each frame start (+ the end)
         float* prog = nullptr;     // [maxFrames][17 N] staged root programs
         Pose* poses = nullptr;     // [maxFrames][N][numSdf]
@@ -203,7 +205,9 @@ struct rs_batch {
     int* d_mismatch = nullptr;
     int* d_blk = nullptr;  // the adjoint's workgroup -> rope ranges
     size_t blkCap = 0;
-    uint16_t* d_adjLists = nullptr;  // the adjoint's rope-rope lists ([chunk vertices][2][MAX_COLL])
+    uint16_t* d_adjLists = nullptr;  // the adjoint's rope-rope lists (adjoint_list_entries_per_vertex per vertex)
+    hipStream_t adjStream = nullptr;  // the adjoint's second stream (groups with rope-rope pairs), default priority
+    hipEvent_t adjFork = nullptr, adjJoin = nullptr;
     size_t adjListsCap = 0;
 
     ~rs_batch() {
@@ -217,12 +221,15 @@ struct rs_batch {
                         D.ncol, D.coll, D.vlist, D.clist, D.poses, (void*)D.motion, D.ncand, D.cand, D.force, D.wrench, D.wrenchFx, D.outPos,
                         D.outVel, D.outQuat, D.err, (void*)D.rootTraj, (void*)D.posesS, (void*)D.cellMap, d_initX, d_initQ, d_ids, d_prog, d_cells, d_order, d_cellTime, d_cellQueue, d_hits,
                         d_grids, d_cur, d_tgt, d_stage, d_stageTele, d_rhoUnit, tape.ck, tape.prog,
-                        tape.poses, tape.motion, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
-                        d_gdrag, d_mismatch, d_blk, d_adjLists};
+                        tape.poses, tape.motion, tape.pairFlags, d_adj, d_hist, d_gk, d_gm, d_gp, d_grho,
+                        d_gdrag, d_mismatch, d_blk, d_adjLists, D.pairFlag};
         for (void* p : ptrs)
             if (p) hipFree(p);
         for (float* g : gridMem) hipFree(g);
         bvh_free(bvh);
+        if (adjStream) hipStreamDestroy(adjStream);
+        if (adjFork) hipEventDestroy(adjFork);
+        if (adjJoin) hipEventDestroy(adjJoin);
         if (stream) hipStreamDestroy(stream);
     }
 
@@ -737,6 +744,8 @@ struct rs_batch {
     }
     // The frame's kernel sequence: per-substep launches (var < 0) or the persistent frame kernel variant var.
     void enqueue_frame(const Params& P, const Drive& dr, int S, int var, const Bvh* tree) {
+        if (P.ropeContact)  // which contact groups have rope-rope pairs this frame (read by the adjoint)
+            hcheck(launch_fill_i32(D.pairFlag, 0, (N + groupRopes - 1) / groupRopes, stream), "clear pair flags");
         if (var < 0) {
             if (dr.mode == DRIVE_CONST) hcheck(launch_root_const(P, D, dr, stream), "root drive");
             for (int s = 0; s < S; s++) hcheck(launch_plain_substep(P, D, dr, s, S, stream, tree), "substep");
@@ -826,6 +835,10 @@ struct rs_batch {
         if (tape.on) tape_record(dr, S);
         stepped = true;
         run_substeps(dr, S);
+        if (tape.on && tape.pairFlags && ropeContact && tape.groups == (N + groupRopes - 1) / groupRopes)
+            hcheck(hipMemcpyAsync(tape.pairFlags + (size_t)(tape.frames - 1) * tape.groups, D.pairFlag,
+                                  tape.groups * sizeof(int), hipMemcpyDeviceToDevice, stream),
+                   "record pair flags");
         time = time + dt;
         check_errors();
         end_step(S);
@@ -902,7 +915,7 @@ struct rs_batch {
         tape.closed = true;
     }
     void tape_free() {
-        for (void* p : {(void*)tape.ck, (void*)tape.prog, (void*)tape.poses, (void*)tape.motion})
+        for (void* p : {(void*)tape.ck, (void*)tape.prog, (void*)tape.poses, (void*)tape.motion, (void*)tape.pairFlags})
             if (p) hipFree(p);
         tape = Tape{};
     }
@@ -1251,6 +1264,7 @@ int rs_set_rope_contact(rs_batch* b, int enabled, float radius, float barrier, f
             if (2.f * b->minSegLen < 2.f * radius + barrier)
                 throw Invalid("rope contact needs 2 * radius + barrier <= 2 * shortest segment");
             if (!b->D.coll) b->D.coll = dalloc<int>((size_t)b->nv * MAX_COLL, "collision lists");
+            if (!b->D.pairFlag) b->D.pairFlag = dalloc<int>(b->N, "pair flags");
             b->groupRopes = group_ropes;
             b->radius = radius;
             b->barrier = barrier;
@@ -1728,6 +1742,12 @@ int rs_tape_begin(rs_batch* b, int max_frames) {
         b->tape.maxFrames = max_frames;
         b->tape.ck = dalloc<float4>((size_t)(max_frames + 1) * 4 * n, "tape checkpoints");
         b->tape.prog = dalloc<float>((size_t)max_frames * 17 * b->N, "tape programs");
+        if (b->ropeContact) {
+            b->tape.groups = (b->N + b->groupRopes - 1) / b->groupRopes;
+            b->tape.pairFlags = dalloc<int>((size_t)max_frames * b->tape.groups, "tape pair flags");
+            // a frame whose flags are not recorded counts every group as having pairs (the full adjoint)
+            hcheck(launch_fill_i32(b->tape.pairFlags, 1, (long)max_frames * b->tape.groups, b->stream), "pair flags");
+        }
         if (b->tape.numSdf > 0) {
             const size_t slots = (size_t)max_frames * b->N * b->tape.numSdf;
             b->tape.poses = dalloc<Pose>(slots, "tape poses");
@@ -1872,10 +1892,16 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
             need = std::max(need, (size_t)c.nv * perV);
             maxChunkV = std::max(maxChunkV, (size_t)c.nv);
         }
-        if (anyRC && maxChunkV * 2 * MAX_COLL > b->adjListsCap) {
+        if (anyRC && !b->adjStream) {
+            hcheck(hipStreamCreate(&b->adjStream), "adjoint stream");
+            hcheck(hipEventCreateWithFlags(&b->adjFork, hipEventDisableTiming), "adjoint event");
+            hcheck(hipEventCreateWithFlags(&b->adjJoin, hipEventDisableTiming), "adjoint event");
+        }
+        const size_t listEntries = maxChunkV * adjoint_list_entries_per_vertex();
+        if (anyRC && listEntries > b->adjListsCap) {
             if (b->d_adjLists) hipFree(b->d_adjLists);
-            b->d_adjLists = dalloc<uint16_t>(maxChunkV * 2 * MAX_COLL, "adjoint rope-rope lists");
-            b->adjListsCap = maxChunkV * 2 * MAX_COLL;
+            b->d_adjLists = dalloc<uint16_t>(listEntries, "adjoint rope-rope lists");
+            b->adjListsCap = listEntries;
         }
         if (need > b->histBytes) {
             if (b->d_hist) hipFree(b->d_hist);
@@ -1950,11 +1976,30 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
                 A.fposes = M > 0 ? T.poses + (size_t)f * N * M : nullptr;
                 A.fmotion = (M > 0 && T.P[f].sdfMoving) ? T.motion + (size_t)f * N * M : nullptr;
                 A.lists = anyRC ? b->d_adjLists : nullptr;
-                for (int pass = 0; pass < 2; pass++) {
-                    A.blk = b->d_blk + chunkBlk0[pass][ci];
-                    hcheck(launch_adjoint_pass(pass == 1, pk[pass].blk, T.P[f], b->D, A, dr, S, chunkBlks[pass][ci],
-                                               ropesPerBlock[pass], st),
-                           pass ? "adjoint reverse pass" : "adjoint forward pass");
+                A.pairFlag = T.pairFlags ? T.pairFlags + (size_t)f * T.groups : nullptr;
+                // With rope contact the groups with rope-rope pairs run their two passes on a second stream while
+                // the pair-free groups run theirs here (groups never interact): the pair groups' kernels hold few,
+                // long workgroups, which would leave most compute units idle on their own.
+                const bool split = T.P[f].ropeContact != 0;
+                if (split) {
+                    hcheck(hipEventRecord(b->adjFork, st), "adjoint fork");
+                    hcheck(hipStreamWaitEvent(b->adjStream, b->adjFork, 0), "adjoint fork");
+                }
+                // the pair groups' (long) workgroups are issued first
+                const int classes[2] = {split ? 2 : 0, 1};
+                for (int ci2 = 0; ci2 < (split ? 2 : 1); ci2++) {
+                    const int cls = classes[ci2];
+                    hipStream_t sc = cls == 2 ? b->adjStream : st;
+                    for (int pass = 0; pass < 2; pass++) {
+                        A.blk = b->d_blk + chunkBlk0[pass][ci];
+                        hcheck(launch_adjoint_pass(pass == 1, pk[pass].blk, T.P[f], b->D, A, dr, S, chunkBlks[pass][ci],
+                                                   ropesPerBlock[pass], sc, cls),
+                               pass ? "adjoint reverse pass" : "adjoint forward pass");
+                    }
+                }
+                if (split) {
+                    hcheck(hipEventRecord(b->adjJoin, b->adjStream), "adjoint join");
+                    hcheck(hipStreamWaitEvent(st, b->adjJoin, 0), "adjoint join");
                 }
             }
         }

@@ -166,6 +166,7 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void k_pos(Params P, Dev D, const floa
         if (LIMIT && (fl & HAS_NEXT)) {
             const float dmp = Ar<F>::div(P.damping, P.h);
             const int n = D.ncol[i];
+            if (n > 0) D.pairFlag[D.ropeOf[i] / P.groupRopes] = 1;
             for (int c = 0; c < n; c++) {
                 const int j = D.coll[(size_t)i * MAX_COLL + c];
                 collision_term<F>(P, dmp, p0, p1, d0, p1dx, ld3(D.x[j]), ld3(D.x[j + 1]), ld3(dxIn[j]), ld3(dxIn[j + 1]),
@@ -826,6 +827,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
                         }
                         if (cnt > MAX_COLL) D.err[0] = ERR_MAX_COLLISIONS;
                         ncol[k] = min(cnt, MAX_COLL);
+                        if (cnt > 0) D.pairFlag[rope[k] / P.groupRopes] = 1;
                     }
                     minD[k] = minDist;
                 }

@@ -105,6 +105,9 @@ struct Dev {
     const int* cellMap;      // [cells * cell capacity] vertex of each cell slot (-1 past the cell's end)
     unsigned* cellTime;      // [cells] each cell's duration in its last frame (100 MHz ticks), or null
     int* cellQueue;          // [1] next unclaimed position of the cell launch order (cleared before each frame)
+    int* pairFlag;           // [contact groups] set when a segment of the group has a rope-rope pair at some substep
+                             // of the frame (cleared before each frame while rope contact is on); the adjoint reads
+                             // it to run pair-free groups without the pair machinery
 };
 
 // What drives the roots during one frame.

@@ -98,7 +98,9 @@ def _ssolve(Hp, f):
 def seg_closest(a0, a1, b0, b1):
     """Closest-point parameters (u on a0-a1, v on b0-b1) by the reference's three steps (KittenEngine
     segmentClosestPoints as restated in rope_oracle.hpp): the infinite-line v, clamped; u from it, clamped;
-    v again from u, clamped. A singular 2x2 system takes v = 0.5."""
+    v again from u, clamped. A singular 2x2 system takes v = 0.5. The infinite-line step of nearly parallel
+    segments (a c - b^2 <= 1e-4 a c) passes no gradient: float32 cannot resolve that v, and its derivative grows
+    as 1 / sin^2 of the angle; the device adjoint holds it fixed as well."""
     A0, A1 = a1 - a0, b1 - b0
     a, b, c = _dot(A0, A0), _dot(A0, A1), _dot(A1, A1)
     diff = b0 - a0
@@ -108,6 +110,7 @@ def seg_closest(a0, a1, b0, b1):
     y = num / torch.where(ok, den, torch.ones_like(den))
     ok = ok & torch.isfinite(y)
     y = torch.where(ok, y, torch.full_like(y, 0.5))
+    y = torch.where(den > 1e-4 * (a * c), y, y.detach())
     y = torch.clamp(y, 0.0, 1.0)
     x = torch.clamp(_dot((b0 + y * A1) - a0, A0) / a, 0.0, 1.0)
     y = torch.clamp(_dot((a0 + x * A0) - b0, A1) / c, 0.0, 1.0)
@@ -308,6 +311,7 @@ class TorchRope:
             a0 = X[ti]
             a1, b0, b1 = X[ti + 1] - a0, X[tj] - a0, X[tj + 1] - a0
             u, v = seg_closest(torch.zeros_like(a1), a1, b0, b1)
+            u, v = u.detach(), v.detach()  # the distance is a minimum over (u, v): no term through them
             dv = u * a1 - (b0 + v * (b1 - b0))
             dist = 0.45 * torch.sqrt(_dot(dv, dv))
             ms = ms.scatter_reduce(0, ti.unsqueeze(-1), dist, reduce="amin", include_self=True)
