@@ -231,7 +231,7 @@ __device__ __forceinline__ PairMat pair_mat(const Params& P) {
 }
 
 template <bool F>
-__device__ __noinline__ void pair_term_adj(PairMat P, float dmp, f3 p0, f3 p1, f3 d0, f3 p1dx, f3 b0, f3 b1, f3 db0,
+__device__ __forceinline__ void pair_term_adj(PairMat P, float dmp, f3 p0, f3 p1, f3 d0, f3 p1dx, f3 b0, f3 b1, f3 db0,
                                            f3 db1, f3 u1, f3 l1, f3 u2, f3 l2, f3* g) {
     using Af = Ar<F>;
     using M = V<F>;
@@ -358,14 +358,24 @@ __device__ __noinline__ void limit_dist_adj(f3 xa0, f3 xa1, f3 xb0, f3 xb1, floa
 // exact test can accept is on those lists (two segments approach by at most twice the skin). A list longer than
 // SKIN_CAP falls back to a scan of the whole contact group. Lists hold workgroup-local slots.
 constexpr int SKIN_CAP = 64;
-constexpr int LIST_STRIDE = 2 * MAX_COLL + SKIN_CAP;  // per local vertex: partner list, reverse list, skin list
+// per local vertex: partner list, reverse list, skin list, and (reverse pass) the pair index of each reverse-list
+// entry
+constexpr int REV_IDX = 2 * MAX_COLL + SKIN_CAP;
+constexpr int LIST_STRIDE = 3 * MAX_COLL + SKIN_CAP;
 
 struct RcLds {
     float *x, *pad, *lim, *x0, *R, *sb, *cb;  // positions [3][B], paddings [B], limits [B], build positions [3][B],
                                               // build radii [B], build boxes [6][B], chunk boxes [6][B/16]
     float *dk, *mb, *mown, *mprev;            // (reverse pass) iterate d^k [3][B], step-limit adjoints [B] x 3
     int *fl, *rope, *arg, *nrev, *ncol, *vc;  // flags, rope, limit argument, list lengths, skin-list length (-1: scan)
+    // (reverse pass with pairs) first pair index of each slot [B + 1] (then 8 wave sums and the chunk count), and
+    // the pair mailbox [PAIR_ADJ_N][slots]
+    int* poff;
+    float* mbx;
 };
+// Floats a pair's adjoint leaves in the mailbox: the adjoints of its 8 inputs (pair_term_adj).
+constexpr int PAIR_ADJ_N = 24;
+constexpr int POFF_N(int B) { return B + 16; }
 
 // The detection arrays in dynamic LDS at `base` (after the mesh pose slots); x: positions already in LDS, or
 // null for a region of their own.
@@ -396,6 +406,8 @@ __device__ __forceinline__ RcLds rc_lds(char* base, float* x) {
     L.nrev = q + 3 * B;
     L.ncol = q + 4 * B;
     L.vc = q + 5 * B;
+    L.poff = q + 6 * B;
+    L.mbx = reinterpret_cast<float*>(L.poff + POFF_N(B));
     return L;
 }
 
@@ -570,6 +582,122 @@ __device__ __forceinline__ void rc_detect(const Params& P, const RcLds& L, int t
         L.arg[t] = -1;
     }
     __syncthreads();
+}
+
+// Pair indexing of the reverse pass (after rc_detect<B, true>): the workgroup's pairs (segment t, entry e of its
+// list) numbered t-major, poff[t] the first index of slot t (an exclusive prefix sum of the list lengths), and for
+// each reverse-list entry k of segment t the index of the pair (k, position of t in k's list). Also the number of
+// mailbox chunks the workgroup's pairs take. Every lane calls it. The offsets and the chunk count are in place on
+// return; the reverse-list indices are read only after the barriers of the iteration that follows.
+template <int B>
+__device__ __forceinline__ int rc_pair_index(const RcLds& L, int t, bool act, uint16_t* lists, int slots, int* err) {
+    int* wsum = L.poff + B + 1;
+    const int lane = t & 63, w = t >> 6;
+    const int v = L.ncol[t];
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; k++) base += wsum[k];
+    L.poff[t] = base + x - v;
+    if (t == B - 1) L.poff[B] = base + x;
+    __syncthreads();
+    if (act) {
+        const int nr = L.nrev[t];
+        const uint16_t* rr = lists + (size_t)t * LIST_STRIDE + MAX_COLL;
+        uint16_t* ri = lists + (size_t)t * LIST_STRIDE + REV_IDX;
+        for (int e = 0; e < nr; e++) {
+            const int k = rr[e];
+            const uint16_t* fk = lists + (size_t)k * LIST_STRIDE;
+            int lo = 0, hi = L.ncol[k];
+            while (lo < hi) {  // t in k's list (ascending): the reverse test is k's own test of t
+                const int m = (lo + hi) >> 1;
+                if ((int)fk[m] < t)
+                    lo = m + 1;
+                else
+                    hi = m;
+            }
+            if (lo >= L.ncol[k] || fk[lo] != t) *err = ERR_MAX_COLLISIONS;
+            ri[e] = (uint16_t)(L.poff[k] + lo);
+        }
+    }
+    return (L.poff[B] + slots - 1) / slots;
+}
+
+// Adjoint of the workgroup's rope-rope pair terms at one iteration, each pair once. The pairs go through the
+// mailbox in chunks of `slots`: every lane evaluates pairs of the chunk (pair_term_adj of owner segment o and its
+// listed partner j, with the Newton adjoints u and directions of o, o + 1 from the exchange) and leaves the eight
+// adjoints there; after a barrier each vertex gathers its parts: of its own segment's pairs (to t and, through
+// the push, to t + 1) and of the pairs whose partner segment holds it (side 0: segment t, side 1: segment t - 1).
+// Each kind is summed in ascending pair order over all chunks and added at the end, so the result does not depend
+// on the chunk size or on which groups share the workgroup. Every lane calls it with the same nch.
+template <int B, bool F>
+__device__ __forceinline__ void rc_pairs_adj(const Params& P, const RcLds& L, const float* ex, const uint16_t* lists,
+                                             int t, bool act, bool hp, int nch, int slots, float dmp, f3& aX, f3& aD,
+                                             f3& pX, f3& pD) {
+    const PairMat pm = pair_mat(P);
+    const int ntot = L.poff[B];
+    f3 oX = mk3(0.f, 0.f, 0.f), oD = oX, oPX = oX, oPD = oX, r0X = oX, r0D = oX, r1X = oX, r1D = oX;
+    const int own0 = act ? L.poff[t] : 0, own1 = act ? L.poff[t + 1] : 0;
+    const uint16_t* ri0 = lists + (size_t)t * LIST_STRIDE + REV_IDX;
+    const uint16_t* ri1 = ri0 - LIST_STRIDE;
+    const int nr0 = act ? L.nrev[t] : 0, nr1 = (act && hp) ? L.nrev[t - 1] : 0;
+    int c0 = 0, c1 = 0;  // reverse-list cursors
+    for (int ch = 0; ch < nch; ch++) {
+        const int q0 = ch * slots, q1 = min(q0 + slots, ntot);
+        for (int q = q0 + t; q < q1; q += B) {
+            int lo = 0, hi = B - 1;  // owner: the last slot o with poff[o] <= q
+            while (lo < hi) {
+                const int m = (lo + hi + 1) >> 1;
+                if (L.poff[m] <= q)
+                    lo = m;
+                else
+                    hi = m - 1;
+            }
+            const int o = lo, j = lists[(size_t)o * LIST_STRIDE + (q - L.poff[o])];
+            f3 g[8];
+            pair_term_adj<F>(pm, dmp, ld3s(L.x, B, o), ld3s(L.x, B, o + 1), ld3s(L.dk, B, o), ld3s(L.dk, B, o + 1),
+                             ld3s(L.x, B, j), ld3s(L.x, B, j + 1), ld3s(L.dk, B, j), ld3s(L.dk, B, j + 1),
+                             ld3s(ex + EX_U * B, B, o), ld3s(ex + (EX_U + 3) * B, B, o), ld3s(ex + EX_U * B, B, o + 1),
+                             ld3s(ex + (EX_U + 3) * B, B, o + 1), g);
+#pragma unroll
+            for (int m = 0; m < 8; m++) st3s(L.mbx + 3 * m * slots, slots, q - q0, g[m]);
+        }
+        __syncthreads();
+        if (act) {
+            const float* mb = L.mbx;
+            for (int q = max(own0, q0); q < min(own1, q1); q++) {
+                oX = oX + ld3s(mb, slots, q - q0);
+                oPX = oPX + ld3s(mb + 3 * slots, slots, q - q0);
+                oD = oD + ld3s(mb + 6 * slots, slots, q - q0);
+                oPD = oPD + ld3s(mb + 9 * slots, slots, q - q0);
+            }
+            for (; c0 < nr0 && (int)ri0[c0] < q1; c0++) {
+                const int q = ri0[c0] - q0;
+                r0X = r0X + ld3s(mb + 12 * slots, slots, q);
+                r0D = r0D + ld3s(mb + 18 * slots, slots, q);
+            }
+            for (; c1 < nr1 && (int)ri1[c1] < q1; c1++) {
+                const int q = ri1[c1] - q0;
+                r1X = r1X + ld3s(mb + 15 * slots, slots, q);
+                r1D = r1D + ld3s(mb + 21 * slots, slots, q);
+            }
+        }
+        if (ch + 1 < nch) __syncthreads();
+    }
+    aX = aX + oX;
+    aD = aD + oD;
+    pX = pX + oPX;
+    pD = pD + oPD;
+    aX = aX + r0X;
+    aD = aD + r0D;
+    aX = aX + r1X;
+    aD = aD + r1D;
 }
 
 // Rope-rope contact modes of the adjoint kernels. With rope contact a frame's contact groups split in two: those
@@ -920,10 +1048,12 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             aD = aD + (invH * (1.f - P.drag * h)) * av;
             gdrag -= (double)h * (double)dot(d * invH, av);
         }
+        int nch = 0;  // mailbox chunks of the substep's pairs
         if (RC) {  // the substep's rope-rope lists and step limits, with the lists of the segments listing each one
             int g0, g1;
             group_range(P, D, r, v0, g0, g1);
             rc_detect<B, true>(P, L, t, act, c.fl, g0, g1, list_row(act ? t : 0), D.err, s == S - 1);
+            nch = rc_pair_index<B>(L, t, act, list_row(0), A.mbSlots, D.err);
         } else {
             __syncthreads();  // sPose of a moving frame is in place
         }
@@ -1079,8 +1209,8 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             }
             __syncthreads();
             // -- adjoint of the segment terms of vertex t (inertia, surfaces, stretch of segment (t, t+1))
+            f3 pD = mk3(0.f, 0.f, 0.f), pX = pD;
             if (act) {
-                f3 pD = mk3(0.f, 0.f, 0.f), pX = pD;
                 const float trG = -dot(u, dl);
                 if (c.w != 0.f) {
                     const float coef = Af::rcp(h * h * c.w);
@@ -1115,39 +1245,9 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                     aQ = qsub(aQ, rot_ex_adj(q0, cb));
                     gk += ((double)kkb + (double)ddb * (double)invl) * (double)invl;
                 }
-                if (RC) {  // rope-rope terms: this segment's own list, then the segments that list (t) or (t - 1)
-                    f3 gq[8];
-                    const f3 zero = mk3(0.f, 0.f, 0.f);
-                    const f3 u2 = hn ? ld3s(ex + EX_U * B, B, t + 1) : zero;
-                    const f3 dl2 = hn ? ld3s(ex + (EX_U + 3) * B, B, t + 1) : zero;
-                    const int ncol = L.ncol[t];
-                    const uint16_t* row = list_row(t);
-                    for (int e = 0; e < ncol; e++) {
-                        const int j = row[e];
-                        pair_term_adj<F>(pair_mat(P), dmp, X0, X1, d0, d1, ld3s(L.x, B, j), ld3s(L.x, B, j + 1), ld3s(L.dk, B, j),
-                                         ld3s(L.dk, B, j + 1), u, dl, u2, dl2, gq);
-                        aX = aX + gq[0];
-                        aD = aD + gq[2];
-                        pX = pX + gq[1];
-                        pD = pD + gq[3];
-                    }
-                    for (int side = 0; side < 2; side++) {
-                        const int jt = t - side;  // the partner segment (jt, jt + 1) holds this vertex at `side`
-                        if (side == 1 && !hp) break;
-                        const int nr = L.nrev[jt];
-                        const uint16_t* rr = list_row(jt) + MAX_COLL;
-                        for (int e = 0; e < nr; e++) {
-                            const int k = rr[e];
-                            pair_term_adj<F>(pair_mat(P), dmp, ld3s(L.x, B, k), ld3s(L.x, B, k + 1), ld3s(L.dk, B, k),
-                                             ld3s(L.dk, B, k + 1), ld3s(L.x, B, jt), ld3s(L.x, B, jt + 1),
-                                             ld3s(L.dk, B, jt), ld3s(L.dk, B, jt + 1), ld3s(ex + EX_U * B, B, k),
-                                             ld3s(ex + (EX_U + 3) * B, B, k), ld3s(ex + EX_U * B, B, k + 1),
-                                             ld3s(ex + (EX_U + 3) * B, B, k + 1), gq);
-                            aX = aX + gq[4 + side];
-                            aD = aD + gq[6 + side];
-                        }
-                    }
-                }
+            }
+            if (RC) rc_pairs_adj<B, F>(P, L, ex, list_row(0), t, act, hp, nch, A.mbSlots, dmp, aX, aD, pX, pD);
+            if (act) {
                 st3s(ex + EX_P * B, B, t, pD);
                 st3s(ex + (EX_P + 3) * B, B, t, pX);
             }
@@ -1357,21 +1457,38 @@ int adjoint_waves_per_simd(int block, bool reverse, bool fast, bool ropeContact)
     }
 }
 
+// LDS of a gfx950 compute unit, all of which one workgroup may take.
+constexpr size_t LDS_BYTES = 160 * 1024;
+
 template <int B, bool F, int RC>
 static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, const AdjDev& A, const Drive& Dr, int S,
                                 int nblocks, size_t lds, hipStream_t s) {
-    // dynamic LDS past the default limit needs the kernel's attribute raised once
-    static bool attr[2] = {false, false};
+    // dynamic LDS past the default limit needs the kernel's attribute raised once: to all the LDS its static
+    // arrays leave
+    static size_t stat[2] = {0, 0};
     const void* fn = pass_fn<B, RC>(reverse, F);
-    if (!attr[reverse]) {
-        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 104 * 1024);
+    if (!stat[reverse]) {
+        hipFuncAttributes a{};
+        hipError_t e = hipFuncGetAttributes(&a, fn);
         if (e != hipSuccess) return e;
-        attr[reverse] = true;
+        if (a.sharedSizeBytes >= LDS_BYTES) return hipErrorInvalidValue;
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(LDS_BYTES - a.sharedSizeBytes));
+        if (e != hipSuccess) return e;
+        stat[reverse] = a.sharedSizeBytes > 0 ? a.sharedSizeBytes : 1;
+    }
+    AdjDev Ak = A;
+    Ak.mbSlots = 0;
+    if (reverse && RC == RC_PAIRS) {  // the pair mailbox takes what is left, up to two slots per thread
+        const size_t used = stat[1] + lds + sizeof(int) * POFF_N(B);
+        const size_t fit = used < LDS_BYTES ? (LDS_BYTES - used) / (PAIR_ADJ_N * sizeof(float)) / 64 * 64 : 0;
+        Ak.mbSlots = (int)std::min<size_t>(2 * B, fit);
+        if (Ak.mbSlots < 64) return hipErrorInvalidValue;
+        lds += sizeof(int) * POFF_N(B) + (size_t)Ak.mbSlots * PAIR_ADJ_N * sizeof(float);
     }
     if (reverse)
-        hipLaunchKernelGGL((k_adjoint_reverse<B, F, RC>), dim3(nblocks), dim3(B), lds, s, P, D, A, Dr, S);
+        hipLaunchKernelGGL((k_adjoint_reverse<B, F, RC>), dim3(nblocks), dim3(B), lds, s, P, D, Ak, Dr, S);
     else
-        hipLaunchKernelGGL((k_adjoint_forward<B, F, RC>), dim3(nblocks), dim3(B), lds, s, P, D, A, Dr, S);
+        hipLaunchKernelGGL((k_adjoint_forward<B, F, RC>), dim3(nblocks), dim3(B), lds, s, P, D, Ak, Dr, S);
     return hipGetLastError();
 }
 template <int B>
