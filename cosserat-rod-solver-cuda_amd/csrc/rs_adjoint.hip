@@ -17,6 +17,7 @@
 // H̄ = -u δ^T with δ = H^-1 F. The pair (u, δ) stands for H̄; every H term here is s I or s n n^T, so
 // only tr H̄ = -u.δ, n^T H̄ n = -(u.n)(δ.n) and (H̄ + H̄^T) n = -(u (δ.n) + δ (u.n)) are needed.
 #include <algorithm>
+#include <cstdlib>
 
 #include "rs_adjoint.hpp"
 #include "rs_device.hpp"
@@ -368,14 +369,26 @@ struct RcLds {
                                               // build radii [B], build boxes [6][B], chunk boxes [6][B/16]
     float *dk, *mb, *mown, *mprev;            // (reverse pass) iterate d^k [3][B], step-limit adjoints [B] x 3
     int *fl, *rope, *arg, *nrev, *ncol, *vc;  // flags, rope, limit argument, list lengths, skin-list length (-1: scan)
-    // (reverse pass with pairs) first pair index of each slot [B + 1] (then 8 wave sums and the chunk count), and
-    // the pair mailbox [PAIR_ADJ_N][slots]
+    // (reverse pass with pairs) first pair index of each slot [B + 1] (then 8 wave sums), the pairs (owner slot |
+    // partner slot << 16) [PAIR_TAB_OF(B)], the reverse lists of up to REV_SLOTS_OF(B) entries [B][REV_SLOTS_OF(B)],
+    // and the pair mailbox [PAIR_ADJ_N][slots]
     int* poff;
+    uint32_t* pairs;
+    uint16_t* rq;
     float* mbx;
 };
 // Floats a pair's adjoint leaves in the mailbox: the adjoints of its 8 inputs (pair_term_adj).
 constexpr int PAIR_ADJ_N = 24;
 constexpr int POFF_N(int B) { return B + 16; }
+// Pairs of a workgroup's substep indexed in LDS (more are found from the offsets and the lists in memory), and
+// reverse-list entries per segment kept in LDS (a longer reverse list lives in memory). The 512-thread form keeps
+// both in memory: its LDS is taken by the arrays it needs at every size.
+constexpr int PAIR_TAB_OF(int B) { return B <= 384 ? 1024 : 0; }
+constexpr int REV_SLOTS_OF(int B) { return B <= 384 ? 8 : 0; }
+// LDS of the pair index arrays before the mailbox.
+constexpr size_t pair_index_lds_bytes(int B) {
+    return sizeof(int) * POFF_N(B) + sizeof(uint32_t) * PAIR_TAB_OF(B) + sizeof(uint16_t) * B * REV_SLOTS_OF(B);
+}
 
 // The detection arrays in dynamic LDS at `base` (after the mesh pose slots); x: positions already in LDS, or
 // null for a region of their own.
@@ -407,7 +420,9 @@ __device__ __forceinline__ RcLds rc_lds(char* base, float* x) {
     L.ncol = q + 4 * B;
     L.vc = q + 5 * B;
     L.poff = q + 6 * B;
-    L.mbx = reinterpret_cast<float*>(L.poff + POFF_N(B));
+    L.pairs = reinterpret_cast<uint32_t*>(L.poff + POFF_N(B));
+    L.rq = reinterpret_cast<uint16_t*>(L.pairs + PAIR_TAB_OF(B));
+    L.mbx = reinterpret_cast<float*>(L.rq + B * REV_SLOTS_OF(B));
     return L;
 }
 
@@ -494,11 +509,10 @@ __device__ __forceinline__ void rc_skin_list(const RcLds& L, int t, uint32_t fl,
     L.vc[t] = cnt <= SKIN_CAP ? cnt : -1;
 }
 
-// The exact tests of segment t against candidate j (the forward's: does t list j), and with REV also from j's
-// side (does j list t). rt, rj: the segments' detection radii.
-__device__ __forceinline__ void rc_pair_tests(const RcLds& L, int B, int t, int j, float rt, float rj, f3 a0, f3 a1,
-                                              int& cnt, uint16_t* fwd, float& lim, int& arg, bool rev, int& rc,
-                                              uint16_t* revRow) {
+// The exact test of segment t against candidate j (the forward's: does t list j). rt, rj: the segments' detection
+// radii.
+__device__ __forceinline__ void rc_pair_test(const RcLds& L, int B, int j, float rt, float rj, f3 a0, f3 a1, int& cnt,
+                                             uint16_t* fwd, float& lim, int& arg) {
     float d2;
     if (pair_hit(a1, ld3s(L.x, B, j) - a0, ld3s(L.x, B, j + 1) - a0, rt + rj, d2)) {
         if (cnt < MAX_COLL) {
@@ -511,26 +525,17 @@ __device__ __forceinline__ void rc_pair_tests(const RcLds& L, int B, int t, int 
         }
         cnt++;
     }
-    if (rev) {
-        const f3 c0 = ld3s(L.x, B, j);
-        float e2;
-        if (pair_hit(ld3s(L.x, B, j + 1) - c0, a0 - c0, ld3s(L.x, B, t + 1) - c0, rj + rt, e2)) {
-            if (rc < MAX_COLL) revRow[rc] = (uint16_t)j;
-            rc++;
-        }
-    }
 }
 
 // The substep's lists of segment t: exact tests over its skin list (or over its whole group after a skin-list
 // overflow), candidates whose current boxes (padded by the detection radii) miss its own skipped first. Writes
-// the partner count, reverse count, limit and the limit's slot (-1: the padding) to LDS.
-template <int B, bool REV>
+// the partner count, limit and the limit's slot (-1: the padding) to LDS, and zeroes the reverse count.
+template <int B>
 __device__ __forceinline__ void rc_lists(const Params& P, const RcLds& L, int t, uint32_t fl, int g0, int g1,
                                          uint16_t* row, int* err) {
     uint16_t* fwd = row;
-    uint16_t* rev = row + MAX_COLL;
     const uint16_t* skin = row + 2 * MAX_COLL;
-    int cnt = 0, rc = 0, arg = -1;
+    int cnt = 0, arg = -1;
     float lim = INFINITY;
     if (fl & HAS_NEXT) {
         lim = L.pad[t] - P.detR;
@@ -552,12 +557,12 @@ __device__ __forceinline__ void rc_lists(const Params& P, const RcLds& L, int t,
             if (fminf(c0.x, c1.x) - rj > hi.x || fminf(c0.y, c1.y) - rj > hi.y || fminf(c0.z, c1.z) - rj > hi.z ||
                 fmaxf(c0.x, c1.x) + rj < lo.x || fmaxf(c0.y, c1.y) + rj < lo.y || fmaxf(c0.z, c1.z) + rj < lo.z)
                 continue;
-            rc_pair_tests(L, B, t, j, rt, rj, a0, a1, cnt, fwd, lim, arg, REV, rc, rev);
+            rc_pair_test(L, B, j, rt, rj, a0, a1, cnt, fwd, lim, arg);
         }
     }
-    if (cnt > MAX_COLL || rc > MAX_COLL) *err = ERR_MAX_COLLISIONS;
+    if (cnt > MAX_COLL) *err = ERR_MAX_COLLISIONS;
     L.ncol[t] = min(cnt, MAX_COLL);
-    L.nrev[t] = min(rc, MAX_COLL);
+    L.nrev[t] = 0;
     L.lim[t] = lim;
     L.arg[t] = arg;
 }
@@ -565,7 +570,7 @@ __device__ __forceinline__ void rc_lists(const Params& P, const RcLds& L, int t,
 // One substep's detection by every lane of the workgroup: the rebuild vote (a barrier; it also publishes what
 // the caller wrote before), a skin-list build when any slot voted or `first`, then the exact tests; ends with a
 // barrier after which every slot's lists, counts and limits are in place.
-template <int B, bool REV>
+template <int B>
 __device__ __forceinline__ void rc_detect(const Params& P, const RcLds& L, int t, bool act, uint32_t fl, int g0, int g1,
                                           uint16_t* row, int* err, bool first) {
     const bool stale = first || rc_skin_stale(P, L, B, t, act, fl);
@@ -574,7 +579,7 @@ __device__ __forceinline__ void rc_detect(const Params& P, const RcLds& L, int t
         __syncthreads();
         if (act) rc_skin_list<B>(L, t, fl, g0, g1, row + 2 * MAX_COLL);
     }
-    if (act) rc_lists<B, REV>(P, L, t, fl, g0, g1, row, err);
+    if (act) rc_lists<B>(P, L, t, fl, g0, g1, row, err);
     else {
         L.ncol[t] = 0;
         L.nrev[t] = 0;
@@ -584,11 +589,14 @@ __device__ __forceinline__ void rc_detect(const Params& P, const RcLds& L, int t
     __syncthreads();
 }
 
-// Pair indexing of the reverse pass (after rc_detect<B, true>): the workgroup's pairs (segment t, entry e of its
-// list) numbered t-major, poff[t] the first index of slot t (an exclusive prefix sum of the list lengths), and for
-// each reverse-list entry k of segment t the index of the pair (k, position of t in k's list). Also the number of
-// mailbox chunks the workgroup's pairs take. Every lane calls it. The offsets and the chunk count are in place on
-// return; the reverse-list indices are read only after the barriers of the iteration that follows.
+// Pair indexing of the reverse pass (after rc_detect): the workgroup's pairs (segment t, entry e of its list)
+// numbered t-major, poff[t] the first index of slot t (an exclusive prefix sum of the list lengths), the pair table
+// (owner and partner of the first PAIR_TAB_OF(B) pairs), and the reverse lists: for each segment j, the indices of the
+// pairs whose partner it is, ascending (so by owner). They are the transpose of the partner lists, filled through
+// LDS counters and then sorted; a list of up to REV_SLOTS_OF(B) entries is kept in LDS, a longer one in its row in
+// memory. Returns the number of mailbox chunks the workgroup's pairs take. Every lane calls it. The offsets are in
+// place on return; the table and the reverse lists are read only after the barriers of the iteration that
+// follows.
 template <int B>
 __device__ __forceinline__ int rc_pair_index(const RcLds& L, int t, bool act, uint16_t* lists, int slots, int* err) {
     int* wsum = L.poff + B + 1;
@@ -604,100 +612,67 @@ __device__ __forceinline__ int rc_pair_index(const RcLds& L, int t, bool act, ui
     __syncthreads();
     int base = 0;
     for (int k = 0; k < w; k++) base += wsum[k];
-    L.poff[t] = base + x - v;
+    const int first = base + x - v;
+    L.poff[t] = first;
     if (t == B - 1) L.poff[B] = base + x;
+    if (act) {  // L.nrev is zero after the detection
+        const uint16_t* row = lists + (size_t)t * LIST_STRIDE;
+        for (int e = 0; e < v; e++) {
+            const int j = row[e], q = first + e;
+            if (q < PAIR_TAB_OF(B)) L.pairs[q] = (uint32_t)t | ((uint32_t)j << 16);
+            const int pos = atomicAdd(&L.nrev[j], 1);
+            if (pos < REV_SLOTS_OF(B)) L.rq[j * REV_SLOTS_OF(B) + pos] = (uint16_t)q;
+            if (pos < MAX_COLL) lists[(size_t)j * LIST_STRIDE + REV_IDX + pos] = (uint16_t)q;
+        }
+    }
     __syncthreads();
     if (act) {
-        const int nr = L.nrev[t];
-        const uint16_t* rr = lists + (size_t)t * LIST_STRIDE + MAX_COLL;
-        uint16_t* ri = lists + (size_t)t * LIST_STRIDE + REV_IDX;
-        for (int e = 0; e < nr; e++) {
-            const int k = rr[e];
-            const uint16_t* fk = lists + (size_t)k * LIST_STRIDE;
-            int lo = 0, hi = L.ncol[k];
-            while (lo < hi) {  // t in k's list (ascending): the reverse test is k's own test of t
-                const int m = (lo + hi) >> 1;
-                if ((int)fk[m] < t)
-                    lo = m + 1;
-                else
-                    hi = m;
+        int n = L.nrev[t];
+        if (n > MAX_COLL) {
+            *err = ERR_MAX_COLLISIONS;
+            n = MAX_COLL;
+            L.nrev[t] = n;
+        }
+        uint16_t* ri = n <= REV_SLOTS_OF(B) ? L.rq + t * REV_SLOTS_OF(B) : lists + (size_t)t * LIST_STRIDE + REV_IDX;
+        for (int a = 1; a < n; a++) {  // insertion sort (short lists)
+            const uint16_t ii = ri[a];
+            int b = a - 1;
+            while (b >= 0 && ri[b] > ii) {
+                ri[b + 1] = ri[b];
+                b--;
             }
-            if (lo >= L.ncol[k] || fk[lo] != t) *err = ERR_MAX_COLLISIONS;
-            ri[e] = (uint16_t)(L.poff[k] + lo);
+            ri[b + 1] = ii;
         }
     }
     return (L.poff[B] + slots - 1) / slots;
 }
 
-// Adjoint of the workgroup's rope-rope pair terms at one iteration, each pair once. The pairs go through the
-// mailbox in chunks of `slots`: every lane evaluates pairs of the chunk (pair_term_adj of owner segment o and its
-// listed partner j, with the Newton adjoints u and directions of o, o + 1 from the exchange) and leaves the eight
-// adjoints there; after a barrier each vertex gathers its parts: of its own segment's pairs (to t and, through
-// the push, to t + 1) and of the pairs whose partner segment holds it (side 0: segment t, side 1: segment t - 1).
-// Each kind is summed in ascending pair order over all chunks and added at the end, so the result does not depend
-// on the chunk size or on which groups share the workgroup. Every lane calls it with the same nch.
-template <int B, bool F>
-__device__ __forceinline__ void rc_pairs_adj(const Params& P, const RcLds& L, const float* ex, const uint16_t* lists,
-                                             int t, bool act, bool hp, int nch, int slots, float dmp, f3& aX, f3& aD,
-                                             f3& pX, f3& pD) {
-    const PairMat pm = pair_mat(P);
-    const int ntot = L.poff[B];
-    f3 oX = mk3(0.f, 0.f, 0.f), oD = oX, oPX = oX, oPD = oX, r0X = oX, r0D = oX, r1X = oX, r1D = oX;
-    const int own0 = act ? L.poff[t] : 0, own1 = act ? L.poff[t + 1] : 0;
-    const uint16_t* ri0 = lists + (size_t)t * LIST_STRIDE + REV_IDX;
-    const uint16_t* ri1 = ri0 - LIST_STRIDE;
-    const int nr0 = act ? L.nrev[t] : 0, nr1 = (act && hp) ? L.nrev[t - 1] : 0;
-    int c0 = 0, c1 = 0;  // reverse-list cursors
-    for (int ch = 0; ch < nch; ch++) {
-        const int q0 = ch * slots, q1 = min(q0 + slots, ntot);
-        for (int q = q0 + t; q < q1; q += B) {
-            int lo = 0, hi = B - 1;  // owner: the last slot o with poff[o] <= q
-            while (lo < hi) {
-                const int m = (lo + hi + 1) >> 1;
-                if (L.poff[m] <= q)
-                    lo = m;
-                else
-                    hi = m - 1;
-            }
-            const int o = lo, j = lists[(size_t)o * LIST_STRIDE + (q - L.poff[o])];
-            f3 g[8];
-            pair_term_adj<F>(pm, dmp, ld3s(L.x, B, o), ld3s(L.x, B, o + 1), ld3s(L.dk, B, o), ld3s(L.dk, B, o + 1),
-                             ld3s(L.x, B, j), ld3s(L.x, B, j + 1), ld3s(L.dk, B, j), ld3s(L.dk, B, j + 1),
-                             ld3s(ex + EX_U * B, B, o), ld3s(ex + (EX_U + 3) * B, B, o), ld3s(ex + EX_U * B, B, o + 1),
-                             ld3s(ex + (EX_U + 3) * B, B, o + 1), g);
-#pragma unroll
-            for (int m = 0; m < 8; m++) st3s(L.mbx + 3 * m * slots, slots, q - q0, g[m]);
-        }
-        __syncthreads();
-        if (act) {
-            const float* mb = L.mbx;
-            for (int q = max(own0, q0); q < min(own1, q1); q++) {
-                oX = oX + ld3s(mb, slots, q - q0);
-                oPX = oPX + ld3s(mb + 3 * slots, slots, q - q0);
-                oD = oD + ld3s(mb + 6 * slots, slots, q - q0);
-                oPD = oPD + ld3s(mb + 9 * slots, slots, q - q0);
-            }
-            for (; c0 < nr0 && (int)ri0[c0] < q1; c0++) {
-                const int q = ri0[c0] - q0;
-                r0X = r0X + ld3s(mb + 12 * slots, slots, q);
-                r0D = r0D + ld3s(mb + 18 * slots, slots, q);
-            }
-            for (; c1 < nr1 && (int)ri1[c1] < q1; c1++) {
-                const int q = ri1[c1] - q0;
-                r1X = r1X + ld3s(mb + 15 * slots, slots, q);
-                r1D = r1D + ld3s(mb + 21 * slots, slots, q);
-            }
-        }
-        if (ch + 1 < nch) __syncthreads();
+// The reverse list of slot j (pair indices, ascending) and its length.
+template <int B>
+__device__ __forceinline__ const uint16_t* rc_rev_list(const RcLds& L, const uint16_t* lists, int j, int& n) {
+    n = L.nrev[j];
+    return n <= REV_SLOTS_OF(B) ? L.rq + j * REV_SLOTS_OF(B) : lists + (size_t)j * LIST_STRIDE + REV_IDX;
+}
+
+// Owner slot and partner slot of pair q.
+template <int B>
+__device__ __forceinline__ void rc_pair_of(const RcLds& L, const uint16_t* lists, int q, int& o, int& j) {
+    if (q < PAIR_TAB_OF(B)) {
+        const uint32_t pr = L.pairs[q];
+        o = (int)(pr & 0xFFFFu);
+        j = (int)(pr >> 16);
+        return;
     }
-    aX = aX + oX;
-    aD = aD + oD;
-    pX = pX + oPX;
-    pD = pD + oPD;
-    aX = aX + r0X;
-    aD = aD + r0D;
-    aX = aX + r1X;
-    aD = aD + r1D;
+    int lo = 0, hi = B - 1;  // the last slot o with poff[o] <= q
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (L.poff[m] <= q)
+            lo = m;
+        else
+            hi = m - 1;
+    }
+    o = lo;
+    j = lists[(size_t)o * LIST_STRIDE + (q - L.poff[o])];
 }
 
 // Rope-rope contact modes of the adjoint kernels. With rope contact a frame's contact groups split in two: those
@@ -706,6 +681,139 @@ __device__ __forceinline__ void rc_pairs_adj(const Params& P, const RcLds& L, co
 // its padding less the detection radius and only the step clamp remains). Both kernels run over the same
 // workgroups; each workgroup keeps to the one its groups' flags name.
 enum { RC_NONE = 0, RC_CLAMP = 1, RC_PAIRS = 2 };
+
+// The adjoint of pair q of the workgroup (pair_term_adj of owner segment o and its listed partner j, with the
+// Newton adjoints u and directions of o, o + 1 from the exchange), left in mailbox slot e.
+template <int B, bool F>
+__device__ __forceinline__ void rc_pair_eval(const PairMat& pm, float dmp, const RcLds& L, const float* ex,
+                                             const uint16_t* lists, int q, int e, int slots) {
+    int o, j;
+    rc_pair_of<B>(L, lists, q, o, j);
+    f3 g[8];
+    pair_term_adj<F>(pm, dmp, ld3s(L.x, B, o), ld3s(L.x, B, o + 1), ld3s(L.dk, B, o), ld3s(L.dk, B, o + 1),
+                     ld3s(L.x, B, j), ld3s(L.x, B, j + 1), ld3s(L.dk, B, j), ld3s(L.dk, B, j + 1),
+                     ld3s(ex + EX_U * B, B, o), ld3s(ex + (EX_U + 3) * B, B, o), ld3s(ex + EX_U * B, B, o + 1),
+                     ld3s(ex + (EX_U + 3) * B, B, o + 1), g);
+#pragma unroll
+    for (int m = 0; m < 8; m++) st3s(L.mbx + 3 * m * slots, slots, e, g[m]);
+}
+
+// Vertex t's parts of the pair adjoints in the mailbox, for the pairs [q0, q1) it holds: of the pairs of segment
+// t - 1 (to its second vertex, t) and of segment t (to its first vertex), and of the pairs whose partner segment
+// holds it (reverse lists of t, side 0, and of t - 1, side 1). The sources are merged in ascending pair index,
+// which within a contact group is the group's own order, so the sums do not depend on how the pairs are chunked
+// or on which groups share the workgroup. The cursors carry over from chunk to chunk.
+template <int B>
+__device__ __forceinline__ void rc_pairs_gather(const RcLds& L, const uint16_t* lists, int t, bool hp, int q0, int q1,
+                                                int slots, int& qo, int& c0, int& c1, f3& aX, f3& aD) {
+    const int mid = L.poff[t], own1 = L.poff[t + 1];
+    int nr0 = 0, nr1 = 0;
+    const uint16_t* ri0 = rc_rev_list<B>(L, lists, t, nr0);
+    const uint16_t* ri1 = hp ? rc_rev_list<B>(L, lists, t - 1, nr1) : ri0;
+    if (!hp) nr1 = 0;
+    const float* mb = L.mbx;
+    while (true) {  // next source in ascending pair index (each pair is in at most one of them)
+        const int a = qo < min(own1, q1) ? qo : INT_MAX;
+        const int b0 = c0 < nr0 ? (int)ri0[c0] : INT_MAX;
+        const int b1 = c1 < nr1 ? (int)ri1[c1] : INT_MAX;
+        const int q = min(a, min(b0, b1));
+        if (q >= q1) break;
+        const int e = q - q0;
+        if (q == a) {
+            if (q < mid) {  // a pair of segment t - 1: its second vertex's part
+                aX = aX + ld3s(mb + 3 * slots, slots, e);
+                aD = aD + ld3s(mb + 9 * slots, slots, e);
+            } else {
+                aX = aX + ld3s(mb, slots, e);
+                aD = aD + ld3s(mb + 6 * slots, slots, e);
+            }
+            qo++;
+        } else if (q == b0) {
+            aX = aX + ld3s(mb + 12 * slots, slots, e);
+            aD = aD + ld3s(mb + 18 * slots, slots, e);
+            c0++;
+        } else {
+            aX = aX + ld3s(mb + 15 * slots, slots, e);
+            aD = aD + ld3s(mb + 21 * slots, slots, e);
+            c1++;
+        }
+    }
+}
+
+// The pair adjoints of one iteration of the reverse pass. Each pair is evaluated once (by the workgroup's lanes
+// with EVAL, else by its pair waves, rc_pair_waves) and leaves its eight adjoints in the mailbox. When the
+// substep's pairs fit in one mailbox chunk (nch == 1) they are only evaluated here: the vertices gather them at the
+// start of the next iteration's Newton adjoint (or after the last iteration), so the evaluation overlaps the
+// segment terms and the next orientation sweep. Otherwise the chunks go through here one by one, each gathered
+// after a barrier. Every vertex lane calls it with the same nch.
+template <int B, bool F, bool EVAL>
+__device__ __forceinline__ void rc_pairs_adj(const Params& P, const RcLds& L, const float* ex, const uint16_t* lists,
+                                             int t, bool act, bool hp, int nch, int slots, float dmp, f3& aX,
+                                             f3& aD) {
+    const int ntot = L.poff[B];
+    const PairMat pm = pair_mat(P);
+    if (nch == 1) {
+        if (EVAL)
+            for (int q = t; q < ntot; q += B) rc_pair_eval<B, F>(pm, dmp, L, ex, lists, q, q, slots);
+        return;
+    }
+    int qo = (act && hp) ? L.poff[t - 1] : act ? L.poff[t] : 0, c0 = 0, c1 = 0;
+    for (int ch = 0; ch < nch; ch++) {
+        const int q0 = ch * slots, q1 = min(q0 + slots, ntot);
+        if (EVAL)
+            for (int q = q0 + t; q < q1; q += B) rc_pair_eval<B, F>(pm, dmp, L, ex, lists, q, q - q0, slots);
+        __syncthreads();
+        if (act) rc_pairs_gather<B>(L, lists, t, hp, q0, q1, slots, qo, c0, c1, aX, aD);
+        if (ch + 1 < nch) __syncthreads();
+    }
+}
+
+// Lanes of the reverse pass's pair waves (threads B .. B + PAIR_LANES - 1 of a workgroup with rope-rope pairs):
+// they hold no vertex, so the pair adjoints they evaluate do not compete with the vertex lanes' state for
+// registers, and they run while the vertex lanes take their segment terms.
+constexpr int PAIR_LANES = 128;
+template <int B, int RCM>
+constexpr bool pair_waves() { return RCM == RC_PAIRS && B <= 384; }
+
+// The pair waves' side of the reverse pass: the same sequence of barriers as the vertex lanes (substep head,
+// detection vote and rebuild, end of detection, the two of the pair indexing; per iteration the orientation
+// sweep, the Newton adjoint and, with more than one mailbox chunk, the chunks'; after the iterations the two of
+// the step-limit adjoints; the two of the drag sum), evaluating each iteration's pairs after the Newton adjoint's
+// barrier.
+template <int B, bool F>
+__device__ void rc_pair_waves(const Params& P, const RcLds& L, const float* ex, const uint16_t* lists, int lane, int S,
+                              int I, int slots) {
+    const PairMat pm = pair_mat(P);
+    const float dmp = Ar<F>::div(P.damping, P.h);
+    for (int s = S - 1; s >= 0; s--) {
+        __syncthreads();  // substep head
+        if (__syncthreads_or(0)) __syncthreads();  // detection: rebuild vote, skin boxes
+        __syncthreads();  // detection: lists in place
+        __syncthreads();  // pair indexing: offsets
+        __syncthreads();  // pair indexing: reverse lists
+        const int ntot = L.poff[B];
+        const int nch = (ntot + slots - 1) / slots;
+        for (int k = I - 1; k >= 0; k--) {
+            __syncthreads();  // orientation sweep
+            __syncthreads();  // Newton adjoint u, direction, d^k in place
+            if (nch == 1) {  // gathered after the next barrier
+                for (int q = lane; q < ntot; q += PAIR_LANES) rc_pair_eval<B, F>(pm, dmp, L, ex, lists, q, q, slots);
+                continue;
+            }
+            for (int ch = 0; ch < nch; ch++) {
+                const int q0 = ch * slots, q1 = min(q0 + slots, ntot);
+                for (int q = q0 + lane; q < q1; q += PAIR_LANES) rc_pair_eval<B, F>(pm, dmp, L, ex, lists, q, q - q0, slots);
+                __syncthreads();  // mailbox chunk full
+                if (ch + 1 < nch) __syncthreads();  // chunk gathered
+            }
+        }
+        __syncthreads();  // after the iterations: pushes and pairs of the last one
+        __syncthreads();  // step-limit adjoints
+    }
+    __syncthreads();  // drag sum
+    __syncthreads();
+}
+
 __device__ __forceinline__ bool rc_wrong_class(const Params& P, const AdjDev& A, int RCM, int rb0, int rb1) {
     if (RCM == RC_NONE) return false;
     bool pairs = A.pairFlag == nullptr;
@@ -831,7 +939,7 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
         if (RC) {  // rope-rope lists and step limits of the substep (the forward's detection); its vote is the barrier
             int g0, g1;
             group_range(P, D, r, v0, g0, g1);
-            rc_detect<B, false>(P, L, t, act, c.fl, g0, g1, A.lists + (size_t)(act ? li : 0) * LIST_STRIDE, D.err, s == 0);
+            rc_detect<B>(P, L, t, act, c.fl, g0, g1, A.lists + (size_t)(act ? li : 0) * LIST_STRIDE, D.err, s == 0);
         } else {
             __syncthreads();
         }
@@ -960,8 +1068,11 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
 // Per iteration three phases: orientation adjoint (pushes to the neighbours), Newton adjoint u = a H^-1 d̄,
 // segment-term adjoint (pushes to the next vertex).
 template <int B, bool F, int RCM>
-__global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A, Drive Dr, int S) {
+__global__ __launch_bounds__(B + (pair_waves<B, RCM>() ? PAIR_LANES : 0)) void k_adjoint_reverse(Params P, Dev D,
+                                                                                                  AdjDev A, Drive Dr,
+                                                                                                  int S) {
     constexpr bool RC = RCM == RC_PAIRS;
+    constexpr bool PW = pair_waves<B, RCM>();
     __shared__ float ex[EX_N * B];
     extern __shared__ Pose sPoseAll[];  // the poses of the workgroup's ropes, sized at launch; then, with rope
                                         // contact, the detection arrays (rc_lds_bytes)
@@ -969,6 +1080,11 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
     const int rb0 = A.blk[blockIdx.x], rb1 = A.blk[blockIdx.x + 1], t = threadIdx.x;
     if (rc_wrong_class(P, A, RCM, rb0, rb1)) return;
     const int v0 = D.roff[rb0], n = D.roff[rb1] - v0;
+    if (PW && t >= B) {  // a pair wave
+        const RcLds L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + (size_t)(rb1 - rb0) * P.numSdf), nullptr);
+        rc_pair_waves<B, F>(P, L, ex, A.lists + (size_t)(v0 - A.hv0) * LIST_STRIDE, t - B, S, I, A.mbSlots);
+        return;
+    }
     const bool act = t < n;
     const int i = v0 + t, li = i - A.hv0;
     const int r = act ? D.ropeOf[i] : rb0;  // the vertex's rope; consecutive ropes share the workgroup
@@ -1015,7 +1131,7 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
         if (P.numSdf > 0 && P.sdfMoving)
             for (int m = t; m < nPose; m += B)
                 sPoseAll[m] = substep_pose(A.fmotion[(size_t)rb0 * P.numSdf + m], D.grids[m % P.numSdf], s, S, h);
-        f3 X0 = mk3(0.f, 0.f, 0.f), X1 = X0, y = X0, vs = X0, vls = X0;
+        f3 X0 = mk3(0.f, 0.f, 0.f), X1 = X0, y = X0;
         int nc = 0;
         uint64_t cand = 0, hitMask = ~0ull;
         if (act) {
@@ -1024,8 +1140,6 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             nc = __float_as_int(xs.w);
             if (hn) X1 = ld3(A.hX[hs(A, s, li + 1)]);
             y = ld3(A.hY[hs(A, s, li)]);
-            vs = ld3(A.hV[hs(A, s, li)]);
-            vls = ld3(A.hVL[hs(A, s, li)]);
             const uint2 cc = A.hC[hs(A, s, li)];
             cand = (uint64_t)cc.x | ((uint64_t)cc.y << 32);
             const uint2 mm = A.hM[hs(A, s, li)];
@@ -1052,7 +1166,7 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
         if (RC) {  // the substep's rope-rope lists and step limits, with the lists of the segments listing each one
             int g0, g1;
             group_range(P, D, r, v0, g0, g1);
-            rc_detect<B, true>(P, L, t, act, c.fl, g0, g1, list_row(act ? t : 0), D.err, s == S - 1);
+            rc_detect<B>(P, L, t, act, c.fl, g0, g1, list_row(act ? t : 0), D.err, s == S - 1);
             nch = rc_pair_index<B>(L, t, act, list_row(0), A.mbSlots, D.err);
         } else {
             __syncthreads();  // sPose of a moving frame is in place
@@ -1098,22 +1212,16 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
             cur = load_iter(I - 1, dI0, dI1);
         }
         for (int k = I - 1; k >= 0; k--) {
-            // without rope contact, iteration k-1's history is loaded here and lands while k computes; with it, the
-            // registers the early loads would hold go to the rope-rope terms, and each iteration loads its own
+            // iteration k-1's history is loaded here and lands while k computes
             IterHist nxt = cur;
-            if (!RC && k > 0) nxt = load_iter(k - 1, cur.d0, cur.d1);
+            if (k > 0) nxt = load_iter(k - 1, cur.d0, cur.d1);
             const f3 dn0 = cur.dn0, dn1 = cur.dn1, d0 = cur.d0, d1 = cur.d1;
             const q4 q0 = cur.q0, qp = cur.qp, qn = cur.qn;
             const s3 H = cur.H;
             const f3 dl = cur.dl;
-            // -- orientation sweep k: q^{k+1}_t from d^{k+1} and q^k_{t-1..t+1} (plus the previous
-            //    iteration's segment-term pushes)
+            // -- orientation sweep k: q^{k+1}_t from d^{k+1} and q^k_{t-1..t+1}
             q4 keepQ = aQ;
             if (act) {
-                if (k < I - 1 && hp) {
-                    aD = aD + ld3s(ex + EX_P * B, B, t - 1);
-                    aX = aX + ld3s(ex + (EX_P + 3) * B, B, t - 1);
-                }
                 q4 pQp{0.f, 0.f, 0.f, 0.f}, pQn = pQp, pRp = pQp;
                 f3 pD = mk3(0.f, 0.f, 0.f);
                 if (upd) {
@@ -1168,9 +1276,20 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                 stq4s(ex + (EX_O + 11) * B, B, t, pRp);
             }
             __syncthreads();
-            // -- gather the orientation pushes; Newton adjoint u = a H^-1 d̄^{k+1}
+            // -- gather the orientation pushes, and the previous iteration's segment-term pushes and pair parts;
+            //    Newton adjoint u = a H^-1 d̄^{k+1}
             f3 u = mk3(0.f, 0.f, 0.f);
             if (act) {
+                if (k < I - 1) {
+                    if (hp) {
+                        aD = aD + ld3s(ex + EX_P * B, B, t - 1);
+                        aX = aX + ld3s(ex + (EX_P + 3) * B, B, t - 1);
+                    }
+                    if (RC && nch == 1) {
+                        int qo = hp ? L.poff[t - 1] : L.poff[t], c0 = 0, c1 = 0;
+                        rc_pairs_gather<B>(L, list_row(0), t, hp, 0, L.poff[B], A.mbSlots, qo, c0, c1, aX, aD);
+                    }
+                }
                 aQ = keepQ;
                 if (hp) {
                     aQ = qadd(aQ, ldq4s(ex + (EX_O + 4) * B, B, t - 1));
@@ -1246,25 +1365,27 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                     gk += ((double)kkb + (double)ddb * (double)invl) * (double)invl;
                 }
             }
-            if (RC) rc_pairs_adj<B, F>(P, L, ex, list_row(0), t, act, hp, nch, A.mbSlots, dmp, aX, aD, pX, pD);
+            if (RC) rc_pairs_adj<B, F, !PW>(P, L, ex, list_row(0), t, act, hp, nch, A.mbSlots, dmp, aX, aD);
             if (act) {
                 st3s(ex + EX_P * B, B, t, pD);
                 st3s(ex + (EX_P + 3) * B, B, t, pX);
             }
-            __syncthreads();
-            if (RC) {
-                if (k > 0) cur = load_iter(k - 1, d0, d1);
-            } else {
-                cur = nxt;
-            }
+            // no barrier here: the pushes and the pair parts are gathered after the next orientation sweep's
+            cur = nxt;
         }
-        if (act && hp) {  // segment-term pushes of iteration 0
-            aD = aD + ld3s(ex + EX_P * B, B, t - 1);
-            aX = aX + ld3s(ex + (EX_P + 3) * B, B, t - 1);
+        __syncthreads();  // the last iteration's pushes and pair parts, every clamp adjoint of the substep
+        if (act) {
+            if (hp) {
+                aD = aD + ld3s(ex + EX_P * B, B, t - 1);
+                aX = aX + ld3s(ex + (EX_P + 3) * B, B, t - 1);
+            }
+            if (RC && nch == 1) {
+                int qo = hp ? L.poff[t - 1] : L.poff[t], c0 = 0, c1 = 0;
+                rc_pairs_gather<B>(L, list_row(0), t, hp, 0, L.poff[B], A.mbSlots, qo, c0, c1, aX, aD);
+            }
         }
         f3 aYp = mk3(0.f, 0.f, 0.f);  // the adjoint of y through the detection padding
         if (RCM != RC_NONE) {  // step-limit adjoints: limit_t = min(pad_t - detR, 0.45 |d_tj| over the listed j)
-            __syncthreads();  // every clamp adjoint of the substep is in L.mown / L.mprev
             const float Mt = L.mown[t] + ((act && hn) ? L.mprev[t + 1] : 0.f);
             L.mb[t] = Mt;
             __syncthreads();
@@ -1290,10 +1411,11 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                 for (int side = 0; RC && side < 2; side++) {
                     const int jt = t - side;
                     if (side == 1 && !hp) break;
-                    const int nr = L.nrev[jt];
-                    const uint16_t* rr = list_row(jt) + MAX_COLL;
+                    int nr;
+                    const uint16_t* rr = rc_rev_list<B>(L, list_row(0), jt, nr);
                     for (int e = 0; e < nr; e++) {
-                        const int k = rr[e];
+                        int k, jj;
+                        rc_pair_of<B>(L, list_row(0), rr[e], k, jj);
                         const float Mk = L.mb[k];
                         if (Mk == 0.f || L.arg[k] != jt) continue;
                         limit_dist_adj(ld3s(L.x, B, k), ld3s(L.x, B, k + 1), ld3s(L.x, B, jt), ld3s(L.x, B, jt + 1), Mk,
@@ -1313,7 +1435,9 @@ __global__ __launch_bounds__(B) void k_adjoint_reverse(Params P, Dev D, AdjDev A
                 const f3 g = mk3(P.gx, P.gy, P.gz);
                 const float g2 = M::len2(g);
                 if (g2 > 0.f) {
-                    const float sr = gravity_fraction<F>(P, vs, vls, g, g2);  // the forward's clamp decision
+                    // the forward's clamp decision, from the substep's velocities (loaded here, not held through
+                    // the iterations)
+                    const float sr = gravity_fraction<F>(P, ld3(A.hV[hs(A, s, li)]), ld3(A.hVL[hs(A, s, li)]), g, g2);
                     if (sr > 0.f && sr < 1.f) {
                         const f3 ab = ((h * h) * dot(g, aD) / g2) * g;
                         vb = vb + ab / P.lastH;
@@ -1479,14 +1603,21 @@ static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, con
     AdjDev Ak = A;
     Ak.mbSlots = 0;
     if (reverse && RC == RC_PAIRS) {  // the pair mailbox takes what is left, up to two slots per thread
-        const size_t used = stat[1] + lds + sizeof(int) * POFF_N(B);
+        const size_t used = stat[1] + lds + pair_index_lds_bytes(B);
         const size_t fit = used < LDS_BYTES ? (LDS_BYTES - used) / (PAIR_ADJ_N * sizeof(float)) / 64 * 64 : 0;
         Ak.mbSlots = (int)std::min<size_t>(2 * B, fit);
         if (Ak.mbSlots < 64) return hipErrorInvalidValue;
-        lds += sizeof(int) * POFF_N(B) + (size_t)Ak.mbSlots * PAIR_ADJ_N * sizeof(float);
+        // ROPESIM_ADJ_MAILBOX_SLOTS caps the slots (at least 4), so that tests can send the pairs through several
+        // mailbox chunks; the gradients do not depend on it
+        if (const char* cap = getenv("ROPESIM_ADJ_MAILBOX_SLOTS")) {
+            const int c = atoi(cap);
+            if (c >= 4) Ak.mbSlots = std::min(Ak.mbSlots, c);
+        }
+        lds += pair_index_lds_bytes(B) + (size_t)Ak.mbSlots * PAIR_ADJ_N * sizeof(float);
     }
     if (reverse)
-        hipLaunchKernelGGL((k_adjoint_reverse<B, F, RC>), dim3(nblocks), dim3(B), lds, s, P, D, Ak, Dr, S);
+        hipLaunchKernelGGL((k_adjoint_reverse<B, F, RC>), dim3(nblocks), dim3(B + (pair_waves<B, RC>() ? PAIR_LANES : 0)),
+                           lds, s, P, D, Ak, Dr, S);
     else
         hipLaunchKernelGGL((k_adjoint_forward<B, F, RC>), dim3(nblocks), dim3(B), lds, s, P, D, Ak, Dr, S);
     return hipGetLastError();

@@ -45,8 +45,8 @@ struct AdjDev {
     const Pose* fposes;     // frame-start poses [N][numSdf]
     const Motion* fmotion;  // frame motion [N][numSdf] while meshes move, else null
     // with rope-rope contact, per local vertex (adjoint_list_entries_per_vertex entries): the substep's partner
-    // list (workgroup-local slots, ascending), the list of the segments whose lists hold it (MAX_COLL each) and
-    // the skin list its candidates come from; each pass rebuilds them
+    // list (workgroup-local slots, ascending), the list of the segments whose lists hold it (MAX_COLL each), the
+    // skin list its candidates come from and the pair index of each reverse-list entry; each pass rebuilds them
     uint16_t* lists;
     // with rope-rope contact, per contact group of this frame: nonzero when some segment of the group had a
     // rope-rope pair at some substep of the recorded forward (null: treat every group so)
@@ -64,7 +64,7 @@ bool adjoint_block_supported(int block, bool reverse);  // 64, 128, 192, 256, 38
 int adjoint_waves_per_simd(int block, bool reverse, bool fast, bool ropeContact);
 // Dynamic LDS of the rope-rope detection arrays of one workgroup (beside its pose slots).
 size_t adjoint_rc_lds_bytes(int block);
-// Entries (uint16) of AdjDev::lists per local vertex: partner list, reverse list, skin list.
+// Entries (uint16) of AdjDev::lists per local vertex: partner list, reverse list, skin list, pair indices.
 size_t adjoint_list_entries_per_vertex();
 // One pass (forward recompute, or reverse) over `nblocks` workgroups of `block` threads, workgroup b on the
 // ropes [A.blk[b], A.blk[b + 1]), at most ropesPerBlock of them (their mesh poses take ropesPerBlock * numSdf

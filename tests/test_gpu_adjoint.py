@@ -329,6 +329,24 @@ def test_rope_contact_gradients_do_not_depend_on_packing(rs):
     assert np.abs(small["ea"]).max() > 0
 
 
+def test_rope_contact_gradients_do_not_depend_on_mailbox_chunks(rs, monkeypatch):
+    """The reverse pass evaluates each iteration's pair adjoints into an LDS mailbox; with more pairs than slots
+    they go through it in chunks. ROPESIM_ADJ_MAILBOX_SLOTS=4 forces chunks of four pairs (the workgroups with
+    pair waves and the 512-thread ones that evaluate on their vertex lanes): every vertex still merges its parts
+    in ascending pair index, so the gradients agree bit for bit with the one-chunk run."""
+    V = 24
+    small = _rc_groups_batch(rs, 1, V=V)
+    monkeypatch.setenv("ROPESIM_ADJ_MAILBOX_SLOTS", "4")
+    chunked = _rc_groups_batch(rs, 1, V=V)
+    big = _rc_groups_batch(rs, 8192, V=V)
+    for name in ("ea", "ei", "m"):
+        assert np.array_equal(chunked[name].view(np.uint32), small[name].view(np.uint32)), name
+        assert np.array_equal(big[name][:2 * V].view(np.uint32), small[name].view(np.uint32)), name
+    for name in ("p0", "v0", "p1"):
+        assert np.array_equal(chunked[name].view(np.uint32), small[name].view(np.uint32)), name
+        assert np.array_equal(big[name][:, :2].view(np.uint32), small[name].view(np.uint32)), name
+
+
 @pytest.mark.parametrize("fast", [True, False], ids=["fast", "ieee"])
 def test_rollout_function_gradcheck(rs, fast):
     """torch.autograd.gradcheck of the autograd.Function (ropesim.diff.rollout) on a tiny smooth case: two
