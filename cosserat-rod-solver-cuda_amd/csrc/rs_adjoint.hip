@@ -589,15 +589,15 @@ __device__ __forceinline__ void rc_detect(const Params& P, const RcLds& L, int t
     __syncthreads();
 }
 
-// Pair indexing of the reverse pass (after rc_detect): the workgroup's pairs (segment t, entry e of its list)
+// Pair indexing (after rc_detect): the workgroup's pairs (segment t, entry e of its list)
 // numbered t-major, poff[t] the first index of slot t (an exclusive prefix sum of the list lengths), the pair table
 // (owner and partner of the first PAIR_TAB_OF(B) pairs), and the reverse lists: for each segment j, the indices of the
 // pairs whose partner it is, ascending (so by owner). They are the transpose of the partner lists, filled through
 // LDS counters and then sorted; a list of up to REV_SLOTS_OF(B) entries is kept in LDS, a longer one in its row in
-// memory. Returns the number of mailbox chunks the workgroup's pairs take. Every lane calls it. The offsets are in
-// place on return; the table and the reverse lists are read only after the barriers of the iteration that
-// follows.
-template <int B>
+// memory (REV: the reverse pass). Returns the number of mailbox chunks the workgroup's pairs take. Every lane
+// calls it. The offsets and the table are in place on return; the reverse lists are read only after the barriers
+// of the iteration that follows.
+template <int B, bool REV = true>
 __device__ __forceinline__ int rc_pair_index(const RcLds& L, int t, bool act, uint16_t* lists, int slots, int* err) {
     int* wsum = L.poff + B + 1;
     const int lane = t & 63, w = t >> 6;
@@ -620,13 +620,14 @@ __device__ __forceinline__ int rc_pair_index(const RcLds& L, int t, bool act, ui
         for (int e = 0; e < v; e++) {
             const int j = row[e], q = first + e;
             if (q < PAIR_TAB_OF(B)) L.pairs[q] = (uint32_t)t | ((uint32_t)j << 16);
+            if (!REV) continue;
             const int pos = atomicAdd(&L.nrev[j], 1);
             if (pos < REV_SLOTS_OF(B)) L.rq[j * REV_SLOTS_OF(B) + pos] = (uint16_t)q;
             if (pos < MAX_COLL) lists[(size_t)j * LIST_STRIDE + REV_IDX + pos] = (uint16_t)q;
         }
     }
     __syncthreads();
-    if (act) {
+    if (REV && act) {
         int n = L.nrev[t];
         if (n > MAX_COLL) {
             *err = ERR_MAX_COLLISIONS;
@@ -768,6 +769,50 @@ __device__ __forceinline__ void rc_pairs_adj(const Params& P, const RcLds& L, co
     }
 }
 
+// Pass 1's pair terms: collision_eval of pair q (owner o, partner j, at the iterate dk) left in mailbox slot e as
+// PAIR_TERM_N floats: n [3], s1, s2, h1, h2, u [3], a1, a2, b1, b2, unused, on. The owner adds the first-vertex part
+// and vertex o + 1 the second-vertex part, each in list order, which is collision_apply's arithmetic split by
+// accumulator (its f, H and f2, H2 updates never mix).
+constexpr int PAIR_TERM_N = 16;
+template <int B, bool F>
+__device__ __forceinline__ void rc_term_eval(const Params& P, float dmp, const RcLds& L, const float* sX,
+                                             const float* dk, const uint16_t* lists, int q, int e, int slots) {
+    int o, j;
+    rc_pair_of<B>(L, lists, q, o, j);
+    const PairTerm T = collision_eval<F>(P, dmp, ld3s(sX, B, o), ld3s(sX, B, o + 1), ld3s(dk, B, o), ld3s(dk, B, o + 1),
+                                         ld3s(sX, B, j), ld3s(sX, B, j + 1), ld3s(dk, B, j), ld3s(dk, B, j + 1));
+    float* mb = L.mbx;
+    mb[15 * slots + e] = __int_as_float(T.on);
+    if (!(T.on & 1)) return;
+    st3s(mb, slots, e, T.n);
+    mb[3 * slots + e] = T.s1;
+    mb[4 * slots + e] = T.s2;
+    mb[5 * slots + e] = T.h1;
+    mb[6 * slots + e] = T.h2;
+    if (!(T.on & 2)) return;
+    st3s(mb + 7 * slots, slots, e, T.u);
+    mb[10 * slots + e] = T.a1;
+    mb[11 * slots + e] = T.a2;
+    mb[12 * slots + e] = T.b1;
+    mb[13 * slots + e] = T.b2;
+}
+// second = 0: the owner's part (f, H); 1: the part of the owner's next vertex (f2, H2)
+template <bool F>
+__device__ __forceinline__ void rc_term_apply(const float* mb, int slots, int e, int second, f3& f, s3& H) {
+    using M = V<F>;
+    const int on = __float_as_int(mb[15 * slots + e]);
+    if (!(on & 1)) return;
+    const f3 n = ld3s(mb, slots, e);
+    f = M::axpy(mb[(3 + second) * slots + e], n, f);
+    s3 op = M::souter(n);
+    M::sadd_scaled(H, op, mb[(5 + second) * slots + e]);
+    if (on & 2) {
+        sadd_diag(op, -1.f);
+        f = M::axpy(mb[(10 + second) * slots + e], ld3s(mb + 7 * slots, slots, e), f);
+        M::sadd_scaled(H, op, mb[(12 + second) * slots + e]);
+    }
+}
+
 // Lanes of the reverse pass's pair waves (threads B .. B + PAIR_LANES - 1 of a workgroup with rope-rope pairs):
 // they hold no vertex, so the pair adjoints they evaluate do not compete with the vertex lanes' state for
 // registers, and they run while the vertex lanes take their segment terms.
@@ -861,8 +906,11 @@ __device__ __forceinline__ Vc load_vc(const Dev& D, int i, bool act) {
 // the frame kernel (so the values, and every branch, are those of the recorded forward). Every
 // substep's inputs and every iteration's d^k, q^k, total H and Newton direction δ = H^-1 F go to the
 // history. The frame-end state is compared bit for bit with the next checkpoint.
+// Held at three waves per SIMD, except with rope-rope pairs: that form's LDS keeps it to one workgroup per
+// compute unit, and the registers go to the pair terms.
 template <int B, bool F, int RCM>
-__global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDev A, Drive Dr, int S) {
+__global__ __launch_bounds__(B, RCM == RC_PAIRS ? 2 : 3) void k_adjoint_forward(Params P, Dev D, AdjDev A, Drive Dr,
+                                                                               int S) {
     constexpr bool RC = RCM == RC_PAIRS;
     __shared__ float4 sQ[2][B];
     __shared__ float sX[3 * B], sD[2][3 * B], sF2[9 * B];
@@ -884,6 +932,8 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
         for (int m = t; m < nPose; m += B) sPoseAll[m] = A.fposes[(size_t)rb0 * P.numSdf + m];
     RcLds L{};
     if (RCM != RC_NONE) L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), sX);
+    uint16_t* lists0 = RC ? A.lists + (size_t)(li - t) * LIST_STRIDE : nullptr;  // the workgroup's list rows
+    const float dmp = Ar<F>::div(P.damping, h);
     if (RC) {
         L.fl[t] = act ? (int)c.fl : 0;
         L.rope[t] = r;
@@ -936,19 +986,26 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
             sQ[0][t] = stq(q);
             if (RCM != RC_NONE) L.pad[t] = padding;
         }
+        int nch = 0;  // mailbox chunks of the substep's pair terms
         if (RC) {  // rope-rope lists and step limits of the substep (the forward's detection); its vote is the barrier
             int g0, g1;
             group_range(P, D, r, v0, g0, g1);
             rc_detect<B>(P, L, t, act, c.fl, g0, g1, A.lists + (size_t)(act ? li : 0) * LIST_STRIDE, D.err, s == 0);
+            nch = rc_pair_index<B, false>(L, t, act, lists0, A.mbSlots, D.err);
         } else {
             __syncthreads();
         }
         uint64_t hitBits = 0;
+        const int ntot = RC ? L.poff[B] : 0;
         for (int it = 0; it < I; it++) {
             const float* dk = sD[it & 1];
             float* dn = sD[(it + 1) & 1];
             f3 f, f2 = mk3(0.f, 0.f, 0.f);
             s3 H, H2 = szero();
+            // with one mailbox chunk, iteration 0's pair terms are evaluated here and later ones during the
+            // previous iteration's orientation sweep
+            if (RC && nch == 1 && it == 0)
+                for (int q = t; q < ntot; q += B) rc_term_eval<B, F>(P, dmp, L, sX, dk, lists0, q, q, A.mbSlots);
             if (act) {
                 const f3 p0 = ld3s(sX, B, t), d0 = ld3s(dk, B, t);
                 f3 p1 = p0, p1dx = d0;
@@ -960,27 +1017,37 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
                 vertex_terms<F, true>(P, D.grids, sPose, nc, cand, c.w, c.l, c.kS, c.fl, p0, d0, y,
                                       ldq(sQ[it & 1][t]), p1, p1dx, f, H, f2, H2, &hit);
                 if (it < 8) hitBits |= (uint64_t)(hit & 0xFFu) << (8 * it);
-                const int ncol = RC ? L.ncol[t] : 0;
-                if (ncol > 0) {  // rope-rope terms in list order, as the forward adds them
-                    const float dmp = Ar<F>::div(P.damping, h);
-                    const uint16_t* row = A.lists + (size_t)li * LIST_STRIDE;
-                    for (int e = 0; e < ncol; e++) {
-                        const int j = row[e];
-                        collision_term<F>(P, dmp, p0, p1, d0, p1dx, ld3s(sX, B, j), ld3s(sX, B, j + 1), ld3s(dk, B, j),
-                                          ld3s(dk, B, j + 1), f, H, f2, H2);
-                    }
-                }
                 st3s(sF2, B, t, f2);
 #pragma unroll
                 for (int m = 0; m < 6; m++) sF2[(3 + m) * B + t] = H2.m[m];
             }
             __syncthreads();
+            // the next vertex's stretch terms of segment t - 1, then (rope contact) the rope-rope terms in list
+            // order as the forward adds them: segment t's to f, H, segment t - 1's to its next-vertex part
+            f3 fp = mk3(0.f, 0.f, 0.f);
+            s3 Hp = szero();
+            if (act && (c.fl & HAS_PREV)) {
+                fp = ld3s(sF2, B, t - 1);
+#pragma unroll
+                for (int m = 0; m < 6; m++) Hp.m[m] = sF2[(3 + m) * B + t - 1];
+            }
+            if (RC && nch > 0) {
+                const int o0 = act ? L.poff[t] : 0, o1 = act ? L.poff[t + 1] : 0;
+                const int pp0 = (act && (c.fl & HAS_PREV)) ? L.poff[t - 1] : o0;
+                for (int ch = 0; ch < nch; ch++) {
+                    const int q0 = ch * A.mbSlots, q1 = min(q0 + A.mbSlots, ntot);
+                    if (nch > 1) {
+                        for (int q = q0 + t; q < q1; q += B) rc_term_eval<B, F>(P, dmp, L, sX, dk, lists0, q, q - q0, A.mbSlots);
+                        __syncthreads();
+                    }
+                    for (int q = max(o0, q0); q < min(o1, q1); q++) rc_term_apply<F>(L.mbx, A.mbSlots, q - q0, 0, f, H);
+                    for (int q = max(pp0, q0); q < min(o0, q1); q++) rc_term_apply<F>(L.mbx, A.mbSlots, q - q0, 1, fp, Hp);
+                    if (ch + 1 < nch) __syncthreads();
+                }
+            }
             if (act) {
                 if (c.fl & HAS_PREV) {
-                    f = f + ld3s(sF2, B, t - 1);
-                    s3 Hp;
-#pragma unroll
-                    for (int m = 0; m < 6; m++) Hp.m[m] = sF2[(3 + m) * B + t - 1];
+                    f = f + fp;
                     sadd(H, Hp);
                 }
                 f3 d = ld3s(dk, B, t), dl = mk3(0.f, 0.f, 0.f);
@@ -1017,6 +1084,8 @@ __global__ __launch_bounds__(B, 3) void k_adjoint_forward(Params P, Dev D, AdjDe
                 sQ[(it + 1) & 1][t] = stq(qo);
                 A.hQ[hk(A, I, s, it + 1, li)] = stq(qo);
             }
+            if (RC && nch == 1 && it + 1 < I)  // the next iteration's pair terms (its d^k is in place)
+                for (int q = t; q < ntot; q += B) rc_term_eval<B, F>(P, dmp, L, sX, dn, lists0, q, q, A.mbSlots);
             // the ring slots written here were last read two barriers ago
         }
         if (act) {
@@ -1602,9 +1671,10 @@ static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, con
     }
     AdjDev Ak = A;
     Ak.mbSlots = 0;
-    if (reverse && RC == RC_PAIRS) {  // the pair mailbox takes what is left, up to two slots per thread
-        const size_t used = stat[1] + lds + pair_index_lds_bytes(B);
-        const size_t fit = used < LDS_BYTES ? (LDS_BYTES - used) / (PAIR_ADJ_N * sizeof(float)) / 64 * 64 : 0;
+    if (RC == RC_PAIRS) {  // the pair mailbox takes what is left, up to two slots per thread
+        const size_t per = (reverse ? PAIR_ADJ_N : PAIR_TERM_N) * sizeof(float);
+        const size_t used = stat[reverse] + lds + pair_index_lds_bytes(B);
+        const size_t fit = used < LDS_BYTES ? (LDS_BYTES - used) / per / 64 * 64 : 0;
         Ak.mbSlots = (int)std::min<size_t>(2 * B, fit);
         if (Ak.mbSlots < 64) return hipErrorInvalidValue;
         // ROPESIM_ADJ_MAILBOX_SLOTS caps the slots (at least 4), so that tests can send the pairs through several
@@ -1613,7 +1683,7 @@ static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, con
             const int c = atoi(cap);
             if (c >= 4) Ak.mbSlots = std::min(Ak.mbSlots, c);
         }
-        lds += pair_index_lds_bytes(B) + (size_t)Ak.mbSlots * PAIR_ADJ_N * sizeof(float);
+        lds += pair_index_lds_bytes(B) + (size_t)Ak.mbSlots * per;
     }
     if (reverse)
         hipLaunchKernelGGL((k_adjoint_reverse<B, F, RC>), dim3(nblocks), dim3(B + (pair_waves<B, RC>() ? PAIR_LANES : 0)),

@@ -51,7 +51,7 @@ struct AdjDev {
     // with rope-rope contact, per contact group of this frame: nonzero when some segment of the group had a
     // rope-rope pair at some substep of the recorded forward (null: treat every group so)
     const int* pairFlag;
-    int mbSlots;  // (set by the launcher) pair-mailbox slots of a reverse-pass workgroup with rope-rope pairs
+    int mbSlots;  // (set by the launcher) pair-mailbox slots of a workgroup with rope-rope pairs
 };
 
 // Longest rope (or rope contact group) the backward kernels support; pose slots a workgroup keeps in LDS (its
