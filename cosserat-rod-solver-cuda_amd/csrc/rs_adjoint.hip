@@ -1026,7 +1026,7 @@ __global__ __launch_bounds__(B, RCM == RC_PAIRS ? 2 : 3) void k_adjoint_forward(
             // order as the forward adds them: segment t's to f, H, segment t - 1's to its next-vertex part
             f3 fp = mk3(0.f, 0.f, 0.f);
             s3 Hp = szero();
-            if (act && (c.fl & HAS_PREV)) {
+            if (RC && act && (c.fl & HAS_PREV)) {
                 fp = ld3s(sF2, B, t - 1);
 #pragma unroll
                 for (int m = 0; m < 6; m++) Hp.m[m] = sF2[(3 + m) * B + t - 1];
@@ -1047,6 +1047,11 @@ __global__ __launch_bounds__(B, RCM == RC_PAIRS ? 2 : 3) void k_adjoint_forward(
             }
             if (act) {
                 if (c.fl & HAS_PREV) {
+                    if (!RC) {
+                        fp = ld3s(sF2, B, t - 1);
+#pragma unroll
+                        for (int m = 0; m < 6; m++) Hp.m[m] = sF2[(3 + m) * B + t - 1];
+                    }
                     f = f + fp;
                     sadd(H, Hp);
                 }
@@ -1200,7 +1205,7 @@ __global__ __launch_bounds__(B + (pair_waves<B, RCM>() ? PAIR_LANES : 0)) void k
         if (P.numSdf > 0 && P.sdfMoving)
             for (int m = t; m < nPose; m += B)
                 sPoseAll[m] = substep_pose(A.fmotion[(size_t)rb0 * P.numSdf + m], D.grids[m % P.numSdf], s, S, h);
-        f3 X0 = mk3(0.f, 0.f, 0.f), X1 = X0, y = X0;
+        f3 X0 = mk3(0.f, 0.f, 0.f), X1 = X0, y = X0, vs = X0, vls = X0;
         int nc = 0;
         uint64_t cand = 0, hitMask = ~0ull;
         if (act) {
@@ -1209,6 +1214,10 @@ __global__ __launch_bounds__(B + (pair_waves<B, RCM>() ? PAIR_LANES : 0)) void k
             nc = __float_as_int(xs.w);
             if (hn) X1 = ld3(A.hX[hs(A, s, li + 1)]);
             y = ld3(A.hY[hs(A, s, li)]);
+            if (!RC) {  // (with rope-rope pairs they are loaded where used, not held through the iterations)
+                vs = ld3(A.hV[hs(A, s, li)]);
+                vls = ld3(A.hVL[hs(A, s, li)]);
+            }
             const uint2 cc = A.hC[hs(A, s, li)];
             cand = (uint64_t)cc.x | ((uint64_t)cc.y << 32);
             const uint2 mm = A.hM[hs(A, s, li)];
@@ -1504,9 +1513,11 @@ __global__ __launch_bounds__(B + (pair_waves<B, RCM>() ? PAIR_LANES : 0)) void k
                 const f3 g = mk3(P.gx, P.gy, P.gz);
                 const float g2 = M::len2(g);
                 if (g2 > 0.f) {
-                    // the forward's clamp decision, from the substep's velocities (loaded here, not held through
-                    // the iterations)
-                    const float sr = gravity_fraction<F>(P, ld3(A.hV[hs(A, s, li)]), ld3(A.hVL[hs(A, s, li)]), g, g2);
+                    if (RC) {
+                        vs = ld3(A.hV[hs(A, s, li)]);
+                        vls = ld3(A.hVL[hs(A, s, li)]);
+                    }
+                    const float sr = gravity_fraction<F>(P, vs, vls, g, g2);  // the forward's clamp decision
                     if (sr > 0.f && sr < 1.f) {
                         const f3 ab = ((h * h) * dot(g, aD) / g2) * g;
                         vb = vb + ab / P.lastH;
