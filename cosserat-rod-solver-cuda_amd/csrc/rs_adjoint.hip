@@ -589,6 +589,51 @@ __device__ __forceinline__ void rc_detect(const Params& P, const RcLds& L, int t
     __syncthreads();
 }
 
+// Pass 1 records each substep's lists for pass 2 in one word per vertex: x the step limit, y the limit's partner
+// (group-relative, 0xFFFF: the padding) and the list length << 16, z and w up to TAPE_PARTNERS partners
+// (group-relative, 16 bits each). A segment with a longer list makes pass 2 detect that substep itself.
+constexpr int TAPE_PARTNERS = 4;
+__device__ __forceinline__ uint4 rc_tape_word(const RcLds& L, const uint16_t* row, int t, int g0) {
+    const int n = L.ncol[t], a = L.arg[t];
+    uint32_t pr[TAPE_PARTNERS] = {0u, 0u, 0u, 0u};
+    for (int e = 0; e < min(n, TAPE_PARTNERS); e++) pr[e] = (uint32_t)(row[e] - g0);
+    return make_uint4(__float_as_uint(L.lim[t]), (uint32_t)(a < 0 ? 0xFFFF : a - g0) | ((uint32_t)n << 16),
+                      pr[0] | (pr[1] << 16), pr[2] | (pr[3] << 16));
+}
+
+// Pass 2's lists of a substep: from the tape words pass 1 recorded, or, when some segment of the workgroup has
+// more than tapeMax partners, by the detection itself (rc_detect: the same lists and limits). Its skin lists are
+// built at the first such substep (skinBuilt) and rebuilt when stale; they stay valid across the substeps read
+// from the tape, since staleness is measured from the build. Every lane calls it; returns whether it detected
+// (the same for every lane). The lists, counts and limits are in place after it.
+template <int B>
+__device__ __forceinline__ bool rc_lists_from_tape(const Params& P, const RcLds& L, int t, bool act, uint32_t fl,
+                                                   int g0, int g1, uint16_t* row, uint4 w, int tapeMax,
+                                                   bool& skinBuilt, int* err) {
+    const int n = (int)(w.y >> 16);
+    if (__syncthreads_or(act && n > tapeMax)) {
+        rc_detect<B>(P, L, t, act, fl, g0, g1, row, err, !skinBuilt);
+        skinBuilt = true;
+        return true;
+    }
+    if (act) {
+        L.ncol[t] = n;
+        L.nrev[t] = 0;
+        L.lim[t] = __uint_as_float(w.x);
+        const uint32_t a = w.y & 0xFFFFu;
+        L.arg[t] = a == 0xFFFFu ? -1 : g0 + (int)a;
+        const uint32_t pr[TAPE_PARTNERS] = {w.z & 0xFFFFu, w.z >> 16, w.w & 0xFFFFu, w.w >> 16};
+        for (int e = 0; e < n; e++) row[e] = (uint16_t)(g0 + (int)pr[e]);
+    } else {
+        L.ncol[t] = 0;
+        L.nrev[t] = 0;
+        L.lim[t] = INFINITY;
+        L.arg[t] = -1;
+    }
+    __syncthreads();
+    return false;
+}
+
 // Pair indexing (after rc_detect): the workgroup's pairs (segment t, entry e of its list)
 // numbered t-major, poff[t] the first index of slot t (an exclusive prefix sum of the list lengths), the pair table
 // (owner and partner of the first PAIR_TAB_OF(B) pairs), and the reverse lists: for each segment j, the indices of the
@@ -821,7 +866,7 @@ template <int B, int RCM>
 constexpr bool pair_waves() { return RCM == RC_PAIRS && B <= 384; }
 
 // The pair waves' side of the reverse pass: the same sequence of barriers as the vertex lanes (substep head,
-// detection vote and rebuild, end of detection, the two of the pair indexing; per iteration the orientation
+// the lists' vote, with a detection its vote and rebuild, the lists' end, the two of the pair indexing; per iteration the orientation
 // sweep, the Newton adjoint and, with more than one mailbox chunk, the chunks'; after the iterations the two of
 // the step-limit adjoints; the two of the drag sum), evaluating each iteration's pairs after the Newton adjoint's
 // barrier.
@@ -832,8 +877,10 @@ __device__ void rc_pair_waves(const Params& P, const RcLds& L, const float* ex, 
     const float dmp = Ar<F>::div(P.damping, P.h);
     for (int s = S - 1; s >= 0; s--) {
         __syncthreads();  // substep head
-        if (__syncthreads_or(0)) __syncthreads();  // detection: rebuild vote, skin boxes
-        __syncthreads();  // detection: lists in place
+        if (__syncthreads_or(0)) {  // lists: the vertex lanes detect this substep (rc_detect's barriers)
+            if (__syncthreads_or(0)) __syncthreads();
+        }
+        __syncthreads();  // lists in place
         __syncthreads();  // pair indexing: offsets
         __syncthreads();  // pair indexing: reverse lists
         const int ntot = L.poff[B];
@@ -991,6 +1038,7 @@ __global__ __launch_bounds__(B, RCM == RC_PAIRS ? 2 : 3) void k_adjoint_forward(
             int g0, g1;
             group_range(P, D, r, v0, g0, g1);
             rc_detect<B>(P, L, t, act, c.fl, g0, g1, A.lists + (size_t)(act ? li : 0) * LIST_STRIDE, D.err, s == 0);
+            if (act) A.hR[hs(A, s, li)] = rc_tape_word(L, A.lists + (size_t)li * LIST_STRIDE, t, g0);
             nch = rc_pair_index<B, false>(L, t, act, lists0, A.mbSlots, D.err);
         } else {
             __syncthreads();
@@ -1200,6 +1248,7 @@ __global__ __launch_bounds__(B + (pair_waves<B, RCM>() ? PAIR_LANES : 0)) void k
     // list rows of slot k (the workgroup's vertices are consecutive local vertices): partners, then reverse
     auto list_row = [&](int k) { return A.lists + (size_t)(li - t + k) * LIST_STRIDE; };
 
+    bool skinBuilt = false;  // (rope-rope pairs) pass 2's own skin lists exist
     for (int s = S - 1; s >= 0; s--) {
         __syncthreads();  // sPose and the exchange are free
         if (P.numSdf > 0 && P.sdfMoving)
@@ -1244,7 +1293,8 @@ __global__ __launch_bounds__(B + (pair_waves<B, RCM>() ? PAIR_LANES : 0)) void k
         if (RC) {  // the substep's rope-rope lists and step limits, with the lists of the segments listing each one
             int g0, g1;
             group_range(P, D, r, v0, g0, g1);
-            rc_detect<B>(P, L, t, act, c.fl, g0, g1, list_row(act ? t : 0), D.err, s == S - 1);
+            const uint4 w = act ? A.hR[hs(A, s, li)] : make_uint4(0u, 0xFFFFu, 0u, 0u);
+            rc_lists_from_tape<B>(P, L, t, act, c.fl, g0, g1, list_row(act ? t : 0), w, A.tapeMax, skinBuilt, D.err);
             nch = rc_pair_index<B>(L, t, act, list_row(0), A.mbSlots, D.err);
         } else {
             __syncthreads();  // sPose of a moving frame is in place
@@ -1682,6 +1732,10 @@ static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, con
     }
     AdjDev Ak = A;
     Ak.mbSlots = 0;
+    // ROPESIM_ADJ_TAPE_PARTNERS (0..4) lowers the list length pass 2 reads from pass 1's record, so that tests can
+    // send substeps through pass 2's own detection; the gradients do not depend on it
+    Ak.tapeMax = TAPE_PARTNERS;
+    if (const char* tp = getenv("ROPESIM_ADJ_TAPE_PARTNERS")) Ak.tapeMax = std::max(0, std::min(TAPE_PARTNERS, atoi(tp)));
     if (RC == RC_PAIRS) {  // the pair mailbox takes what is left, up to two slots per thread
         const size_t per = (reverse ? PAIR_ADJ_N : PAIR_TERM_N) * sizeof(float);
         const size_t used = stat[reverse] + lds + pair_index_lds_bytes(B);

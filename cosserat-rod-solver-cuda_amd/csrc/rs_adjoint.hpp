@@ -29,6 +29,8 @@ struct AdjDev {
     uint2* hC;                   // [S][hn] packed broad-phase candidate ids
     uint2* hM;                   // [S][hn] mesh candidates in contact: bit 8k + c for iteration k < 8 and
                                  // candidate c (all bits set on scan-all substeps and past 8 iterations)
+    uint4* hR;                   // [S][hn] (rope-rope pairs) the substep's step limit, limit partner and list of
+                                 // the vertex's segment: rc_tape_word
     float4 *hD, *hQ;             // [S][I + 1][hn] iterates d^k and frames q^k
     float* hH;                   // [S][I][9][hn] total Hessian (packed 6) and Newton direction H^-1 F (3)
     // adjoint of the state at the end of the frame on entry, at its start on exit [nv]
@@ -52,6 +54,7 @@ struct AdjDev {
     // rope-rope pair at some substep of the recorded forward (null: treat every group so)
     const int* pairFlag;
     int mbSlots;  // (set by the launcher) pair-mailbox slots of a workgroup with rope-rope pairs
+    int tapeMax;  // (set by the launcher) longest list pass 2 takes from pass 1's record (at most 4)
 };
 
 // Longest rope (or rope contact group) the backward kernels support; pose slots a workgroup keeps in LDS (its

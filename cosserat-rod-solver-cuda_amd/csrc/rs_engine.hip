@@ -1959,7 +1959,8 @@ int rs_adjoint(rs_batch* b, const float* dl_dx, float* g_ea, float* g_ei, float*
                 A.hQ = A.hD + Sn * (I + 1);
                 A.hC = reinterpret_cast<uint2*>(A.hQ + Sn * (I + 1));
                 A.hM = A.hC + Sn;
-                A.hH = reinterpret_cast<float*>(A.hM + Sn);
+                A.hR = reinterpret_cast<uint4*>(A.hM + Sn);
+                A.hH = reinterpret_cast<float*>(A.hR + Sn);
                 A.ax = b->d_adj;
                 A.av = A.ax + n;
                 A.avl = A.av + n;

@@ -347,6 +347,19 @@ def test_rope_contact_gradients_do_not_depend_on_mailbox_chunks(rs, monkeypatch)
         assert np.array_equal(big[name][:, :2].view(np.uint32), small[name].view(np.uint32)), name
 
 
+def test_rope_contact_gradients_do_not_depend_on_the_list_record(rs, monkeypatch):
+    """Pass 1 records each substep's rope-rope lists (up to four partners per segment) and pass 2 reads them; a
+    longer list makes pass 2 run the exact tests itself. ROPESIM_ADJ_TAPE_PARTNERS=0 sends every substep with a
+    pair through pass 2's own detection: the lists, and so the gradients, are the same bit for bit."""
+    V = 24
+    taped = _rc_groups_batch(rs, 1, V=V)
+    monkeypatch.setenv("ROPESIM_ADJ_TAPE_PARTNERS", "0")
+    detected = _rc_groups_batch(rs, 1, V=V)
+    for name in ("ea", "ei", "m", "p0", "v0", "p1"):
+        assert np.array_equal(detected[name].view(np.uint32), taped[name].view(np.uint32)), name
+    assert np.abs(taped["ea"]).max() > 0
+
+
 @pytest.mark.parametrize("fast", [True, False], ids=["fast", "ieee"])
 def test_rollout_function_gradcheck(rs, fast):
     """torch.autograd.gradcheck of the autograd.Function (ropesim.diff.rollout) on a tiny smooth case: two
