@@ -589,16 +589,17 @@ __device__ __forceinline__ void rc_detect(const Params& P, const RcLds& L, int t
     __syncthreads();
 }
 
-// Pass 1 records each substep's lists for pass 2 in one word per vertex: x the step limit, y the limit's partner
-// (group-relative, 0xFFFF: the padding) and the list length << 16, z and w up to TAPE_PARTNERS partners
-// (group-relative, 16 bits each). A segment with a longer list makes pass 2 detect that substep itself.
-constexpr int TAPE_PARTNERS = 4;
+// Pass 1 records each substep's lists for pass 2 in one word per vertex, slots relative to the contact group's
+// first (groups hold at most 512 vertices: 10 bits): x the step limit; y the limit's partner (0x3FF: the padding)
+// in bits 0-9, the list length in bits 10-17 and partner 0 in bits 18-27; z and w partners 1-3 and 4-6, 10 bits
+// each. A segment with a longer list makes pass 2 detect that substep itself.
+constexpr int TAPE_PARTNERS = 7;
 __device__ __forceinline__ uint4 rc_tape_word(const RcLds& L, const uint16_t* row, int t, int g0) {
     const int n = L.ncol[t], a = L.arg[t];
-    uint32_t pr[TAPE_PARTNERS] = {0u, 0u, 0u, 0u};
+    uint32_t pr[TAPE_PARTNERS] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
     for (int e = 0; e < min(n, TAPE_PARTNERS); e++) pr[e] = (uint32_t)(row[e] - g0);
-    return make_uint4(__float_as_uint(L.lim[t]), (uint32_t)(a < 0 ? 0xFFFF : a - g0) | ((uint32_t)n << 16),
-                      pr[0] | (pr[1] << 16), pr[2] | (pr[3] << 16));
+    return make_uint4(__float_as_uint(L.lim[t]), (uint32_t)(a < 0 ? 0x3FF : a - g0) | ((uint32_t)n << 10) | (pr[0] << 18),
+                      pr[1] | (pr[2] << 10) | (pr[3] << 20), pr[4] | (pr[5] << 10) | (pr[6] << 20));
 }
 
 // Pass 2's lists of a substep: from the tape words pass 1 recorded, or, when some segment of the workgroup has
@@ -610,7 +611,7 @@ template <int B>
 __device__ __forceinline__ bool rc_lists_from_tape(const Params& P, const RcLds& L, int t, bool act, uint32_t fl,
                                                    int g0, int g1, uint16_t* row, uint4 w, int tapeMax,
                                                    bool& skinBuilt, int* err) {
-    const int n = (int)(w.y >> 16);
+    const int n = (int)((w.y >> 10) & 0xFFu);
     if (__syncthreads_or(act && n > tapeMax)) {
         rc_detect<B>(P, L, t, act, fl, g0, g1, row, err, !skinBuilt);
         skinBuilt = true;
@@ -620,9 +621,10 @@ __device__ __forceinline__ bool rc_lists_from_tape(const Params& P, const RcLds&
         L.ncol[t] = n;
         L.nrev[t] = 0;
         L.lim[t] = __uint_as_float(w.x);
-        const uint32_t a = w.y & 0xFFFFu;
-        L.arg[t] = a == 0xFFFFu ? -1 : g0 + (int)a;
-        const uint32_t pr[TAPE_PARTNERS] = {w.z & 0xFFFFu, w.z >> 16, w.w & 0xFFFFu, w.w >> 16};
+        const uint32_t a = w.y & 0x3FFu;
+        L.arg[t] = a == 0x3FFu ? -1 : g0 + (int)a;
+        const uint32_t pr[TAPE_PARTNERS] = {(w.y >> 18) & 0x3FFu, w.z & 0x3FFu, (w.z >> 10) & 0x3FFu, (w.z >> 20) & 0x3FFu,
+                                            w.w & 0x3FFu, (w.w >> 10) & 0x3FFu, (w.w >> 20) & 0x3FFu};
         for (int e = 0; e < n; e++) row[e] = (uint16_t)(g0 + (int)pr[e]);
     } else {
         L.ncol[t] = 0;
@@ -1293,7 +1295,7 @@ __global__ __launch_bounds__(B + (pair_waves<B, RCM>() ? PAIR_LANES : 0)) void k
         if (RC) {  // the substep's rope-rope lists and step limits, with the lists of the segments listing each one
             int g0, g1;
             group_range(P, D, r, v0, g0, g1);
-            const uint4 w = act ? A.hR[hs(A, s, li)] : make_uint4(0u, 0xFFFFu, 0u, 0u);
+            const uint4 w = act ? A.hR[hs(A, s, li)] : make_uint4(0u, 0x3FFu, 0u, 0u);
             rc_lists_from_tape<B>(P, L, t, act, c.fl, g0, g1, list_row(act ? t : 0), w, A.tapeMax, skinBuilt, D.err);
             nch = rc_pair_index<B>(L, t, act, list_row(0), A.mbSlots, D.err);
         } else {
@@ -1732,7 +1734,7 @@ static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, con
     }
     AdjDev Ak = A;
     Ak.mbSlots = 0;
-    // ROPESIM_ADJ_TAPE_PARTNERS (0..4) lowers the list length pass 2 reads from pass 1's record, so that tests can
+    // ROPESIM_ADJ_TAPE_PARTNERS (0..7) lowers the list length pass 2 reads from pass 1's record, so that tests can
     // send substeps through pass 2's own detection; the gradients do not depend on it
     Ak.tapeMax = TAPE_PARTNERS;
     if (const char* tp = getenv("ROPESIM_ADJ_TAPE_PARTNERS")) Ak.tapeMax = std::max(0, std::min(TAPE_PARTNERS, atoi(tp)));

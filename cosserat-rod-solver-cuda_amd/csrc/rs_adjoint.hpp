@@ -54,7 +54,7 @@ struct AdjDev {
     // rope-rope pair at some substep of the recorded forward (null: treat every group so)
     const int* pairFlag;
     int mbSlots;  // (set by the launcher) pair-mailbox slots of a workgroup with rope-rope pairs
-    int tapeMax;  // (set by the launcher) longest list pass 2 takes from pass 1's record (at most 4)
+    int tapeMax;  // (set by the launcher) longest list pass 2 takes from pass 1's record (at most 7)
 };
 
 // Longest rope (or rope contact group) the backward kernels support; pose slots a workgroup keeps in LDS (its

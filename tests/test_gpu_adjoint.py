@@ -348,7 +348,7 @@ def test_rope_contact_gradients_do_not_depend_on_mailbox_chunks(rs, monkeypatch)
 
 
 def test_rope_contact_gradients_do_not_depend_on_the_list_record(rs, monkeypatch):
-    """Pass 1 records each substep's rope-rope lists (up to four partners per segment) and pass 2 reads them; a
+    """Pass 1 records each substep's rope-rope lists (up to seven partners per segment) and pass 2 reads them; a
     longer list makes pass 2 run the exact tests itself. ROPESIM_ADJ_TAPE_PARTNERS=0 sends every substep with a
     pair through pass 2's own detection: the lists, and so the gradients, are the same bit for bit."""
     V = 24
