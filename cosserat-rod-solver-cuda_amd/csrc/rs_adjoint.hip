@@ -1189,8 +1189,9 @@ __global__ __launch_bounds__(B, RCM == RC_PAIRS ? 2 : 3) void k_adjoint_forward(
 }
 
 // Pass 2: backwards through the frame of one rope from the history (no recompute of the iterations).
-// Per iteration three phases: orientation adjoint (pushes to the neighbours), Newton adjoint u = a H^-1 d̄,
-// segment-term adjoint (pushes to the next vertex).
+// Per iteration three phases and two barriers: orientation adjoint (pushes to the neighbours), Newton adjoint
+// u = a H^-1 d̄ (after gathering the orientation pushes, and the previous iteration's segment-term pushes and
+// pair parts), segment-term adjoint (pushes to the next vertex, gathered after the next orientation sweep).
 template <int B, bool F, int RCM>
 __global__ __launch_bounds__(B + (pair_waves<B, RCM>() ? PAIR_LANES : 0)) void k_adjoint_reverse(Params P, Dev D,
                                                                                                   AdjDev A, Drive Dr,
