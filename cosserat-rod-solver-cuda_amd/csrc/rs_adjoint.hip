@@ -426,6 +426,19 @@ __device__ __forceinline__ RcLds rc_lds(char* base, float* x) {
     return L;
 }
 
+// The clamp-only kernels' arrays (no detection, no pairs): paddings and the step-limit adjoints, [B] each.
+template <int B>
+__device__ __forceinline__ RcLds rc_lds_clamp(char* base) {
+    RcLds L{};
+    float* f = reinterpret_cast<float*>(base);
+    L.pad = f;
+    L.mb = f + B;
+    L.mown = f + 2 * B;
+    L.mprev = f + 3 * B;
+    return L;
+}
+constexpr size_t rc_lds_clamp_bytes(int B) { return sizeof(float) * 4 * (size_t)B; }
+
 __device__ __forceinline__ bool box_meets(const float* bx, int stride, int k, f3 lo, f3 hi) {
     return !(bx[k] > hi.x || bx[stride + k] > hi.y || bx[2 * stride + k] > hi.z || bx[3 * stride + k] < lo.x ||
              bx[4 * stride + k] < lo.y || bx[5 * stride + k] < lo.z);
@@ -980,7 +993,8 @@ __global__ __launch_bounds__(B, RCM == RC_PAIRS ? 2 : 3) void k_adjoint_forward(
     if (P.numSdf > 0 && !P.sdfMoving)
         for (int m = t; m < nPose; m += B) sPoseAll[m] = A.fposes[(size_t)rb0 * P.numSdf + m];
     RcLds L{};
-    if (RCM != RC_NONE) L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), sX);
+    if (RCM == RC_PAIRS) L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), sX);
+    if (RCM == RC_CLAMP) L = rc_lds_clamp<B>(reinterpret_cast<char*>(sPoseAll + nPose));
     uint16_t* lists0 = RC ? A.lists + (size_t)(li - t) * LIST_STRIDE : nullptr;  // the workgroup's list rows
     const float dmp = Ar<F>::div(P.damping, h);
     if (RC) {
@@ -1243,7 +1257,8 @@ __global__ __launch_bounds__(B + (pair_waves<B, RCM>() ? PAIR_LANES : 0)) void k
     const bool hn = (c.fl & HAS_NEXT) != 0, hp = (c.fl & HAS_PREV) != 0;
     const q4 EX{1.f, 0.f, 0.f, 0.f};
     RcLds L{};
-    if (RCM != RC_NONE) L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), nullptr);
+    if (RCM == RC_PAIRS) L = rc_lds<B>(reinterpret_cast<char*>(sPoseAll + nPose), nullptr);
+    if (RCM == RC_CLAMP) L = rc_lds_clamp<B>(reinterpret_cast<char*>(sPoseAll + nPose));
     if (RC) {
         L.fl[t] = act ? (int)c.fl : 0;
         L.rope[t] = r;
@@ -1733,6 +1748,7 @@ static hipError_t launch_pass_f(bool reverse, const Params& P, const Dev& D, con
         if (e != hipSuccess) return e;
         stat[reverse] = a.sharedSizeBytes > 0 ? a.sharedSizeBytes : 1;
     }
+    if (RC == RC_CLAMP) lds = lds - adjoint_rc_lds_bytes(B) + rc_lds_clamp_bytes(B);  // the compact layout
     AdjDev Ak = A;
     Ak.mbSlots = 0;
     // ROPESIM_ADJ_TAPE_PARTNERS (0..7) lowers the list length pass 2 reads from pass 1's record, so that tests can
