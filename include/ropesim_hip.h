@@ -230,9 +230,10 @@ int rs_set_chunk_stats(rs_batch* b, int on, long long* out4);
  * A recorded rollout keeps one checkpoint per frame (x, v, v_last, q, the staged root program and the mesh
  * poses). rs_adjoint recomputes each frame from its checkpoint and runs the hand-written adjoint of the
  * substep backwards. The forward active set is frozen: contact on/off, friction branch, gravity clamp,
- * orientation signs and broad-phase lists. Covered: root program (or constant drive), init, solver iterations,
- * end update with linear drag, plane and SDF contact. Not covered: rope-rope contact and quadratic drag;
- * recording refuses both.
+ * orientation signs, broad-phase lists and each substep's rope-rope lists. Covered: root program (or constant
+ * drive), init, solver iterations, end update with linear drag, plane and SDF contact, rope-rope contact
+ * (barrier, friction, step limit). Not covered: quadratic drag; recording refuses it, and ropes or rope-rope
+ * contact groups above 512 vertices (RS_INVALID).
  *
  * rs_set_material sets per-vertex EA [N], EI [N m^2] (per segment, entry at a rope's last vertex ignored) and
  * node masses [kg] (pinned entries ignored). Any pointer may be null to keep that field. Unlike the
