@@ -838,13 +838,18 @@ __global__ __launch_bounds__(BLOCK) void k_cell(Params P, Dev D, Drive Dr, const
             }
 
             // numItr x (position sweep, orientation sweep). A wave with rope-rope pairs this substep issues ahead
-            // of the others on its SIMD: the barriers wait for it.
-            if (P.ropeContact) {
-                bool pairs = false;
+            // of the others on its SIMD, and one with mesh candidates ahead of the rest: the barriers wait for them.
+            {
+                bool pairs = false, meshes = false;
 #pragma unroll
-                for (int k = 0; k < VPT; k++) pairs = pairs || ncol[k] > 0;
+                for (int k = 0; k < VPT; k++) {
+                    pairs = pairs || (P.ropeContact && ncol[k] > 0);
+                    meshes = meshes || (gi[k] >= 0 && nc[k] != 0);
+                }
                 if (__any(pairs))
                     __builtin_amdgcn_s_setprio(2);
+                else if (__any(meshes))
+                    __builtin_amdgcn_s_setprio(1);
                 else
                     __builtin_amdgcn_s_setprio(0);
             }
