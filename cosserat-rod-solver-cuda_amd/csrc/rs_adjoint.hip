@@ -126,7 +126,8 @@ __device__ __forceinline__ float surfaces_adj(const Params& P, const Grid* grids
             const f3 pp = mk3(a.x, a.y, a.z);
             float phi;
             f3 gr, mx;
-            if (!sdf_sample_t<true, F>(grids + m, M::qrot(qconj(q), (p - pp) + d), phi, gr, mx)) continue;
+            const Grid g = load_grid(grids + m);
+            if (!sdf_sample_t<true, F>(g, M::qrot(qconj(q), (p - pp) + d), phi, gr, mx)) continue;
             const float gap = phi - P.cRadius;
             if (gap >= 0.f) continue;
             const float gl = M::len(gr);
@@ -141,7 +142,7 @@ __device__ __forceinline__ float surfaces_adj(const Params& P, const Grid* grids
             }
             f3 dRb, nb;
             float gapb, kb;
-            surface_contact_adj<F>(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, fb, u, dl, dRb, nb, gapb, kb);
+            surface_contact_adj<F>(P, n, gap, dRel, k, g.muS, g.muK, fb, u, dl, dRb, nb, gapb, kb);
             db = db + dRb;
             if (P.sdfMoving) xb = xb + P.h * cross(om, dRb);
             const f3 ghb = M::qrot(qconj(q), nb);

@@ -102,24 +102,51 @@ __device__ __forceinline__ void surface_contact(const Params& P, f3 n, float gap
     M::sadd_scaled(H, nn, -kt);
 }
 
+// The header of a lattice as the three 16-byte rows a sample reads, loaded together. Read field by field, the
+// compiler moved the loads of fields needed only past the short-circuit bounds test (the sizes, the node
+// pointer, the spacing, the friction) below it: a contact waited on three dependent rounds of header loads
+// before the node values and two after them. Here they are one round, issued with the pose.
+__device__ __forceinline__ Grid load_grid(const Grid* gp) {
+    const uint4* r = reinterpret_cast<const uint4*>(gp);
+    const uint4 a = r[0], b = r[1], c = r[2];
+    Grid g;
+    g.v = reinterpret_cast<const float*>((uint64_t)a.x | ((uint64_t)a.y << 32));
+    g.nx = (int)a.z;
+    g.ny = (int)a.w;
+    g.nz = (int)b.x;
+    g.ox = __uint_as_float(b.y);
+    g.oy = __uint_as_float(b.z);
+    g.oz = __uint_as_float(b.w);
+    g.invH = __uint_as_float(c.x);
+    g.h = __uint_as_float(c.y);
+    g.muS = __uint_as_float(c.z);
+    g.muK = __uint_as_float(c.w);
+    g.hx = g.hy = g.hz = g.pad_ = 0.f;  // not loaded
+    return g;
+}
+
 // Trilinear SDF sample and the exact gradient of the interpolant (YB/sim/contact.cuh:57-76). With MIXED the
 // interpolant's mixed second derivatives (d2/dxdy, d2/dxdz, d2/dydz; the pure ones vanish) are returned too.
+// `g` is a header from load_grid.
 template <bool MIXED, bool F>
-__device__ __forceinline__ bool sdf_sample_t(const Grid* gp, f3 p, float& phi, f3& grad, f3& mixed) {
+__device__ __forceinline__ bool sdf_sample_t(const Grid& g, f3 p, float& phi, f3& grad, f3& mixed) {
     using A = Ar<F>;
-    const f3 o = mk3(gp->ox, gp->oy, gp->oz);
-    const float invH = gp->invH;
-    const int nx = gp->nx, ny = gp->ny, nz = gp->nz;
+    const f3 o = mk3(g.ox, g.oy, g.oz);
+    const float invH = g.invH;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
     const f3 u = (p - o) * invH;
     if (!(u.x >= 0.f && u.y >= 0.f && u.z >= 0.f && u.x < float(nx - 1) && u.y < float(ny - 1) && u.z < float(nz - 1)))
         return false;
     const int i = (int)u.x, j = (int)u.y, k = (int)u.z;
-    const f3 fr = (p - (o + mk3(float(i), float(j), float(k)) * gp->h)) * invH;
+    const f3 fr = (p - (o + mk3(float(i), float(j), float(k)) * g.h)) * invH;
     const float fx = fminf(fmaxf(fr.x, 0.f), 1.f), fy = fminf(fmaxf(fr.y, 0.f), 1.f), fz = fminf(fmaxf(fr.z, 0.f), 1.f);
     const int sy = nz, sx = ny * nz;
-    const float* c = gp->v + (size_t)i * sx + (size_t)j * sy + k;
-    const float c000 = __ldg(c), c001 = __ldg(c + 1), c010 = __ldg(c + sy), c011 = __ldg(c + sy + 1);
-    const float c100 = __ldg(c + sx), c101 = __ldg(c + sx + 1), c110 = __ldg(c + sx + sy), c111 = __ldg(c + sx + sy + 1);
+    // The node values are device memory: global loads, which wait on the vector-memory counter only (a
+    // generic pointer made them flat loads, which also wait on the LDS counter).
+    const __attribute__((address_space(1))) float* c =
+        (const __attribute__((address_space(1))) float*)g.v + (size_t)i * sx + (size_t)j * sy + k;
+    const float c000 = c[0], c001 = c[1], c010 = c[sy], c011 = c[sy + 1];
+    const float c100 = c[sx], c101 = c[sx + 1], c110 = c[sx + sy], c111 = c[sx + sy + 1];
     const float dx00 = c100 - c000, dx01 = c101 - c001, dx10 = c110 - c010, dx11 = c111 - c011;
     const float c00 = A::fma(fx, dx00, c000), c01 = A::fma(fx, dx01, c001), c10 = A::fma(fx, dx10, c010),
                 c11 = A::fma(fx, dx11, c011);
@@ -135,15 +162,16 @@ __device__ __forceinline__ bool sdf_sample_t(const Grid* gp, f3 p, float& phi, f
     return true;
 }
 template <bool F>
-__device__ __forceinline__ bool sdf_sample(const Grid* gp, f3 p, float& phi, f3& grad) {
+__device__ __forceinline__ bool sdf_sample(const Grid& g, f3 p, float& phi, f3& grad) {
     f3 unused;
-    return sdf_sample_t<false, F>(gp, p, phi, grad, unused);
+    return sdf_sample_t<false, F>(g, p, phi, grad, unused);
 }
 
-// Contact of a vertex with one posed mesh at the probe (pos - P.pos) + dx (YB/sim/contact.cuh:94-110). Reads
-// only the pose fields it needs (the velocities only while meshes move).
+// Contact of a vertex with one posed mesh at the probe (pos - P.pos) + dx (YB/sim/contact.cuh:94-110). `g` is
+// the mesh's header from load_grid. Reads only the pose fields it needs (the velocities only while meshes move,
+// and only for a contact).
 template <bool F>
-__device__ __forceinline__ bool sdf_contact(const Params& Pm, const Grid* gp, const Pose* P, f3 p, f3 d, f3& n,
+__device__ __forceinline__ bool sdf_contact(const Params& Pm, const Grid& g, const Pose* P, f3 p, f3 d, f3& n,
                                             float& gap, f3& dRel) {
     using M = V<F>;
     const float4* s = reinterpret_cast<const float4*>(P);
@@ -152,14 +180,14 @@ __device__ __forceinline__ bool sdf_contact(const Params& Pm, const Grid* gp, co
     const f3 pp = mk3(a.x, a.y, a.z);
     float phi;
     f3 gr;
-    if (!sdf_sample<F>(gp, M::qrot(qconj(q), (p - pp) + d), phi, gr)) return false;
+    if (!sdf_sample<F>(g, M::qrot(qconj(q), (p - pp) + d), phi, gr)) return false;
     gap = phi - Pm.cRadius;
     if (gap >= 0.f) return false;
     const float gl = M::len(gr);
     if (!(gl > 0.f)) return false;
     n = M::qrot(q, M::div(gr, gl));
     dRel = d;
-    if (Pm.sdfMoving) {
+    if (Pm.sdfMoving) {  // read here: issued with the rest of the pose, they made 4096 contact ropes 2 % slower
         const float4 c = s[2], e = s[3];
         dRel = dRel - Pm.h * (mk3(b.w, c.x, c.y) + M::cross(mk3(c.z, c.w, e.x), p - pp));
     }
@@ -200,11 +228,12 @@ __device__ __forceinline__ void surfaces_at(const Params& P, const Grid* grids, 
             if (ncand < 0 && !probe_near_box(envPoses + m, p + d)) continue;
             f3 n, dRel;
             float gap;
-            if (!sdf_contact<F>(P, grids + m, envPoses + m, p, d, n, gap, dRel)) continue;
+            const Grid g = load_grid(grids + m);
+            if (!sdf_contact<F>(P, g, envPoses + m, p, d, n, gap, dRel)) continue;
             if constexpr (HITS) {
                 if (ncand >= 0) *hits |= 1u << c;  // listed candidates only (c < MAX_CAND)
             }
-            surface_contact<F>(P, n, gap, dRel, k, grids[m].muS, grids[m].muK, f, H);
+            surface_contact<F>(P, n, gap, dRel, k, g.muS, g.muK, f, H);
         }
     }
 }
@@ -337,7 +366,7 @@ __device__ __forceinline__ f3 surface_readout(const Params& P, const Dev& D, con
                 f3 n, dRel;
                 float gap;
                 if ((ncand >= 0 || probe_near_box(envPoses + m, p + d)) &&
-                    sdf_contact<F>(P, D.grids + m, envPoses + m, p, d, n, gap, dRel)) {
+                    sdf_contact<F>(P, load_grid(D.grids + m), envPoses + m, p, d, n, gap, dRel)) {
                     f3 fm = mk3(0.f, 0.f, 0.f);
                     surface_contact<F>(P, n, gap, dRel, k, D.grids[m].muS, D.grids[m].muK, fm, H);
                     fv = fv + fm;

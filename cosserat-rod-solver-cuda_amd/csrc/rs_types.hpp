@@ -18,15 +18,20 @@ constexpr int VERLET_CAP = 64; // skin-list entries per segment in the persisten
 enum { ERR_NONE = 0, ERR_MAX_COLLISIONS = 1, ERR_BVH_DEPTH = 2 };
 
 // A signed-distance grid: phi on an (nx, ny, nz) node lattice, C order (i slowest), at
-// origin + (i, j, k) * h in the object's frame (YarnBall.h:141).
-struct Grid {
-    const float* v;
-    int nx, ny, nz;
+// origin + (i, j, k) * h in the object's frame (YarnBall.h:141). Four 16-byte rows; a sample reads the first
+// three (load_grid), and each of them holds a field its bounds test needs, so all three loads are issued at
+// once ahead of that test.
+struct alignas(16) Grid {
+    const float* v;    // row 0: the node values and the first two sizes
+    int nx, ny;
+    int nz;            // row 1
     float ox, oy, oz;  // origin
-    float hx, hy, hz;  // far lattice corner origin + (n - 1) * h
-    float h, invH;
+    float invH, h;     // row 2: the spacing and the friction coefficients
     float muS, muK;
+    float hx, hy, hz;  // row 3: far lattice corner origin + (n - 1) * h (world boxes only)
+    float pad_;
 };
+static_assert(sizeof(Grid) == 64, "Grid is four 16-byte rows");
 
 // Pose of (rope, mesh) at the end of the current substep plus its velocity over it and the world
 // AABB of its lattice (YarnBall.h:147). 20 floats so a slot is five float4 loads.
