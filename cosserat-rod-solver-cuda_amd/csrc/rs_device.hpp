@@ -102,10 +102,12 @@ __device__ __forceinline__ void surface_contact(const Params& P, f3 n, float gap
     M::sadd_scaled(H, nn, -kt);
 }
 
-// The header of a lattice as the three 16-byte rows a sample reads, loaded together. Read field by field, the
-// compiler moved the loads of fields needed only past the short-circuit bounds test (the sizes, the node
-// pointer, the spacing, the friction) below it: a contact waited on three dependent rounds of header loads
-// before the node values and two after them. Here they are one round, issued with the pose.
+// The header of a lattice as the three 16-byte rows a sample reads. Read field by field, the compiler sank the
+// loads of the fields the short-circuit bounds test does not need first (the sizes, the node pointer, the
+// spacing, the friction) below it, one dependent round each: a contact waited on three rounds of header loads
+// before the node values and two after them. From the rows it issues most of them with the pose. Where it still
+// splits a row and sinks a field (the frame kernel: the first size and the node pointer), forcing that field
+// into the first round measured slower.
 __device__ __forceinline__ Grid load_grid(const Grid* gp) {
     const uint4* r = reinterpret_cast<const uint4*>(gp);
     const uint4 a = r[0], b = r[1], c = r[2];
