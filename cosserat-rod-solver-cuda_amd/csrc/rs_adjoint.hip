@@ -695,7 +695,10 @@ __device__ __forceinline__ int rc_pair_index(const RcLds& L, int t, bool act, ui
             n = MAX_COLL;
             L.nrev[t] = n;
         }
-        uint16_t* ri = n <= REV_SLOTS_OF(B) ? L.rq + t * REV_SLOTS_OF(B) : lists + (size_t)t * LIST_STRIDE + REV_IDX;
+        uint16_t* ri = lists + (size_t)t * LIST_STRIDE + REV_IDX;
+        if constexpr (REV_SLOTS_OF(B) > 0) {  // else the pointer stays global: global, not flat, memory operations
+            if (n <= REV_SLOTS_OF(B)) ri = L.rq + t * REV_SLOTS_OF(B);
+        }
         for (int a = 1; a < n; a++) {  // insertion sort (short lists)
             const uint16_t ii = ri[a];
             int b = a - 1;
@@ -713,6 +716,7 @@ __device__ __forceinline__ int rc_pair_index(const RcLds& L, int t, bool act, ui
 template <int B>
 __device__ __forceinline__ const uint16_t* rc_rev_list(const RcLds& L, const uint16_t* lists, int j, int& n) {
     n = L.nrev[j];
+    if constexpr (REV_SLOTS_OF(B) == 0) return lists + (size_t)j * LIST_STRIDE + REV_IDX;  // global loads
     return n <= REV_SLOTS_OF(B) ? L.rq + j * REV_SLOTS_OF(B) : lists + (size_t)j * LIST_STRIDE + REV_IDX;
 }
 
