@@ -391,22 +391,39 @@ __device__ __forceinline__ f3 surface_readout(const Params& P, const Dev& D, con
     return fv;
 }
 
+// Is probe p inside a pose's world box (rows e, f of the pose) grown by the vertex padding?
+__device__ __forceinline__ bool in_padded_box(const Params& P, f3 p, float padv, float4 e, float4 f) {
+    // moving meshes: + h (|v| + |w| |hi - lo| / 2), evaluated with the pose (substep_pose)
+    const float pm = P.sdfMoving ? padv + f.w : padv;
+    return (p.x >= e.y - pm) & (p.x <= f.x + pm) & (p.y >= e.z - pm) & (p.y <= f.y + pm) & (p.z >= e.w - pm) &
+           (p.z <= f.z + pm);
+}
 // Candidate meshes of a vertex for the next substep (YB/sim/contact.cuh:121-136). Returns the count
 // (-1 on overflow) and the packed ids.
 __device__ __forceinline__ int build_candidates(const Params& P, const Pose* envPoses, f3 p, float padv, uint64_t& packed) {
     int n = 0;
     packed = 0;
-    for (int m = 0; m < P.numSdf; m++) {
-        const float4* s = reinterpret_cast<const float4*>(envPoses + m);
-        const float4 e = s[3], f = s[4];
-        const f3 lo = mk3(e.y, e.z, e.w), hi = mk3(f.x, f.y, f.z);
-        // moving meshes: + h (|v| + |w| |hi - lo| / 2), evaluated with the pose (substep_pose)
-        const float pm = P.sdfMoving ? padv + f.w : padv;
-        if (p.x >= lo.x - pm && p.x <= hi.x + pm && p.y >= lo.y - pm && p.y <= hi.y + pm && p.z >= lo.z - pm &&
-            p.z <= hi.z + pm) {
-            if (n < MAX_CAND) packed |= (uint64_t)m << (6 * n);
-            n++;
+    const float4* s = reinterpret_cast<const float4*>(envPoses);  // 5 float4 per pose; the box is rows 3 and 4
+    int m = 0;
+    // Four meshes' boxes are loaded before any is tested: one round of loads for four meshes, not one each.
+    for (; m + 4 <= P.numSdf; m += 4) {
+        float4 e[4], f[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            e[u] = s[5 * (m + u) + 3];
+            f[u] = s[5 * (m + u) + 4];
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool in = in_padded_box(P, p, padv, e[u], f[u]);
+            packed |= (in && n < MAX_CAND) ? (uint64_t)(m + u) << (6 * n) : 0ull;
+            n += in ? 1 : 0;
+        }
+    }
+    for (; m < P.numSdf; m++) {
+        const bool in = in_padded_box(P, p, padv, s[5 * m + 3], s[5 * m + 4]);
+        packed |= (in && n < MAX_CAND) ? (uint64_t)m << (6 * n) : 0ull;
+        n += in ? 1 : 0;
     }
     return n <= MAX_CAND ? n : -1;
 }
